@@ -1,0 +1,92 @@
+// Device-side data layout shared by the HIP kernels and the host runtime.
+//
+// One handle holds an ensemble of disks (a single disk is an ensemble of one).
+// Every per-zone field is FP64, row-major [i][j] with j (azimuth) contiguous,
+// pooled over disks: disk d's field starts at DiskDev::so.
+//
+//   sig, vr, vp, phi        state + potential pools (sum of Nu*Nphi)
+//   spec                    half-spectrum pool, double2, per disk [Nc][Nu]
+//                           (column-major so the column transform reads
+//                           contiguous memory); holds S-hat after the row r2c
+//                           and V-hat after the column convolution (in place)
+//   khat                    real K-hat, per disk [Nc][Nu+1] (rows p=0..Nu of the
+//                           real-even M x Nc transform; row M-p equals row p)
+//   geom                    per disk GT_COUNT tables of Nu+1 doubles
+//   tw                      per transform length n: exp(-2 pi i t/n), t<n
+#ifndef DFK_INTERNAL_H
+#define DFK_INTERNAL_H
+
+#include <stdint.h>
+
+#define DFK_MAX_STAGES 16
+
+// geometry tables (index t -> geom[go + t*(Nu+1) + i])
+enum {
+  GT_R = 0,      // R_i (centres)
+  GT_RF,         // Rf_i (faces, Nu+1 entries)
+  GT_INVVOL,     // 1/Vol_i, Vol_i = R_i (Rf_{i+1}-Rf_i) dphi
+  GT_INVDRC,     // 1/(R_{i+1 clamped} - R_{i-1 clamped})   (source dP/dR)
+  GT_INVR,       // 1/R_i
+  GT_GSTAR,      // -GM*/(R_i R_i)
+  GT_INVDR,      // 1/(Rf_{i+1}-Rf_i)
+  GT_INVDL,      // 1/(R_i dphi)
+  GT_AF,         // Rf_i dphi  (radial face length, Nu+1 entries)
+  GT_AFACE,      // Rf_{i+1}-Rf_i (azimuthal face length)
+  GT_R32,        // R_i^{3/2}
+  GT_PHISC,      // pref / sqrt(R_i),  pref = -G du dphi/(M Nphi)
+  GT_DA,         // R_i (Rf_{i+1}-Rf_i) dphi  (C_m area weight)
+  GT_GRC,        // 1/(2 du R_i) interior, 1/(du R_i) at i=0, Nu-1 (self-gravity g_R)
+  GT_GPC,        // 1/(2 dphi R_i)                                  (self-gravity g_phi)
+  GT_COUNT
+};
+
+struct DiskDev {
+  int Nu, Nphi, Nc, M;
+  int gamma2;          // gamma == 2: P = K s^2, c_s^2 = 2 K s
+  int selfgrav;
+  double K, gamma, gmstar, G, eps;
+  double du, dphi;
+  double inv2dphi;     // 1/(2 dphi)
+  int64_t so;          // state offset (doubles)
+  int64_t spo;         // spectrum offset (double2)
+  int64_t kho;         // khat offset (doubles)
+  int64_t go;          // geometry offset (doubles)
+  int64_t two_row;     // twiddle offset for length Nphi (double2)
+  int64_t two_col;     // twiddle offset for length M (double2)
+  int nst_row, nst_col;
+  int rad_row[DFK_MAX_STAGES];
+  int rad_col[DFK_MAX_STAGES];
+  int ti, tj;          // update tile (rows x cols)
+  int ntile_i, ntile_j;
+  int64_t part;        // offset of this disk's tiles in the partial arrays
+  int pr;              // complex rows (= 2 real rows) per workgroup in row transforms
+  int cb;              // columns per workgroup in the column convolution
+  int cos_off;         // offset of cos(2 phi_j)/sin(2 phi_j) tables (Nphi each) in trig pool
+  int pad_;
+};
+
+// Per-disk loop control, device resident (diskmain.cpp:86-113 state).
+struct DiskCtl {
+  double t, dt, tnext;
+  double tlim, dtout, cfl;
+  double fin_re2, fin_im2;   // C_2 sums at loop end
+  long long step, max_steps;
+  int cur;                   // state buffer (0/1) holding the current state
+  int src, dst;              // buffers read / written by the step in flight
+  int done;                  // loop finished (t >= tlim, step cap, or failure)
+  int nsamples;              // samples recorded so far (ring buffer index)
+  int flags;                 // DFK_FLAG_*
+  int stop_on_nonfinite;
+};
+
+#define DFK_FLAG_NONFINITE 1
+
+// Partials written by the update / diag kernels per tile:
+//   pcfl[t]        max over the tile of (|vR|+cs)/dR + (|vphi|+cs)/(R dphi)
+//   pcm[4t+0..3]   sum Sigma dA, sum Sigma cos(2phi) dA, -sum Sigma sin(2phi) dA,
+//                  count of non-finite zones
+
+// Sample record: t and the raw C_0 / C_2 sums (host finishes |C|, arg).
+struct DfkSampleDev { double t, re0, re2, im2; };
+
+#endif

@@ -1,0 +1,902 @@
+// HIP kernels (gfx950, FP64) for the diskfft time-step loop.
+//
+// Per step, for every disk of the handle (a single disk is an ensemble of one):
+//   k_control   dt = cfl / max(...) from the previous step's per-tile maxima,
+//               output-time check and C_m sample, last-step clip, t += dt
+//               (diskmain.cpp:92-110, hydro.hpp:48-61)
+//   k_rows_r2c  S = R^{3/2} Sigma, row transforms of length Nphi (two real rows
+//               packed per complex transform), half spectrum -> spec[Nc][Nu]
+//   k_cols_conv per column: zero-padded length-2Nu transform, x real K-hat,
+//               inverse, keep rows < Nu                    (kalnajs.hpp:48-57)
+//   k_rows_c2r  row c2r + scale pref/sqrt(R) -> Phi       (kalnajs.hpp:57-62)
+//   k_update    forces (kalnajs.hpp:65-80) + source (hydro.hpp:73-96) + radial
+//               MUSCL sweep (:99-140) + azimuthal sweep (:143-179) + floor
+//               (:68-69) fused on an LDS tile; epilogue writes the next step's
+//               CFL maximum and the C_0/C_2 partial sums per tile.
+// All FFTs are this file's own LDS Stockham transforms (radix 4/2/3/5/7/11/13).
+#include <hip/hip_runtime.h>
+
+#include "dfk_internal.h"
+
+#define WG 256
+
+typedef long long i64;
+
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+
+// hydro.hpp:16-18
+__device__ __forceinline__ double vanleer(double a, double b) {
+  return (a * b > 0.0) ? 2.0 * a * b / (a + b) : 0.0;
+}
+
+__device__ __forceinline__ double pres(const DiskDev &D, double s) {
+  return D.gamma2 ? D.K * (s * s) : D.K * pow(s, D.gamma);
+}
+__device__ __forceinline__ double cs2f(const DiskDev &D, double s) {
+  return D.gamma2 ? (2.0 * D.K) * s : D.gamma * D.K * pow(s, D.gamma - 1.0);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+__device__ __forceinline__ int wrapj(int j, int n) {
+  if (j < 0) j += n;
+  if (j >= n) j -= n;
+  if (j >= n) j -= n;
+  return j;
+}
+
+__device__ __forceinline__ const double *gtab(const double *geom, const DiskDev &D, int t) {
+  return geom + D.go + (i64)t * (D.Nu + 1);
+}
+
+// ------------------------------------------------------------------ FFT
+// One Stockham stage of radix R over `batch` transforms of length n stored
+// contiguously in LDS (a -> b).  Ns = product of the radices already applied.
+// Twiddles: tw[t] = exp(-2 pi i t / n); SIGN > 0 uses the conjugate.
+template <int R, int SIGN>
+__device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2 *__restrict__ b,
+                                          int n, int batch, int Ns,
+                                          const double2 *__restrict__ tw) {
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  const int total = nb * batch;
+  const int rstep = n / R;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int p = e / nb;
+    const int j = e - p * nb;
+    const int k = j % Ns;
+    const double2 *src = a + p * n + j;
+    double2 *dst = b + p * n + (j / Ns) * Ns * R + k;
+    double2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      double2 x = src[m * nb];
+      if (m > 0 && k > 0) {
+        double2 w = tw[k * m * tstep];
+        if (SIGN > 0) w.y = -w.y;
+        x = cmul(x, w);
+      }
+      v[m] = x;
+    }
+    if (R == 2) {
+      dst[0] = cadd(v[0], v[1]);
+      dst[Ns] = csub(v[0], v[1]);
+    } else if (R == 4) {
+      const double sg = (SIGN < 0) ? 1.0 : -1.0;
+      double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+      double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+      double2 t = make_double2(sg * b1.y, -sg * b1.x);   // -i*b1 (forward), +i*b1 (inverse)
+      dst[0] = cadd(a0, b0);
+      dst[Ns] = cadd(a1, t);
+      dst[2 * Ns] = csub(a0, b0);
+      dst[3 * Ns] = csub(a1, t);
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        double2 s = v[0];
+#pragma unroll
+        for (int m = 1; m < R; ++m) {
+          double2 w = tw[((m * q) % R) * rstep];
+          if (SIGN > 0) w.y = -w.y;
+          double2 c = cmul(v[m], w);
+          s.x += c.x;
+          s.y += c.y;
+        }
+        dst[q * Ns] = s;
+      }
+    }
+  }
+}
+
+// Full transform: returns the LDS buffer holding the result (a or b).
+template <int SIGN>
+__device__ double2 *lds_fft(double2 *a, double2 *b, int n, int batch, int nst,
+                            const int *__restrict__ rad, const double2 *__restrict__ tw) {
+  int Ns = 1;
+  for (int s = 0; s < nst; ++s) {
+    const int r = rad[s];
+    switch (r) {
+      case 4: fft_stage<4, SIGN>(a, b, n, batch, Ns, tw); break;
+      case 2: fft_stage<2, SIGN>(a, b, n, batch, Ns, tw); break;
+      case 3: fft_stage<3, SIGN>(a, b, n, batch, Ns, tw); break;
+      case 5: fft_stage<5, SIGN>(a, b, n, batch, Ns, tw); break;
+      case 7: fft_stage<7, SIGN>(a, b, n, batch, Ns, tw); break;
+      case 11: fft_stage<11, SIGN>(a, b, n, batch, Ns, tw); break;
+      case 13: fft_stage<13, SIGN>(a, b, n, batch, Ns, tw); break;
+      default: break;
+    }
+    __syncthreads();
+    double2 *t = a;
+    a = b;
+    b = t;
+    Ns *= r;
+  }
+  return a;
+}
+
+// ------------------------------------------------------------------ reductions
+// Workgroup reduction of (max, 4 sums) in a fixed order (deterministic).
+__device__ void wg_reduce5(double &mx, double &s0, double &s1, double &s2, double &s3,
+                           double *red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(mx, off);
+    mx = (o > mx) ? o : mx;
+    s0 += __shfl_xor(s0, off);
+    s1 += __shfl_xor(s1, off);
+    s2 += __shfl_xor(s2, off);
+    s3 += __shfl_xor(s3, off);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[w * 5 + 0] = mx; red[w * 5 + 1] = s0; red[w * 5 + 2] = s1;
+    red[w * 5 + 3] = s2; red[w * 5 + 4] = s3;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    mx = red[0]; s0 = red[1]; s1 = red[2]; s2 = red[3]; s3 = red[4];
+    for (int q = 1; q < nw; ++q) {
+      double o = red[q * 5];
+      mx = (o > mx) ? o : mx;
+      s0 += red[q * 5 + 1]; s1 += red[q * 5 + 2]; s2 += red[q * 5 + 3]; s3 += red[q * 5 + 4];
+    }
+  }
+}
+
+// CFL rate of one zone (hydro.hpp:52-57); NaN rates never win the max.
+__device__ __forceinline__ double cfl_rate(const DiskDev &D, double s, double u, double w,
+                                           double invdR, double invdl) {
+  double cs = sqrt(cs2f(D, smax(s, 1e-12)));
+  return (fabs(u) + cs) * invdR + (fabs(w) + cs) * invdl;
+}
+
+__device__ __forceinline__ double nonfinite(double s, double u, double w) {
+  return (isfinite(s) && isfinite(u) && isfinite(w)) ? 0.0 : 1.0;
+}
+
+__device__ __forceinline__ const double *stp(const double *st, i64 ntot, int buf, int f,
+                                             const DiskDev &D) {
+  return st + (i64)(buf * 3 + f) * ntot + D.so;
+}
+__device__ __forceinline__ double *stpw(double *st, i64 ntot, int buf, int f, const DiskDev &D) {
+  return st + (i64)(buf * 3 + f) * ntot + D.so;
+}
+
+// ------------------------------------------------------------------ control
+// One workgroup per disk.  Reduces the per-tile partials of the state that
+// the previous update (or the initial diag) produced, then performs the top
+// of one iteration of diskmain.cpp:92-101.
+__global__ void __launch_bounds__(WG)
+k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
+          const double *__restrict__ pcfl, const double *__restrict__ pcm,
+          DfkSampleDev *__restrict__ samples, int cap) {
+  __shared__ double red[5 * (WG / 64)];
+  const int d = blockIdx.x;
+  const DiskDev &D = disks[d];
+  DiskCtl &C = ctl[d];
+  if (C.done) return;
+  const int nt = D.ntile_i * D.ntile_j;
+  double mx = 1e-30, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int t = threadIdx.x; t < nt; t += WG) {
+    double v = pcfl[D.part + t];
+    mx = (v > mx) ? v : mx;
+    s0 += pcm[4 * (D.part + t) + 0];
+    s1 += pcm[4 * (D.part + t) + 1];
+    s2 += pcm[4 * (D.part + t) + 2];
+    s3 += pcm[4 * (D.part + t) + 3];
+  }
+  wg_reduce5(mx, s0, s1, s2, s3, red);
+  if (threadIdx.x != 0) return;
+  if (s3 > 0.0) {
+    C.flags |= DFK_FLAG_NONFINITE;
+    if (C.stop_on_nonfinite) {
+      C.done = 1;
+      C.fin_re2 = s1;
+      C.fin_im2 = s2;
+      return;
+    }
+  }
+  const double t = C.t;
+  if (!(t < C.tlim) || (C.max_steps > 0 && C.step >= C.max_steps)) {
+    C.done = 1;
+    C.fin_re2 = s1;
+    C.fin_im2 = s2;
+    return;
+  }
+  if (t >= C.tnext - 1e-12) {
+    DfkSampleDev smp;
+    smp.t = t; smp.re0 = s0; smp.re2 = s1; smp.im2 = s2;
+    samples[(i64)d * cap + (C.nsamples % cap)] = smp;
+    C.nsamples += 1;
+    C.tnext += C.dtout;
+  }
+  double dt = C.cfl / mx;
+  if (t + dt > C.tlim) dt = C.tlim - t;
+  C.dt = dt;
+  C.t = t + dt;
+  C.step += 1;
+  C.src = C.cur;
+  C.dst = 1 - C.cur;
+  C.cur = C.dst;
+}
+
+// ------------------------------------------------------------------ diag
+// Per-tile CFL maximum and C_m sums of the current state (used before the
+// first step and by the timestep entry point).
+__global__ void __launch_bounds__(WG)
+k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+       const DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,
+       const double *__restrict__ geom, const double *__restrict__ trig,
+       double *__restrict__ pcfl, double *__restrict__ pcm) {
+  __shared__ double red[5 * (WG / 64)];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  const int cur = ctl[wm.x].cur;
+  const int Nu = D.Nu, Nphi = D.Nphi;
+  const int ti = wm.y / D.ntile_j, tj = wm.y - ti * D.ntile_j;
+  const int i0 = ti * D.ti, j0 = tj * D.tj;
+  const double *S = stp(st, ntot, cur, 0, D), *U = stp(st, ntot, cur, 1, D),
+               *W = stp(st, ntot, cur, 2, D);
+  const double *invdR = gtab(geom, D, GT_INVDR), *invdl = gtab(geom, D, GT_INVDL),
+               *dA = gtab(geom, D, GT_DA);
+  const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
+  double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int e = threadIdx.x; e < D.ti * D.tj; e += WG) {
+    const int li = e / D.tj, lj = e - li * D.tj;
+    const int i = i0 + li, j = j0 + lj;
+    if (i >= Nu || j >= Nphi) continue;
+    const i64 c = (i64)i * Nphi + j;
+    const double s = S[c], u = U[c], w = W[c];
+    const double r = cfl_rate(D, s, u, w, invdR[i], invdl[i]);
+    mx = (r > mx) ? r : mx;
+    const double sa = s * dA[i];
+    a0 += sa;
+    a1 += s * c2[j] * dA[i];
+    a2 -= s * s2[j] * dA[i];
+    a3 += nonfinite(s, u, w);
+  }
+  wg_reduce5(mx, a0, a1, a2, a3, red);
+  if (threadIdx.x == 0) {
+    const i64 t = D.part + wm.y;
+    pcfl[t] = mx;
+    pcm[4 * t + 0] = a0; pcm[4 * t + 1] = a1; pcm[4 * t + 2] = a2; pcm[4 * t + 3] = a3;
+  }
+}
+
+// ------------------------------------------------------------------ gravity
+// Row r2c: S = R^{3/2} Sigma for rows [2 pr g, 2 pr (g+1)) of the current state.
+__global__ void __launch_bounds__(WG)
+k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+           const DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,
+           const double *__restrict__ geom, const double2 *__restrict__ tw,
+           double2 *__restrict__ spec) {
+  extern __shared__ double2 lds2[];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  const DiskCtl &C = ctl[wm.x];
+  if (C.done) return;
+  const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
+  const int row0 = 2 * pr * wm.y;
+  double2 *A = lds2, *B = lds2 + pr * n;
+  const double *S = stp(st, ntot, C.src, 0, D);
+  const double *R32 = gtab(geom, D, GT_R32);
+  for (int e = threadIdx.x; e < pr * n; e += WG) {
+    const int p = e / n, j = e - p * n;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    double xa = 0.0, xb = 0.0;
+    if (ra < Nu) xa = R32[ra] * S[(i64)ra * n + j];
+    if (rb < Nu) xb = R32[rb] * S[(i64)rb * n + j];
+    A[e] = make_double2(xa, xb);
+  }
+  __syncthreads();
+  const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
+  double2 *out = spec + D.spo;
+  const int w2 = 2 * pr;
+  for (int e = threadIdx.x; e < w2 * Nc; e += WG) {
+    const int k = e / w2, rr = e - k * w2;
+    const int p = rr >> 1, isb = rr & 1;
+    const int row = row0 + rr;
+    if (row >= Nu) continue;
+    const double2 z = Z[p * n + k];
+    double2 y = Z[p * n + (k == 0 ? 0 : n - k)];
+    y.y = -y.y;
+    double2 X;
+    if (!isb) X = make_double2(0.5 * (z.x + y.x), 0.5 * (z.y + y.y));
+    else X = make_double2(0.5 * (z.y - y.y), -0.5 * (z.x - y.x));
+    out[(i64)k * Nu + row] = X;
+  }
+}
+
+// Column convolution: for columns [cb g, cb (g+1)): zero-padded forward
+// transform of length M, multiply by real K-hat, inverse, keep rows < Nu.
+__global__ void __launch_bounds__(WG)
+k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+            const DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
+            const double2 *__restrict__ tw, double2 *__restrict__ spec) {
+  extern __shared__ double2 lds2[];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
+  const int k0 = wm.y * cb;
+  double2 *A = lds2, *B = lds2 + cb * M;
+  double2 *io = spec + D.spo;
+  for (int e = threadIdx.x; e < cb * M; e += WG) {
+    const int c = e / M, p = e - c * M;
+    const int k = k0 + c;
+    double2 v = make_double2(0.0, 0.0);
+    if (k < Nc && p < Nu) v = io[(i64)k * Nu + p];
+    A[e] = v;
+  }
+  __syncthreads();
+  double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
+  double2 *Wb = (Z == A) ? B : A;
+  const double *kh = khat + D.kho;
+  for (int e = threadIdx.x; e < cb * M; e += WG) {
+    const int c = e / M, p = e - c * M;
+    const int k = k0 + c;
+    const int pp = (p <= Nu) ? p : M - p;
+    const double kv = (k < Nc) ? kh[(i64)k * (Nu + 1) + pp] : 0.0;
+    double2 z = Z[e];
+    z.x *= kv;
+    z.y *= kv;
+    Z[e] = z;
+  }
+  __syncthreads();
+  const double2 *Y = lds_fft<+1>(Z, Wb, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
+  for (int e = threadIdx.x; e < cb * Nu; e += WG) {
+    const int c = e / Nu, p = e - c * Nu;
+    const int k = k0 + c;
+    if (k < Nc) io[(i64)k * Nu + p] = Y[c * M + p];
+  }
+}
+
+// Row c2r (FFTW convention: imaginary parts of the DC and Nyquist bins are
+// ignored) and Phi = pref * V / sqrt(R).
+__global__ void __launch_bounds__(WG)
+k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+           const DiskCtl *__restrict__ ctl, const double2 *__restrict__ spec,
+           const double *__restrict__ geom, const double2 *__restrict__ tw,
+           double *__restrict__ phi) {
+  extern __shared__ double2 lds2[];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
+  const int row0 = 2 * pr * wm.y;
+  double2 *A = lds2, *B = lds2 + pr * n;
+  const double2 *in = spec + D.spo;
+  for (int e = threadIdx.x; e < pr * Nc; e += WG) {
+    const int k = e / pr, p = e - k * pr;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
+    if (ra < Nu) xa = in[(i64)k * Nu + ra];
+    if (rb < Nu) xb = in[(i64)k * Nu + rb];
+    if (k == 0 || 2 * k == n) { xa.y = 0.0; xb.y = 0.0; }
+    A[p * n + k] = make_double2(xa.x - xb.y, xa.y + xb.x);
+    if (k > 0 && n - k >= Nc) A[p * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
+  }
+  __syncthreads();
+  const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
+  const double *sc = gtab(geom, D, GT_PHISC);
+  double *P = phi + D.so;
+  for (int e = threadIdx.x; e < 2 * pr * n; e += WG) {
+    const int rr = e / n, j = e - rr * n;
+    const int row = row0 + rr;
+    if (row >= Nu) continue;
+    const double2 z = Z[(rr >> 1) * n + j];
+    const double v = (rr & 1) ? z.y : z.x;
+    P[(i64)row * n + j] = v * sc[row];
+  }
+}
+
+// ------------------------------------------------------------------ K-hat setup
+// Row r2c of the M x Nphi kernel table (kalnajs.hpp:105-114), all M rows, into
+// tmp[Nc][M]; then column transforms keep the real part of rows 0..Nu.
+__global__ void __launch_bounds__(WG)
+k_khat_rows(const DiskDev *__restrict__ D_, const double *__restrict__ ktab,
+            const double2 *__restrict__ tw, double2 *__restrict__ tmp) {
+  extern __shared__ double2 lds2[];
+  const DiskDev &D = *D_;
+  const int n = D.Nphi, M = D.M, pr = D.pr, Nc = D.Nc;
+  const int row0 = 2 * pr * blockIdx.x;
+  double2 *A = lds2, *B = lds2 + pr * n;
+  for (int e = threadIdx.x; e < pr * n; e += WG) {
+    const int p = e / n, j = e - p * n;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    A[e] = make_double2(ra < M ? ktab[(i64)ra * n + j] : 0.0, rb < M ? ktab[(i64)rb * n + j] : 0.0);
+  }
+  __syncthreads();
+  const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
+  const int w2 = 2 * pr;
+  for (int e = threadIdx.x; e < w2 * Nc; e += WG) {
+    const int k = e / w2, rr = e - k * w2;
+    const int p = rr >> 1, isb = rr & 1;
+    const int row = row0 + rr;
+    if (row >= M) continue;
+    const double2 z = Z[p * n + k];
+    double2 y = Z[p * n + (k == 0 ? 0 : n - k)];
+    y.y = -y.y;
+    double2 X;
+    if (!isb) X = make_double2(0.5 * (z.x + y.x), 0.5 * (z.y + y.y));
+    else X = make_double2(0.5 * (z.y - y.y), -0.5 * (z.x - y.x));
+    tmp[(i64)k * M + row] = X;
+  }
+}
+
+__global__ void __launch_bounds__(WG)
+k_khat_cols(const DiskDev *__restrict__ D_, const double2 *__restrict__ tw,
+            double2 *__restrict__ tmp, double *__restrict__ khat) {
+  extern __shared__ double2 lds2[];
+  const DiskDev &D = *D_;
+  const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
+  const int k0 = blockIdx.x * cb;
+  double2 *A = lds2, *B = lds2 + cb * M;
+  for (int e = threadIdx.x; e < cb * M; e += WG) {
+    const int c = e / M, p = e - c * M;
+    const int k = k0 + c;
+    A[e] = (k < Nc) ? tmp[(i64)k * M + p] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
+  for (int e = threadIdx.x; e < cb * M; e += WG) {
+    const int c = e / M, p = e - c * M;
+    const int k = k0 + c;
+    if (k >= Nc) continue;
+    tmp[(i64)k * M + p] = Z[e];                       // full complex K-hat (for checks)
+    if (p <= Nu) khat[D.kho + (i64)k * (Nu + 1) + p] = Z[e].x;
+  }
+}
+
+// ------------------------------------------------------------------ update
+// Fused step on an output tile of ti x tj zones.  LDS holds the post-source
+// state on (ti+4) x (tj+4) zones, the radial face fluxes, then the post-radial
+// state and the azimuthal face fluxes.  GARR: gravity from given g_R/g_phi
+// arrays (unit entry point) instead of the potential.
+template <bool GARR>
+__global__ void __launch_bounds__(WG)
+k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+         const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
+         const double *__restrict__ phi, const double *__restrict__ gRa,
+         const double *__restrict__ gpa, int has_gR, int has_gp,
+         const double *__restrict__ geom, const double *__restrict__ trig,
+         double *__restrict__ pcfl, double *__restrict__ pcm) {
+  extern __shared__ double lds[];
+  __shared__ double red[5 * (WG / 64)];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  const DiskCtl &C = ctl[wm.x];
+  if (C.done) return;
+  const double dt = C.dt;
+  const int Nu = D.Nu, Nphi = D.Nphi, TI = D.ti, TJ = D.tj;
+  const int ti = wm.y / D.ntile_j, tj = wm.y - ti * D.ntile_j;
+  const int i0 = ti * TI, j0 = tj * TJ;
+  const int SW = TJ + 4, SH = TI + 4;
+  double *sS = lds, *sU = sS + SH * SW, *sW = sU + SH * SW;
+  double *fM = sW + SH * SW, *fU = fM + (TI + 1) * SW, *fW = fU + (TI + 1) * SW;
+
+  const double *S = stp(st, ntot, C.src, 0, D), *U = stp(st, ntot, C.src, 1, D),
+               *W = stp(st, ntot, C.src, 2, D);
+  const double *Ph = phi + D.so;
+  const double *tR = gtab(geom, D, GT_R);
+  (void)tR;
+  const double *tInvdRc = gtab(geom, D, GT_INVDRC), *tInvR = gtab(geom, D, GT_INVR),
+               *tGstar = gtab(geom, D, GT_GSTAR), *tGrc = gtab(geom, D, GT_GRC),
+               *tGpc = gtab(geom, D, GT_GPC);
+  const bool sg = D.selfgrav != 0;
+
+  // ---- phase 1: source (+ forces) on rows i0-2..i0+TI+1, cols j0-2..j0+TJ+1
+  for (int e = threadIdx.x; e < SH * SW; e += WG) {
+    const int lr = e / SW, lc = e - lr * SW;
+    const int i = i0 - 2 + lr;
+    if (i < 0 || i >= Nu) continue;
+    const int j = wrapj(j0 - 2 + lc, Nphi);
+    const int jm = wrapj(j - 1, Nphi), jp = wrapj(j + 1, Nphi);
+    const int im = (i > 0) ? i - 1 : 0, ip = (i < Nu - 1) ? i + 1 : Nu - 1;
+    const i64 c = (i64)i * Nphi + j;
+    const double s = S[c], u = U[c], w = W[c];
+    const double dPdR = (pres(D, S[(i64)ip * Nphi + j]) - pres(D, S[(i64)im * Nphi + j])) * tInvdRc[i];
+    const double dPdphi = (pres(D, S[(i64)i * Nphi + jp]) - pres(D, S[(i64)i * Nphi + jm])) * D.inv2dphi;
+    double gR = tGstar[i];
+    double gp = 0.0;
+    if (GARR) {
+      if (has_gR) gR += gRa[D.so + c];
+      if (has_gp) gp += gpa[D.so + c];
+    } else if (sg) {
+      // kalnajs.hpp:71-78, one-sided at the radial edges
+      const int iu = (i < Nu - 1) ? i + 1 : i, id = (i > 0) ? i - 1 : i;
+      gR += -(Ph[(i64)iu * Nphi + j] - Ph[(i64)id * Nphi + j]) * tGrc[i];
+      gp += -(Ph[(i64)i * Nphi + jp] - Ph[(i64)i * Nphi + jm]) * tGpc[i];
+    }
+    const double rs = 1.0 / s;
+    const double invR = tInvR[i];
+    const double aR = -dPdR * rs + gR + w * w * invR;
+    const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
+    sS[e] = s;
+    sU[e] = u + dt * aR;
+    sW[e] = w + dt * ap;
+  }
+  __syncthreads();
+
+  // ---- phase 2: radial faces f = i0..i0+TI (between cells f-1 and f)
+  const double *tRf = gtab(geom, D, GT_RF), *tInvdR = gtab(geom, D, GT_INVDR),
+               *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL);
+  (void)tRf;
+  for (int e = threadIdx.x; e < (TI + 1) * SW; e += WG) {
+    const int lf = e / SW, lc = e - lf * SW;
+    const int f = i0 + lf;
+    double fm = 0.0, fu = 0.0, fw = 0.0;
+    if (f >= 1 && f <= Nu - 1) {
+      const int la = (lf + 1) * SW + lc;   // cell f-1
+      const int lb = la + SW;              // cell f
+      const double vf = 0.5 * (sU[la] + sU[lb]);
+      int lcell, cell;
+      double fac;
+      if (vf > 0) {
+        lcell = la; cell = f - 1;
+        fac = 0.5 * (1.0 - vf * dt * tInvdR[f - 1]);
+      } else {
+        lcell = lb; cell = f;
+        fac = -0.5 * (1.0 + vf * dt * tInvdR[f]);
+      }
+      double ss = sS[lcell], su = sU[lcell], sw = sW[lcell];
+      if (cell > 0 && cell < Nu - 1) {
+        ss += vanleer(sS[lcell + SW] - sS[lcell], sS[lcell] - sS[lcell - SW]) * fac;
+        su += vanleer(sU[lcell + SW] - sU[lcell], sU[lcell] - sU[lcell - SW]) * fac;
+        sw += vanleer(sW[lcell + SW] - sW[lcell], sW[lcell] - sW[lcell - SW]) * fac;
+      }
+      fm = vf * ss * tAf[f];
+      fu = fm * su;
+      fw = fm * sw;
+    }
+    fM[e] = fm; fU[e] = fu; fW[e] = fw;
+  }
+  __syncthreads();
+
+  // ---- phase 3: radial gather for rows i0..i0+TI-1 (in place into s*)
+  for (int e = threadIdx.x; e < TI * SW; e += WG) {
+    const int lr = e / SW, lc = e - lr * SW;
+    const int i = i0 + lr;
+    if (i >= Nu) continue;
+    const int l = (lr + 2) * SW + lc;
+    const double s = sS[l];
+    const double mR = s * sU[l], mP = s * sW[l];
+    const double iv = dt * tInvVol[i];
+    const double ns = s + (fM[e] - fM[e + SW]) * iv;
+    const double nmR = mR + (fU[e] - fU[e + SW]) * iv;
+    const double nmP = mP + (fW[e] - fW[e + SW]) * iv;
+    const double rn = 1.0 / smax(ns, 1e-10);
+    sS[l] = ns; sU[l] = nmR * rn; sW[l] = nmP * rn;
+  }
+  __syncthreads();
+
+  // ---- phase 4: azimuthal faces j = j0..j0+TJ (between j-1 and j), rows i0..i0+TI-1
+  const double *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE);
+  const int FW = TJ + 1;
+  for (int e = threadIdx.x; e < TI * FW; e += WG) {
+    const int lr = e / FW, lf = e - lr * FW;
+    const int i = i0 + lr;
+    double fm = 0.0, fu = 0.0, fw = 0.0;
+    if (i < Nu) {
+      const int la = (lr + 2) * SW + lf + 1;   // cell j-1
+      const int lb = la + 1;                   // cell j
+      const double vf = 0.5 * (sW[la] + sW[lb]);
+      int lcell;
+      double fac;
+      if (vf > 0) {
+        lcell = la;
+        fac = 0.5 * (1.0 - vf * dt * tInvdl[i]);
+      } else {
+        lcell = lb;
+        fac = -0.5 * (1.0 + vf * dt * tInvdl[i]);
+      }
+      const double ss = sS[lcell] + vanleer(sS[lcell + 1] - sS[lcell], sS[lcell] - sS[lcell - 1]) * fac;
+      const double su = sU[lcell] + vanleer(sU[lcell + 1] - sU[lcell], sU[lcell] - sU[lcell - 1]) * fac;
+      const double sw = sW[lcell] + vanleer(sW[lcell + 1] - sW[lcell], sW[lcell] - sW[lcell - 1]) * fac;
+      fm = vf * ss * tAface[i];
+      fu = fm * su;
+      fw = fm * sw;
+    }
+    fM[lr * FW + lf] = fm; fU[lr * FW + lf] = fu; fW[lr * FW + lf] = fw;
+  }
+  __syncthreads();
+
+  // ---- phase 5: azimuthal gather + floor, write, next-step CFL and C_m partials
+  double *So = stpw(st, ntot, C.dst, 0, D), *Uo = stpw(st, ntot, C.dst, 1, D),
+         *Wo = stpw(st, ntot, C.dst, 2, D);
+  const double *dA = gtab(geom, D, GT_DA);
+  const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
+  double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  for (int e = threadIdx.x; e < TI * TJ; e += WG) {
+    const int lr = e / TJ, lc = e - lr * TJ;
+    const int i = i0 + lr, j = j0 + lc;
+    if (i >= Nu || j >= Nphi) continue;
+    const int l = (lr + 2) * SW + lc + 2;
+    const int f = lr * FW + lc;
+    const double s = sS[l];
+    const double mR = s * sU[l], mP = s * sW[l];
+    const double iv = dt * tInvVol[i];
+    const double ns = s + (fM[f] - fM[f + 1]) * iv;
+    const double nmR = mR + (fU[f] - fU[f + 1]) * iv;
+    const double nmP = mP + (fW[f] - fW[f + 1]) * iv;
+    const double rn = 1.0 / smax(ns, 1e-10);
+    const double so = smax(ns, 1e-10);
+    const double uo = nmR * rn, wo = nmP * rn;
+    const i64 c = (i64)i * Nphi + j;
+    So[c] = so; Uo[c] = uo; Wo[c] = wo;
+    const double r = cfl_rate(D, so, uo, wo, tInvdR[i], tInvdl[i]);
+    mx = (r > mx) ? r : mx;
+    a0 += so * dA[i];
+    a1 += so * c2[j] * dA[i];
+    a2 -= so * s2[j] * dA[i];
+    a3 += nonfinite(so, uo, wo);
+  }
+  wg_reduce5(mx, a0, a1, a2, a3, red);
+  if (threadIdx.x == 0) {
+    const i64 t = D.part + wm.y;
+    pcfl[t] = mx;
+    pcm[4 * t + 0] = a0; pcm[4 * t + 1] = a1; pcm[4 * t + 2] = a2; pcm[4 * t + 3] = a3;
+  }
+}
+
+// ------------------------------------------------------------------ unit stages
+// Single-stage kernels that follow the reference's loop structure one to one
+// (source, radial faces/gather, azimuthal faces/gather, floor, forces).  They
+// back the per-stage entry points used to localise parity differences; the
+// time loop uses k_update.  One thread per zone (or face), state buffer `cur`.
+__global__ void k_forces(const DiskDev *__restrict__ Dp, const double *__restrict__ phi,
+                         const double *__restrict__ geom, double *__restrict__ gR,
+                         double *__restrict__ gp) {
+  const DiskDev &D = *Dp;
+  const i64 N = (i64)D.Nu * D.Nphi;
+  for (i64 c = blockIdx.x * (i64)blockDim.x + threadIdx.x; c < N; c += (i64)gridDim.x * blockDim.x) {
+    const int i = (int)(c / D.Nphi), j = (int)(c - (i64)i * D.Nphi);
+    const int jm = wrapj(j - 1, D.Nphi), jp = wrapj(j + 1, D.Nphi);
+    const int iu = (i < D.Nu - 1) ? i + 1 : i, id = (i > 0) ? i - 1 : i;
+    const double *P = phi + D.so;
+    gR[D.so + c] = -(P[(i64)iu * D.Nphi + j] - P[(i64)id * D.Nphi + j]) * gtab(geom, D, GT_GRC)[i];
+    gp[D.so + c] = -(P[(i64)i * D.Nphi + jp] - P[(i64)i * D.Nphi + jm]) * gtab(geom, D, GT_GPC)[i];
+  }
+}
+
+// source, in place on buffer `cur` (reads only own-cell velocities)
+__global__ void k_source(const DiskDev *__restrict__ Dp, double *__restrict__ st, i64 ntot, int cur,
+                         double dt, const double *__restrict__ gRa, const double *__restrict__ gpa,
+                         const double *__restrict__ geom) {
+  const DiskDev &D = *Dp;
+  const int Nu = D.Nu, Nphi = D.Nphi;
+  const i64 N = (i64)Nu * Nphi;
+  const double *S = stp(st, ntot, cur, 0, D);
+  double *U = stpw(st, ntot, cur, 1, D), *W = stpw(st, ntot, cur, 2, D);
+  for (i64 c = blockIdx.x * (i64)blockDim.x + threadIdx.x; c < N; c += (i64)gridDim.x * blockDim.x) {
+    const int i = (int)(c / Nphi), j = (int)(c - (i64)i * Nphi);
+    const int jm = wrapj(j - 1, Nphi), jp = wrapj(j + 1, Nphi);
+    const int im = (i > 0) ? i - 1 : 0, ip = (i < Nu - 1) ? i + 1 : Nu - 1;
+    const double s = S[c], u = U[c], w = W[c];
+    const double dPdR = (pres(D, S[(i64)ip * Nphi + j]) - pres(D, S[(i64)im * Nphi + j])) * gtab(geom, D, GT_INVDRC)[i];
+    const double dPdphi = (pres(D, S[(i64)i * Nphi + jp]) - pres(D, S[(i64)i * Nphi + jm])) * D.inv2dphi;
+    double gR = gtab(geom, D, GT_GSTAR)[i], gp = 0.0;
+    if (gRa) gR += gRa[D.so + c];
+    if (gpa) gp += gpa[D.so + c];
+    const double rs = 1.0 / s, invR = gtab(geom, D, GT_INVR)[i];
+    const double aR = -dPdR * rs + gR + w * w * invR;
+    const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
+    U[c] = u + dt * aR;
+    W[c] = w + dt * ap;
+  }
+}
+
+// face fluxes; dir 0 = radial (faces (Nu+1) x Nphi), 1 = azimuthal (Nu x Nphi)
+__global__ void k_faces(const DiskDev *__restrict__ Dp, const double *__restrict__ st, i64 ntot,
+                        int cur, double dt, int dir, const double *__restrict__ geom,
+                        double *__restrict__ F) {
+  const DiskDev &D = *Dp;
+  const int Nu = D.Nu, Nphi = D.Nphi;
+  const double *S = stp(st, ntot, cur, 0, D), *U = stp(st, ntot, cur, 1, D), *W = stp(st, ntot, cur, 2, D);
+  const i64 NF = (i64)(dir == 0 ? Nu + 1 : Nu) * Nphi;
+  double *Fm = F, *FU = F + NF, *FW = F + 2 * NF;
+  for (i64 c = blockIdx.x * (i64)blockDim.x + threadIdx.x; c < NF; c += (i64)gridDim.x * blockDim.x) {
+    const int f = (int)(c / Nphi), j = (int)(c - (i64)f * Nphi);
+    double fm = 0.0, fu = 0.0, fw = 0.0;
+    if (dir == 0) {
+      if (f >= 1 && f <= Nu - 1) {
+        const i64 a = (i64)(f - 1) * Nphi + j, b = (i64)f * Nphi + j;
+        const double vf = 0.5 * (U[a] + U[b]);
+        i64 q; int cell; double fac;
+        if (vf > 0) { q = a; cell = f - 1; fac = 0.5 * (1.0 - vf * dt * gtab(geom, D, GT_INVDR)[f - 1]); }
+        else { q = b; cell = f; fac = -0.5 * (1.0 + vf * dt * gtab(geom, D, GT_INVDR)[f]); }
+        double ss = S[q], su = U[q], sw = W[q];
+        if (cell > 0 && cell < Nu - 1) {
+          ss += vanleer(S[q + Nphi] - S[q], S[q] - S[q - Nphi]) * fac;
+          su += vanleer(U[q + Nphi] - U[q], U[q] - U[q - Nphi]) * fac;
+          sw += vanleer(W[q + Nphi] - W[q], W[q] - W[q - Nphi]) * fac;
+        }
+        fm = vf * ss * gtab(geom, D, GT_AF)[f];
+        fu = fm * su;
+        fw = fm * sw;
+      }
+    } else {
+      const int i = f;
+      const int jm = wrapj(j - 1, Nphi);
+      const i64 row = (i64)i * Nphi;
+      const double vf = 0.5 * (W[row + jm] + W[row + j]);
+      int cj; double fac;
+      if (vf > 0) { cj = jm; fac = 0.5 * (1.0 - vf * dt * gtab(geom, D, GT_INVDL)[i]); }
+      else { cj = j; fac = -0.5 * (1.0 + vf * dt * gtab(geom, D, GT_INVDL)[i]); }
+      const int cm = wrapj(cj - 1, Nphi), cp = wrapj(cj + 1, Nphi);
+      const double ss = S[row + cj] + vanleer(S[row + cp] - S[row + cj], S[row + cj] - S[row + cm]) * fac;
+      const double su = U[row + cj] + vanleer(U[row + cp] - U[row + cj], U[row + cj] - U[row + cm]) * fac;
+      const double sw = W[row + cj] + vanleer(W[row + cp] - W[row + cj], W[row + cj] - W[row + cm]) * fac;
+      fm = vf * ss * gtab(geom, D, GT_AFACE)[i];
+      fu = fm * su;
+      fw = fm * sw;
+    }
+    Fm[c] = fm; FU[c] = fu; FW[c] = fw;
+  }
+}
+
+__global__ void k_gather(const DiskDev *__restrict__ Dp, double *__restrict__ st, i64 ntot, int cur,
+                         double dt, int dir, const double *__restrict__ geom,
+                         const double *__restrict__ F) {
+  const DiskDev &D = *Dp;
+  const int Nu = D.Nu, Nphi = D.Nphi;
+  const i64 N = (i64)Nu * Nphi;
+  double *S = stpw(st, ntot, cur, 0, D), *U = stpw(st, ntot, cur, 1, D), *W = stpw(st, ntot, cur, 2, D);
+  const i64 NF = (i64)(dir == 0 ? Nu + 1 : Nu) * Nphi;
+  const double *Fm = F, *FU = F + NF, *FW = F + 2 * NF;
+  for (i64 c = blockIdx.x * (i64)blockDim.x + threadIdx.x; c < N; c += (i64)gridDim.x * blockDim.x) {
+    const int i = (int)(c / Nphi), j = (int)(c - (i64)i * Nphi);
+    const i64 f0 = c;
+    const i64 f1 = (dir == 0) ? c + Nphi : (i64)i * Nphi + wrapj(j + 1, Nphi);
+    const double s = S[c];
+    const double mR = s * U[c], mP = s * W[c];
+    const double iv = dt * gtab(geom, D, GT_INVVOL)[i];
+    const double ns = s + (Fm[f0] - Fm[f1]) * iv;
+    const double nmR = mR + (FU[f0] - FU[f1]) * iv;
+    const double nmP = mP + (FW[f0] - FW[f1]) * iv;
+    const double rn = 1.0 / smax(ns, 1e-10);
+    S[c] = ns; U[c] = nmR * rn; W[c] = nmP * rn;
+  }
+}
+
+__global__ void k_floor(const DiskDev *__restrict__ Dp, double *__restrict__ st, i64 ntot, int cur) {
+  const DiskDev &D = *Dp;
+  const i64 N = (i64)D.Nu * D.Nphi;
+  double *S = stpw(st, ntot, cur, 0, D);
+  for (i64 c = blockIdx.x * (i64)blockDim.x + threadIdx.x; c < N; c += (i64)gridDim.x * blockDim.x)
+    S[c] = smax(S[c], 1e-10);
+}
+
+// ------------------------------------------------------------------ host launchers
+// (plain C++ wrappers so the runtime file stays free of kernel syntax)
+#include "dfk_launch.h"
+
+#define CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
+
+hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
+                              const double *pcfl, const double *pcm, DfkSampleDev *samples, int cap) {
+  if (ndisk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_control, dim3(ndisk), dim3(WG), 0, s, disks, ctl, pcfl, pcm, samples, cap);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const int2 *map, int base,
+                           const DiskCtl *ctl, const double *st, i64 ntot, const double *geom,
+                           const double *trig, double *pcfl, double *pcm) {
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_diag, dim3(nblk), dim3(WG), 0, s, disks, map, base, ctl, st, ntot, geom, trig,
+                     pcfl, pcm);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_r2c(hipStream_t s, int nblk, size_t lds, const DiskDev *disks, const int2 *map,
+                          int base, const DiskCtl *ctl, const double *st, i64 ntot,
+                          const double *geom, const double2 *tw, double2 *spec) {
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_r2c, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, st, ntot, geom,
+                     tw, spec);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_conv(hipStream_t s, int nblk, size_t lds, const DiskDev *disks, const int2 *map,
+                           int base, const DiskCtl *ctl, const double *khat, const double2 *tw,
+                           double2 *spec) {
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cols_conv, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, khat, tw, spec);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_c2r(hipStream_t s, int nblk, size_t lds, const DiskDev *disks, const int2 *map,
+                          int base, const DiskCtl *ctl, const double2 *spec, const double *geom,
+                          const double2 *tw, double *phi) {
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_c2r, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, spec, geom, tw,
+                     phi);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_update(hipStream_t s, int nblk, size_t lds, int garr, const DiskDev *disks,
+                             const int2 *map, int base, const DiskCtl *ctl, double *st, i64 ntot,
+                             const double *phi, const double *gR, const double *gp, int has_gR,
+                             int has_gp, const double *geom, const double *trig, double *pcfl,
+                             double *pcm) {
+  if (nblk <= 0) return hipSuccess;
+  if (garr)
+    hipLaunchKernelGGL(k_update<true>, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, st, ntot,
+                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+  else
+    hipLaunchKernelGGL(k_update<false>, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, st, ntot,
+                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_khat(hipStream_t s, const DiskDev *D, int nrow_blk, int ncol_blk,
+                           size_t lds_rows, size_t lds_cols, const double *ktab, const double2 *tw,
+                           double2 *tmp, double *khat) {
+  hipLaunchKernelGGL(k_khat_rows, dim3(nrow_blk), dim3(WG), lds_rows, s, D, ktab, tw, tmp);
+  CHK(hipGetLastError());
+  hipLaunchKernelGGL(k_khat_cols, dim3(ncol_blk), dim3(WG), lds_cols, s, D, tw, tmp, khat);
+  return hipGetLastError();
+}
+
+static inline int nblocks(i64 n) {
+  i64 b = (n + WG - 1) / WG;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+hipError_t dfk_launch_forces(hipStream_t s, const DiskDev *D, i64 N, const double *phi,
+                             const double *geom, double *gR, double *gp) {
+  hipLaunchKernelGGL(k_forces, dim3(nblocks(N)), dim3(WG), 0, s, D, phi, geom, gR, gp);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_stage(hipStream_t s, const DiskDev *D, i64 N, int stage, double *st, i64 ntot,
+                            int cur, double dt, const double *gR, const double *gp,
+                            const double *geom, double *F) {
+  switch (stage) {
+    case 1:
+      hipLaunchKernelGGL(k_source, dim3(nblocks(N)), dim3(WG), 0, s, D, st, ntot, cur, dt, gR, gp, geom);
+      break;
+    case 2:
+    case 3: {
+      const int dir = stage - 2;
+      hipLaunchKernelGGL(k_faces, dim3(nblocks(N + N)), dim3(WG), 0, s, D, st, ntot, cur, dt, dir, geom, F);
+      CHK(hipGetLastError());
+      hipLaunchKernelGGL(k_gather, dim3(nblocks(N)), dim3(WG), 0, s, D, st, ntot, cur, dt, dir, geom, F);
+      break;
+    }
+    case 4:
+      hipLaunchKernelGGL(k_floor, dim3(nblocks(N)), dim3(WG), 0, s, D, st, ntot, cur);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}

@@ -1,0 +1,38 @@
+// Host-callable launchers of the HIP kernels in dfk_kernels.hip.
+#ifndef DFK_LAUNCH_H
+#define DFK_LAUNCH_H
+
+#include <hip/hip_runtime.h>
+
+#include "dfk_internal.h"
+
+hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
+                              const double *pcfl, const double *pcm, DfkSampleDev *samples,
+                              int cap);
+hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const int2 *map,
+                           int base, const DiskCtl *ctl, const double *st, long long ntot,
+                           const double *geom, const double *trig, double *pcfl, double *pcm);
+hipError_t dfk_launch_r2c(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+                          const int2 *map, int base, const DiskCtl *ctl, const double *st,
+                          long long ntot, const double *geom, const double2 *tw, double2 *spec);
+hipError_t dfk_launch_conv(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+                           const int2 *map, int base, const DiskCtl *ctl, const double *khat,
+                           const double2 *tw, double2 *spec);
+hipError_t dfk_launch_c2r(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+                          const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
+                          const double *geom, const double2 *tw, double *phi);
+hipError_t dfk_launch_update(hipStream_t s, int nblk, size_t lds, int garr, const DiskDev *disks,
+                             const int2 *map, int base, const DiskCtl *ctl, double *st,
+                             long long ntot, const double *phi, const double *gR,
+                             const double *gp, int has_gR, int has_gp, const double *geom,
+                             const double *trig, double *pcfl, double *pcm);
+hipError_t dfk_launch_khat(hipStream_t s, const DiskDev *D, int nrow_blk, int ncol_blk,
+                           size_t lds_rows, size_t lds_cols, const double *ktab,
+                           const double2 *tw, double2 *tmp, double *khat);
+hipError_t dfk_launch_forces(hipStream_t s, const DiskDev *D, long long N, const double *phi,
+                             const double *geom, double *gR, double *gp);
+hipError_t dfk_launch_stage(hipStream_t s, const DiskDev *D, long long N, int stage, double *st,
+                            long long ntot, int cur, double dt, const double *gR,
+                            const double *gp, const double *geom, double *F);
+
+#endif

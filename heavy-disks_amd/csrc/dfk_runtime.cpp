@@ -1,0 +1,907 @@
+// Host runtime of libdiskfft: the C ABI of include/diskfft.h over the HIP
+// kernels of dfk_kernels.hip.
+//
+// Set-up (geometry tables, C_m trig tables, FFT twiddles, the Kalnajs kernel
+// table) is computed on the host with libm using the reference's formulas,
+// uploaded once, and stays resident; the K-hat transform itself runs on the
+// device with the same LDS FFT as the time loop.  The time loop is device
+// resident: dt, t, the output schedule and the C_m samples live in DiskCtl,
+// and dfk_advance launches a captured HIP graph of whole steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/diskfft.h"
+#include "dfk_internal.h"
+#include "dfk_launch.h"
+
+typedef long long i64;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      set_err("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return DFK_EHIP;                                                             \
+    }                                                                              \
+  } while (0)
+
+#define TRY(x)                  \
+  do {                          \
+    int r_ = (x);               \
+    if (r_ != DFK_OK) return r_; \
+  } while (0)
+
+// Radix factorisation used by both transforms (4s first, then 2, then 3, 5, 7, 11, 13).
+bool factor(int n, int *rad, int *nst) {
+  int m = n, k = 0;
+  while (m % 4 == 0) { rad[k++] = 4; m /= 4; }
+  while (m % 2 == 0) { rad[k++] = 2; m /= 2; }
+  for (int q : {3, 5, 7, 11, 13})
+    while (m % q == 0) {
+      if (k >= DFK_MAX_STAGES) return false;
+      rad[k++] = q;
+      m /= q;
+    }
+  *nst = k;
+  return m == 1 && k <= DFK_MAX_STAGES;
+}
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+struct dfk_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nd = 0;
+  std::vector<dfk_disk_params> P;
+  std::vector<DiskDev> hD;
+  std::vector<int> hcur;        // host mirror of DiskCtl::cur
+  std::vector<double> mD;       // per-disk |C_m| normaliser of the current run
+  DiskDev *dD = nullptr;
+  i64 ntot = 0, nspec = 0, nkhat = 0, ngeom = 0, ntw = 0, ntrig = 0, maxN = 0, maxF = 0;
+  double *st = nullptr;         // 2 buffers x 3 fields x ntot
+  double *phi = nullptr, *gR = nullptr, *gp = nullptr, *F = nullptr;
+  double2 *spec = nullptr, *tw = nullptr;
+  double *khat = nullptr, *geom = nullptr, *trig = nullptr;
+  // layout (rebuilt when shapes change)
+  bool layout_ok = false;
+  double *pcfl = nullptr, *pcm = nullptr;
+  i64 ntiles = 0;
+  int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr;
+  std::vector<int> upd0, r2c0, conv0;   // per-disk first block (size nd+1)
+  size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0;
+  DiskCtl *ctl = nullptr;
+  // run
+  bool run_ready = false;
+  int cap = 0;
+  DfkSampleDev *samples = nullptr;
+  std::vector<int> drained;
+  std::vector<std::vector<dfk_sample>> hsamples;
+  std::map<int, hipGraphExec_t> graphs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int chunk = 128;               // loop iterations per captured graph
+};
+
+namespace {
+
+int check_disk(dfk_handle *h, int d) {
+  if (!h) { set_err("null handle"); return DFK_EINVAL; }
+  if (d < 0 || d >= h->nd) { set_err("disk index %d out of range [0,%d)", d, h->nd); return DFK_EINVAL; }
+  return DFK_OK;
+}
+
+void drop_graphs(dfk_handle *h) {
+  for (auto &g : h->graphs) hipGraphExecDestroy(g.second);
+  h->graphs.clear();
+}
+
+template <class T>
+int dalloc(T **p, i64 n) {
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    set_err("hipMalloc(%lld bytes) failed: %s", (i64)(sizeof(T) * n), hipGetErrorString(e));
+    return DFK_ENOMEM;
+  }
+  return DFK_OK;
+}
+
+// Layout of tiles, transform batches and workgroup maps for the current shapes.
+int build_layout(dfk_handle *h) {
+  HIPCHK(hipSetDevice(h->device));
+  if (h->layout_ok) return DFK_OK;
+  drop_graphs(h);
+  h->run_ready = false;
+  std::vector<int2> mu, mr, mc;
+  h->upd0.assign(h->nd + 1, 0);
+  h->r2c0.assign(h->nd + 1, 0);
+  h->conv0.assign(h->nd + 1, 0);
+  i64 part = 0;
+  size_t lu = 0, lr = 0, lc = 0;
+  for (int d = 0; d < h->nd; ++d) {
+    DiskDev &D = h->hD[d];
+    D.tj = std::max(1, std::min(D.tj, D.Nphi));
+    D.ti = std::max(1, std::min(D.ti, D.Nu));
+    D.ntile_i = ceil_div(D.Nu, D.ti);
+    D.ntile_j = ceil_div(D.Nphi, D.tj);
+    D.part = part;
+    const int nt = D.ntile_i * D.ntile_j;
+    part += nt;
+    h->upd0[d] = (int)mu.size();
+    for (int t = 0; t < nt; ++t) mu.push_back(make_int2(d, t));
+    size_t l = (size_t)(3 * (D.ti + 4) * (D.tj + 4) + 3 * (D.ti + 1) * (D.tj + 4)) * sizeof(double);
+    lu = std::max(lu, l);
+    D.pr = std::max(1, std::min(D.pr, ceil_div(D.Nu, 2)));
+    D.cb = std::max(1, std::min(D.cb, D.Nc));
+    h->r2c0[d] = (int)mr.size();
+    h->conv0[d] = (int)mc.size();
+    if (D.selfgrav) {
+      for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
+      for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) mc.push_back(make_int2(d, g));
+      lr = std::max(lr, (size_t)2 * D.pr * D.Nphi * sizeof(double2));
+      lc = std::max(lc, (size_t)2 * D.cb * D.M * sizeof(double2));
+    }
+  }
+  h->upd0[h->nd] = (int)mu.size();
+  h->r2c0[h->nd] = (int)mr.size();
+  h->conv0[h->nd] = (int)mc.size();
+  const size_t kMaxLds = 64 * 1024;
+  if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds) {
+    set_err("tile/batch shapes need %zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, kMaxLds);
+    return DFK_EINVAL;
+  }
+  h->lds_upd = lu;
+  h->lds_r2c = lr;
+  h->lds_conv = lc;
+  hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv);
+  h->pcfl = h->pcm = nullptr;
+  h->map_upd = h->map_r2c = h->map_conv = nullptr;
+  h->ntiles = part;
+  TRY(dalloc(&h->pcfl, part));
+  TRY(dalloc(&h->pcm, 4 * part));
+  TRY(dalloc(&h->map_upd, (i64)mu.size()));
+  TRY(dalloc(&h->map_r2c, (i64)mr.size()));
+  TRY(dalloc(&h->map_conv, (i64)mc.size()));
+  HIPCHK(hipMemcpy(h->map_upd, mu.data(), sizeof(int2) * mu.size(), hipMemcpyHostToDevice));
+  if (!mr.empty()) HIPCHK(hipMemcpy(h->map_r2c, mr.data(), sizeof(int2) * mr.size(), hipMemcpyHostToDevice));
+  if (!mc.empty()) HIPCHK(hipMemcpy(h->map_conv, mc.data(), sizeof(int2) * mc.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->dD, h->hD.data(), sizeof(DiskDev) * h->nd, hipMemcpyHostToDevice));
+  h->layout_ok = true;
+  return DFK_OK;
+}
+
+// Host view of one disk's control block.
+int read_ctl(dfk_handle *h, int d, DiskCtl *c) {
+  HIPCHK(hipMemcpy(c, h->ctl + d, sizeof(DiskCtl), hipMemcpyDeviceToHost));
+  return DFK_OK;
+}
+int write_ctl(dfk_handle *h, int d, const DiskCtl *c) {
+  HIPCHK(hipMemcpy(h->ctl + d, c, sizeof(DiskCtl), hipMemcpyHostToDevice));
+  return DFK_OK;
+}
+
+// Control block for a unit call: not done, src/dst as given, dt as given.
+int unit_ctl(dfk_handle *h, int d, int src, int dst, double dt) {
+  DiskCtl c;
+  memset(&c, 0, sizeof c);
+  c.cur = h->hcur[d];
+  c.src = src;
+  c.dst = dst;
+  c.dt = dt;
+  c.done = 0;
+  h->run_ready = false;
+  return write_ctl(h, d, &c);
+}
+
+double *field(dfk_handle *h, int buf, int f, int d) {
+  return h->st + (i64)(buf * 3 + f) * h->ntot + h->hD[d].so;
+}
+
+// Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk.
+int launch_gravity_disk(dfk_handle *h, int d) {
+  const int nr = h->r2c0[d + 1] - h->r2c0[d], nc = h->conv0[d + 1] - h->conv0[d];
+  HIPCHK(dfk_launch_r2c(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->st,
+                        h->ntot, h->geom, h->tw, h->spec));
+  HIPCHK(dfk_launch_conv(h->stream, nc, h->lds_conv, h->dD, h->map_conv, h->conv0[d], h->ctl,
+                         h->khat, h->tw, h->spec));
+  HIPCHK(dfk_launch_c2r(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
+                        h->geom, h->tw, h->phi));
+  return DFK_OK;
+}
+
+// One whole loop iteration for every disk (captured into the graph).
+int launch_step_all(dfk_handle *h) {
+  const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
+  HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap));
+  if (nr > 0) {
+    HIPCHK(dfk_launch_r2c(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
+                          h->geom, h->tw, h->spec));
+    HIPCHK(dfk_launch_conv(h->stream, nc, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
+                           h->tw, h->spec));
+    HIPCHK(dfk_launch_c2r(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
+                          h->geom, h->tw, h->phi));
+  }
+  HIPCHK(dfk_launch_update(h->stream, nu, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
+                           h->ntot, h->phi, nullptr, nullptr, 0, 0, h->geom, h->trig, h->pcfl,
+                           h->pcm));
+  return DFK_OK;
+}
+
+int get_graph(dfk_handle *h, int nsteps, hipGraphExec_t *out) {
+  auto it = h->graphs.find(nsteps);
+  if (it != h->graphs.end()) { *out = it->second; return DFK_OK; }
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = DFK_OK;
+  for (int s = 0; s < nsteps && rc == DFK_OK; ++s) rc = launch_step_all(h);
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  if (rc != DFK_OK) return rc;
+  if (e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); return DFK_EHIP; }
+  hipGraphExec_t ex;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) { set_err("graph instantiate failed: %s", hipGetErrorString(e)); return DFK_EHIP; }
+  h->graphs[nsteps] = ex;
+  *out = ex;
+  return DFK_OK;
+}
+
+// Copies the samples produced since the last drain into the host buffers.
+int drain(dfk_handle *h) {
+  std::vector<DiskCtl> c(h->nd);
+  HIPCHK(hipMemcpy(c.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
+  for (int d = 0; d < h->nd; ++d) {
+    h->hcur[d] = c[d].cur;
+    const int n = c[d].nsamples, from = h->drained[d];
+    if (n - from > h->cap) {
+      set_err("sample ring overflow on disk %d (%d new, capacity %d)", d, n - from, h->cap);
+      return DFK_ESTATE;
+    }
+    for (int q = from; q < n; ++q) {
+      DfkSampleDev s;
+      HIPCHK(hipMemcpy(&s, h->samples + (i64)d * h->cap + (q % h->cap), sizeof s, hipMemcpyDeviceToHost));
+      dfk_sample o;
+      const double mD = h->mD[d];
+      o.t = s.t;
+      o.c0 = std::sqrt(s.re0 * s.re0 + 0.0 * 0.0) / mD;    // diskmain.cpp:82 with im = 0 for m = 0
+      o.c2 = std::sqrt(s.re2 * s.re2 + s.im2 * s.im2) / mD;
+      o.ph2 = std::atan2(s.im2, s.re2);
+      h->hsamples[d].push_back(o);
+    }
+    h->drained[d] = n;
+  }
+  return DFK_OK;
+}
+
+// Per-tile partials of disk d's current state, reduced on the host in tile order.
+int diag_disk(dfk_handle *h, int d, double *mx, double sums[4]) {
+  TRY(build_layout(h));
+  TRY(unit_ctl(h, d, h->hcur[d], 1 - h->hcur[d], 0.0));
+  const int nb = h->upd0[d + 1] - h->upd0[d];
+  HIPCHK(dfk_launch_diag(h->stream, nb, h->dD, h->map_upd, h->upd0[d], h->ctl, h->st, h->ntot,
+                         h->geom, h->trig, h->pcfl, h->pcm));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const DiskDev &D = h->hD[d];
+  std::vector<double> pc(nb), pm(4 * (size_t)nb);
+  HIPCHK(hipMemcpy(pc.data(), h->pcfl + D.part, sizeof(double) * nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(pm.data(), h->pcm + 4 * D.part, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost));
+  double m = 1e-30;
+  double s[4] = {0, 0, 0, 0};
+  for (int t = 0; t < nb; ++t) {
+    if (pc[t] > m) m = pc[t];
+    for (int q = 0; q < 4; ++q) s[q] += pm[4 * t + q];
+  }
+  *mx = m;
+  for (int q = 0; q < 4; ++q) sums[q] = s[q];
+  return DFK_OK;
+}
+
+// Geometry / trig / twiddle tables and the K table on the host (libm).
+void host_tables(const DiskDev &D, const dfk_disk_params &p, std::vector<double> &geom,
+                 std::vector<double> &trig) {
+  const int Nu = D.Nu, Nphi = D.Nphi, L = Nu + 1;
+  geom.assign((size_t)GT_COUNT * L, 0.0);
+  auto T = [&](int t, int i) -> double & { return geom[(size_t)t * L + i]; };
+  const double umin = std::log(p.Rin);
+  const double du = D.du, dphi = D.dphi;
+  std::vector<double> R(Nu), Rf(Nu + 1);
+  for (int i = 0; i <= Nu; ++i) Rf[i] = std::exp(umin + i * du);              // hydro.hpp:34
+  for (int i = 0; i < Nu; ++i) R[i] = std::exp(umin + (i + 0.5) * du);        // hydro.hpp:36
+  const double pref = -p.G * du * dphi / (static_cast<double>(D.M) * Nphi);   // kalnajs.hpp:58
+  for (int i = 0; i <= Nu; ++i) {
+    T(GT_RF, i) = Rf[i];
+    T(GT_AF, i) = Rf[i] * dphi;                                                // hydro.hpp:113
+  }
+  for (int i = 0; i < Nu; ++i) {
+    const int im = std::max(i - 1, 0), ip = std::min(i + 1, Nu - 1);
+    const double vol = R[i] * (Rf[i + 1] - Rf[i]) * dphi;                       // hydro.hpp:37
+    const double sq = std::sqrt(R[i]);
+    T(GT_R, i) = R[i];
+    T(GT_INVVOL, i) = 1.0 / vol;
+    T(GT_INVDRC, i) = 1.0 / (R[ip] - R[im]);                                   // hydro.hpp:77-78
+    T(GT_INVR, i) = 1.0 / R[i];
+    T(GT_GSTAR, i) = -p.gmstar / (R[i] * R[i]);                                 // hydro.hpp:85
+    T(GT_INVDR, i) = 1.0 / (Rf[i + 1] - Rf[i]);                                 // hydro.hpp:52
+    T(GT_INVDL, i) = 1.0 / (R[i] * dphi);                                       // hydro.hpp:56,152
+    T(GT_AFACE, i) = Rf[i + 1] - Rf[i];                                         // hydro.hpp:152
+    T(GT_R32, i) = R[i] * sq;                                                   // kalnajs.hpp:33
+    T(GT_PHISC, i) = pref / sq;                                                 // kalnajs.hpp:62
+    T(GT_DA, i) = R[i] * (Rf[i + 1] - Rf[i]) * dphi;                            // diskmain.cpp:75
+    T(GT_GRC, i) = (i == 0 || i == Nu - 1) ? 1.0 / (du * R[i]) : 1.0 / (2.0 * du * R[i]);  // kalnajs.hpp:74-77
+    T(GT_GPC, i) = 1.0 / (2.0 * dphi * R[i]);                                   // kalnajs.hpp:72,78
+  }
+  trig.assign((size_t)2 * Nphi, 0.0);
+  for (int j = 0; j < Nphi; ++j) {
+    const double phi = (j + 0.5) * dphi;                                        // diskmain.cpp:77
+    trig[j] = std::cos(2 * phi);
+    trig[Nphi + j] = std::sin(2 * phi);
+  }
+}
+
+void twiddles(int n, std::vector<double2> &out) {
+  out.resize(n);
+  for (int t = 0; t < n; ++t) {
+    const double a = 2.0 * M_PI * (double)t / (double)n;
+    out[t] = make_double2(std::cos(a), -std::sin(a));
+  }
+}
+
+// K(p,q) table of kalnajs.hpp:106-113, M x Nphi.
+void ktable(const DiskDev &D, std::vector<double> &k) {
+  const int M = D.M, Nphi = D.Nphi;
+  k.resize((size_t)M * Nphi);
+  for (int p = 0; p < M; ++p) {
+    const int ioff = (p <= M / 2) ? p : p - M;
+    const double Du = ioff * D.du;
+    for (int q = 0; q < Nphi; ++q) {
+      const double Dphi = q * D.dphi;
+      k[(size_t)p * Nphi + q] = 1.0 / std::sqrt(2.0 * std::cosh(Du) - 2.0 * std::cos(Dphi) + D.eps);
+    }
+  }
+}
+
+// Runs the K-hat transform of disk d; keeps the real rows 0..Nu in h->khat and,
+// if full != nullptr, returns the complex [Nc][M] transform.
+int build_khat(dfk_handle *h, int d, std::vector<double2> *full) {
+  DiskDev D = h->hD[d];
+  // batch shapes for the M-row table: same pr / cb rules as the loop
+  D.pr = std::max(1, std::min(1024 / std::max(D.Nphi, 1), ceil_div(D.M, 2)));
+  D.cb = std::max(1, std::min(1024 / std::max(D.M, 1), D.Nc));
+  std::vector<double> kt;
+  ktable(D, kt);
+  double *dk = nullptr;
+  double2 *tmp = nullptr;
+  DiskDev *dd = nullptr;
+  TRY(dalloc(&dk, (i64)kt.size()));
+  TRY(dalloc(&tmp, (i64)D.Nc * D.M));
+  TRY(dalloc(&dd, 1));
+  int rc = DFK_OK;
+  hipError_t e = hipMemcpy(dk, kt.data(), sizeof(double) * kt.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dd, &D, sizeof D, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = dfk_launch_khat(h->stream, dd, ceil_div(D.M, 2 * D.pr), ceil_div(D.Nc, D.cb),
+                        (size_t)2 * D.pr * D.Nphi * sizeof(double2),
+                        (size_t)2 * D.cb * D.M * sizeof(double2), dk, h->tw, tmp, h->khat);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && full) {
+    full->resize((size_t)D.Nc * D.M);
+    e = hipMemcpy(full->data(), tmp, sizeof(double2) * full->size(), hipMemcpyDeviceToHost);
+  }
+  if (e != hipSuccess) {
+    set_err("K-hat build failed: %s", hipGetErrorString(e));
+    rc = DFK_EHIP;
+  }
+  hipFree(dk);
+  hipFree(tmp);
+  hipFree(dd);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *dfk_last_error(void) { return g_err.c_str(); }
+
+int dfk_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle **out) {
+  g_err.clear();
+  if (!params || ndisks <= 0 || !out) { set_err("dfk_create: bad arguments"); return DFK_EINVAL; }
+  *out = nullptr;
+  for (int d = 0; d < ndisks; ++d) {
+    const dfk_disk_params &p = params[d];
+    if (p.Nu < 3 || p.Nphi < 4) { set_err("disk %d: need Nu >= 3 and Nphi >= 4 (got %d x %d)", d, p.Nu, p.Nphi); return DFK_EINVAL; }
+    if (!(p.Rin > 0) || !(p.Rout > p.Rin)) { set_err("disk %d: need 0 < Rin < Rout", d); return DFK_EINVAL; }
+    if (p.Nu > 1024 || p.Nphi > 2048) {
+      // one row (Nphi) or one padded column (2 Nu) must fit a 64 KB LDS transform
+      set_err("disk %d: need Nu <= 1024 and Nphi <= 2048 (got %d x %d)", d, p.Nu, p.Nphi);
+      return DFK_EINVAL;
+    }
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    set_err("no HIP device available (%s)", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+    return DFK_EHIP;
+  }
+  if (device < 0 || device >= ndev) { set_err("device %d out of range (%d devices)", device, ndev); return DFK_EINVAL; }
+  dfk_handle *h = new dfk_handle();
+  h->device = device;
+  h->nd = ndisks;
+  h->P.assign(params, params + ndisks);
+  h->hD.resize(ndisks);
+  h->hcur.assign(ndisks, 0);
+  h->mD.assign(ndisks, 1.0);
+  h->drained.assign(ndisks, 0);
+  h->hsamples.resize(ndisks);
+  auto fail = [&](int rc) { dfk_destroy(h); return rc; };
+  if (hipSetDevice(device) != hipSuccess) { set_err("hipSetDevice failed"); return fail(DFK_EHIP); }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { set_err("stream create failed"); return fail(DFK_EHIP); }
+  hipEventCreate(&h->ev0);
+  hipEventCreate(&h->ev1);
+
+  // descriptors, pool offsets, twiddle dedup
+  std::map<int, i64> twoff;
+  std::vector<double2> twall;
+  i64 so = 0, spo = 0, kho = 0, go = 0, co = 0;
+  for (int d = 0; d < ndisks; ++d) {
+    const dfk_disk_params &p = params[d];
+    DiskDev &D = h->hD[d];
+    memset(&D, 0, sizeof D);
+    D.Nu = p.Nu; D.Nphi = p.Nphi; D.Nc = p.Nphi / 2 + 1; D.M = 2 * p.Nu;
+    D.gamma2 = (p.gamma == 2.0);
+    D.selfgrav = p.selfgrav ? 1 : 0;
+    D.K = p.K; D.gamma = p.gamma; D.gmstar = p.gmstar; D.G = p.G; D.eps = p.eps;
+    D.du = (std::log(p.Rout) - std::log(p.Rin)) / p.Nu;   // hydro.hpp:31-32
+    D.dphi = 2.0 * M_PI / p.Nphi;
+    D.inv2dphi = 1.0 / (2.0 * D.dphi);
+    if (D.selfgrav && (!factor(D.Nphi, D.rad_row, &D.nst_row) || !factor(D.M, D.rad_col, &D.nst_col))) {
+      set_err("disk %d: transform lengths %d and %d must factor into 2, 3, 5, 7, 11, 13", d,
+              D.Nphi, D.M);
+      return fail(DFK_EUNSUPPORTED);
+    }
+    for (int n : {D.Nphi, D.M}) {
+      if (!D.selfgrav) break;
+      if (!twoff.count(n)) {
+        std::vector<double2> t;
+        twiddles(n, t);
+        twoff[n] = (i64)twall.size();
+        twall.insert(twall.end(), t.begin(), t.end());
+      }
+    }
+    D.two_row = twoff[D.Nphi];
+    D.two_col = twoff[D.M];
+    D.so = so; so += (i64)p.Nu * p.Nphi;
+    D.spo = spo; spo += (i64)p.Nu * D.Nc;
+    D.kho = kho; kho += (i64)(p.Nu + 1) * D.Nc;
+    D.go = go; go += (i64)GT_COUNT * (p.Nu + 1);
+    D.cos_off = (int)co; co += 2 * (i64)p.Nphi;
+    // default shapes
+    D.ti = 8;
+    D.tj = 64;
+    D.pr = std::max(1, 1024 / D.Nphi);
+    D.cb = std::max(1, 1024 / D.M);
+    h->maxN = std::max(h->maxN, (i64)p.Nu * p.Nphi);
+    h->maxF = std::max(h->maxF, (i64)(p.Nu + 1) * p.Nphi);
+  }
+  h->ntot = so; h->nspec = spo; h->nkhat = kho; h->ngeom = go; h->ntrig = co; h->ntw = (i64)twall.size();
+  int rc;
+  if ((rc = dalloc(&h->dD, ndisks)) || (rc = dalloc(&h->st, 6 * h->ntot)) ||
+      (rc = dalloc(&h->phi, h->ntot)) || (rc = dalloc(&h->gR, h->ntot)) ||
+      (rc = dalloc(&h->gp, h->ntot)) || (rc = dalloc(&h->F, 3 * h->maxF)) ||
+      (rc = dalloc(&h->spec, h->nspec)) || (rc = dalloc(&h->khat, h->nkhat)) ||
+      (rc = dalloc(&h->geom, h->ngeom)) || (rc = dalloc(&h->trig, h->ntrig)) ||
+      (rc = dalloc(&h->tw, h->ntw)) || (rc = dalloc(&h->ctl, ndisks)))
+    return fail(rc);
+  if (hipMemset(h->st, 0, sizeof(double) * 6 * h->ntot) != hipSuccess ||
+      hipMemset(h->phi, 0, sizeof(double) * h->ntot) != hipSuccess ||
+      hipMemset(h->ctl, 0, sizeof(DiskCtl) * ndisks) != hipSuccess) {
+    set_err("hipMemset failed");
+    return fail(DFK_EHIP);
+  }
+  {
+    std::vector<double> geom, trig, gall((size_t)h->ngeom), tall((size_t)h->ntrig);
+    for (int d = 0; d < ndisks; ++d) {
+      host_tables(h->hD[d], params[d], geom, trig);
+      std::copy(geom.begin(), geom.end(), gall.begin() + h->hD[d].go);
+      std::copy(trig.begin(), trig.end(), tall.begin() + h->hD[d].cos_off);
+    }
+    if (hipMemcpy(h->geom, gall.data(), sizeof(double) * gall.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->trig, tall.data(), sizeof(double) * tall.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->tw, twall.data(), sizeof(double2) * twall.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      set_err("table upload failed");
+      return fail(DFK_EHIP);
+    }
+  }
+  if ((rc = build_layout(h))) return fail(rc);
+  for (int d = 0; d < ndisks; ++d)
+    if (params[d].selfgrav && (rc = build_khat(h, d, nullptr))) return fail(rc);
+  *out = h;
+  return DFK_OK;
+}
+
+void dfk_destroy(dfk_handle *h) {
+  if (!h) return;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  drop_graphs(h);
+  hipFree(h->dD); hipFree(h->st); hipFree(h->phi); hipFree(h->gR); hipFree(h->gp); hipFree(h->F);
+  hipFree(h->spec); hipFree(h->khat); hipFree(h->geom); hipFree(h->trig); hipFree(h->tw);
+  hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv);
+  hipFree(h->ctl); hipFree(h->samples);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+}
+
+int dfk_num_disks(const dfk_handle *h) { return h ? h->nd : 0; }
+
+int dfk_geometry(dfk_handle *h, int d, double *R, double *Rf, double *Vol) {
+  TRY(check_disk(h, d));
+  const dfk_disk_params &p = h->P[d];
+  const DiskDev &D = h->hD[d];
+  const double umin = std::log(p.Rin);
+  for (int i = 0; i <= p.Nu; ++i) {
+    const double rf = std::exp(umin + i * D.du);
+    if (Rf) Rf[i] = rf;
+  }
+  for (int i = 0; i < p.Nu; ++i) {
+    const double r = std::exp(umin + (i + 0.5) * D.du);
+    if (R) R[i] = r;
+    if (Vol) Vol[i] = r * (std::exp(umin + (i + 1) * D.du) - std::exp(umin + i * D.du)) * D.dphi;
+  }
+  return DFK_OK;
+}
+
+int dfk_set_state(dfk_handle *h, int d, const double *Sig, const double *vR, const double *vp) {
+  TRY(check_disk(h, d));
+  if (!Sig || !vR || !vp) { set_err("null state pointer"); return DFK_EINVAL; }
+  HIPCHK(hipSetDevice(h->device));
+  const size_t n = (size_t)h->hD[d].Nu * h->hD[d].Nphi * sizeof(double);
+  const int c = h->hcur[d];
+  HIPCHK(hipMemcpy(field(h, c, 0, d), Sig, n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(field(h, c, 1, d), vR, n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(field(h, c, 2, d), vp, n, hipMemcpyHostToDevice));
+  h->run_ready = false;
+  return DFK_OK;
+}
+
+int dfk_get_state(dfk_handle *h, int d, double *Sig, double *vR, double *vp) {
+  TRY(check_disk(h, d));
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t n = (size_t)h->hD[d].Nu * h->hD[d].Nphi * sizeof(double);
+  const int c = h->hcur[d];
+  if (Sig) HIPCHK(hipMemcpy(Sig, field(h, c, 0, d), n, hipMemcpyDeviceToHost));
+  if (vR) HIPCHK(hipMemcpy(vR, field(h, c, 1, d), n, hipMemcpyDeviceToHost));
+  if (vp) HIPCHK(hipMemcpy(vp, field(h, c, 2, d), n, hipMemcpyDeviceToHost));
+  return DFK_OK;
+}
+
+int dfk_timestep(dfk_handle *h, int d, double cfl, double *dt) {
+  TRY(check_disk(h, d));
+  if (!dt) { set_err("null dt"); return DFK_EINVAL; }
+  double mx, s[4];
+  TRY(diag_disk(h, d, &mx, s));
+  *dt = cfl / mx;
+  return DFK_OK;
+}
+
+int dfk_modes(dfk_handle *h, int d, int m, double mD, double *mag, double *phase) {
+  TRY(check_disk(h, d));
+  if (m != 0 && m != 2) { set_err("dfk_modes: m must be 0 or 2 (the driver's diagnostics)"); return DFK_EINVAL; }
+  double mx, s[4];
+  TRY(diag_disk(h, d, &mx, s));
+  const double re = (m == 0) ? s[0] : s[1], im = (m == 0) ? 0.0 : s[2];
+  if (mag) *mag = std::sqrt(re * re + im * im) / mD;
+  if (phase) *phase = std::atan2(im, re);
+  return DFK_OK;
+}
+
+int dfk_disk_mass(dfk_handle *h, int d, double *mD) {
+  TRY(check_disk(h, d));
+  double mx, s[4];
+  TRY(diag_disk(h, d, &mx, s));
+  if (mD) *mD = s[0];
+  return DFK_OK;
+}
+
+int dfk_potential(dfk_handle *h, int d, const double *Sigma, double *Phi) {
+  TRY(check_disk(h, d));
+  if (!Sigma || !Phi) { set_err("null pointer"); return DFK_EINVAL; }
+  if (!h->hD[d].selfgrav) { set_err("disk %d was created with selfgrav = 0 (no K-hat)", d); return DFK_ESTATE; }
+  TRY(build_layout(h));
+  const int scratch = 1 - h->hcur[d];
+  const size_t n = (size_t)h->hD[d].Nu * h->hD[d].Nphi * sizeof(double);
+  HIPCHK(hipMemcpy(field(h, scratch, 0, d), Sigma, n, hipMemcpyHostToDevice));
+  TRY(unit_ctl(h, d, scratch, h->hcur[d], 0.0));
+  TRY(launch_gravity_disk(h, d));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(Phi, h->phi + h->hD[d].so, n, hipMemcpyDeviceToHost));
+  return DFK_OK;
+}
+
+int dfk_forces(dfk_handle *h, int d, const double *Phi, double *gR, double *gp) {
+  TRY(check_disk(h, d));
+  if (!Phi || !gR || !gp) { set_err("null pointer"); return DFK_EINVAL; }
+  HIPCHK(hipSetDevice(h->device));
+  const DiskDev &D = h->hD[d];
+  const size_t n = (size_t)D.Nu * D.Nphi * sizeof(double);
+  HIPCHK(hipMemcpy(h->phi + D.so, Phi, n, hipMemcpyHostToDevice));
+  HIPCHK(dfk_launch_forces(h->stream, h->dD + d, (i64)D.Nu * D.Nphi, h->phi, h->geom, h->gR, h->gp));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(gR, h->gR + D.so, n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(gp, h->gp + D.so, n, hipMemcpyDeviceToHost));
+  return DFK_OK;
+}
+
+int dfk_step(dfk_handle *h, int d, double dt, const double *gR, const double *gp) {
+  TRY(check_disk(h, d));
+  TRY(build_layout(h));
+  const DiskDev &D = h->hD[d];
+  const size_t n = (size_t)D.Nu * D.Nphi * sizeof(double);
+  if (gR) HIPCHK(hipMemcpy(h->gR + D.so, gR, n, hipMemcpyHostToDevice));
+  if (gp) HIPCHK(hipMemcpy(h->gp + D.so, gp, n, hipMemcpyHostToDevice));
+  const int c = h->hcur[d];
+  TRY(unit_ctl(h, d, c, 1 - c, dt));
+  const int nb = h->upd0[d + 1] - h->upd0[d];
+  HIPCHK(dfk_launch_update(h->stream, nb, h->lds_upd, 1, h->dD, h->map_upd, h->upd0[d], h->ctl,
+                           h->st, h->ntot, h->phi, h->gR, h->gp, gR != nullptr, gp != nullptr,
+                           h->geom, h->trig, h->pcfl, h->pcm));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->hcur[d] = 1 - c;
+  return DFK_OK;
+}
+
+int dfk_stage(dfk_handle *h, int d, int stage, double dt, const double *gR, const double *gp) {
+  TRY(check_disk(h, d));
+  if (stage < 1 || stage > 4) { set_err("stage must be 1..4"); return DFK_EINVAL; }
+  HIPCHK(hipSetDevice(h->device));
+  const DiskDev &D = h->hD[d];
+  const size_t n = (size_t)D.Nu * D.Nphi * sizeof(double);
+  if (stage == 1) {
+    if (gR) HIPCHK(hipMemcpy(h->gR + D.so, gR, n, hipMemcpyHostToDevice));
+    if (gp) HIPCHK(hipMemcpy(h->gp + D.so, gp, n, hipMemcpyHostToDevice));
+  }
+  h->run_ready = false;
+  HIPCHK(dfk_launch_stage(h->stream, h->dD + d, (i64)D.Nu * D.Nphi, stage, h->st, h->ntot,
+                          h->hcur[d], dt, (stage == 1 && gR) ? h->gR : nullptr,
+                          (stage == 1 && gp) ? h->gp : nullptr, h->geom, h->F));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DFK_OK;
+}
+
+int dfk_khat(dfk_handle *h, int d, double *re, double *im) {
+  TRY(check_disk(h, d));
+  if (!h->hD[d].selfgrav) { set_err("disk %d has no K-hat (selfgrav = 0)", d); return DFK_ESTATE; }
+  HIPCHK(hipSetDevice(h->device));
+  std::vector<double2> full;
+  TRY(build_khat(h, d, &full));
+  const DiskDev &D = h->hD[d];
+  for (int p = 0; p < D.M; ++p)
+    for (int k = 0; k < D.Nc; ++k) {
+      const double2 v = full[(size_t)k * D.M + p];
+      if (re) re[(size_t)p * D.Nc + k] = v.x;
+      if (im) im[(size_t)p * D.Nc + k] = v.y;
+    }
+  return DFK_OK;
+}
+
+int dfk_run_init(dfk_handle *h, const dfk_run_params *rp, int sample_cap) {
+  if (!h || !rp) { set_err("null argument"); return DFK_EINVAL; }
+  TRY(build_layout(h));
+  const int cap = std::max(sample_cap, 4);
+  if (cap != h->cap || !h->samples) {
+    hipFree(h->samples);
+    h->samples = nullptr;
+    TRY(dalloc(&h->samples, (i64)cap * h->nd));
+    h->cap = cap;
+    drop_graphs(h);   // graphs bake the sample buffer address and capacity
+  }
+  std::vector<DiskCtl> c(h->nd);
+  for (int d = 0; d < h->nd; ++d) {
+    memset(&c[d], 0, sizeof(DiskCtl));
+    c[d].t = 0.0;
+    c[d].tnext = 0.0;
+    c[d].tlim = rp[d].tlim;
+    c[d].dtout = rp[d].dtout;
+    c[d].cfl = rp[d].cfl;
+    c[d].max_steps = rp[d].max_steps;
+    c[d].stop_on_nonfinite = rp[d].stop_on_nonfinite;
+    c[d].cur = h->hcur[d];
+    c[d].src = c[d].cur;
+    c[d].dst = 1 - c[d].cur;
+    h->mD[d] = rp[d].mD;
+    h->drained[d] = 0;
+    h->hsamples[d].clear();
+  }
+  HIPCHK(hipMemcpy(h->ctl, c.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
+  HIPCHK(dfk_launch_diag(h->stream, h->upd0[h->nd], h->dD, h->map_upd, 0, h->ctl, h->st, h->ntot,
+                         h->geom, h->trig, h->pcfl, h->pcm));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->run_ready = true;
+  return DFK_OK;
+}
+
+int dfk_advance(dfk_handle *h, int nsteps, int *all_done) {
+  if (!h) { set_err("null handle"); return DFK_EINVAL; }
+  if (!h->run_ready) { set_err("dfk_advance before dfk_run_init (or after a unit call)"); return DFK_ESTATE; }
+  if (nsteps < 0) { set_err("nsteps < 0"); return DFK_EINVAL; }
+  HIPCHK(hipSetDevice(h->device));
+  const int chunk_max = std::max(1, std::min(h->cap - 2, h->chunk));
+  double total_ms = 0.0;
+  int left = nsteps;
+  while (left > 0) {
+    const int chunk = std::min(left, chunk_max);
+    hipGraphExec_t ex;
+    TRY(get_graph(h, chunk, &ex));
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    HIPCHK(hipGraphLaunch(ex, h->stream));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    total_ms += ms;
+    TRY(drain(h));
+    left -= chunk;
+  }
+  h->last_ms = total_ms;
+  if (all_done) {
+    std::vector<DiskCtl> c(h->nd);
+    HIPCHK(hipMemcpy(c.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
+    int ad = 1;
+    for (int d = 0; d < h->nd; ++d) ad &= (c[d].done != 0);
+    *all_done = ad;
+  }
+  return DFK_OK;
+}
+
+int dfk_samples(dfk_handle *h, int d, dfk_sample *buf, int cap, int *n) {
+  TRY(check_disk(h, d));
+  std::vector<dfk_sample> &v = h->hsamples[d];
+  const int k = std::min<int>(cap, (int)v.size());
+  if (buf) std::copy(v.begin(), v.begin() + k, buf);
+  v.erase(v.begin(), v.begin() + k);
+  if (n) *n = k;
+  return DFK_OK;
+}
+
+int dfk_get_status(dfk_handle *h, int d, dfk_status *st) {
+  TRY(check_disk(h, d));
+  if (!st) { set_err("null status"); return DFK_EINVAL; }
+  HIPCHK(hipSetDevice(h->device));
+  DiskCtl c;
+  TRY(read_ctl(h, d, &c));
+  st->t = c.t;
+  st->steps = c.step;
+  st->done = c.done;
+  st->flags = c.flags;
+  st->c2_final = std::sqrt(c.fin_re2 * c.fin_re2 + c.fin_im2 * c.fin_im2) / h->mD[d];
+  st->ph2_final = std::atan2(c.fin_im2, c.fin_re2);
+  return DFK_OK;
+}
+
+int dfk_set_shapes(dfk_handle *h, int d, int ti, int tj, int pr, int cb) {
+  TRY(check_disk(h, d));
+  DiskDev &D = h->hD[d];
+  if (ti > 0) D.ti = ti;
+  if (tj > 0) D.tj = tj;
+  if (pr > 0) D.pr = pr;
+  if (cb > 0) D.cb = cb;
+  h->layout_ok = false;
+  return build_layout(h);
+}
+
+int dfk_get_shapes(dfk_handle *h, int d, int *ti, int *tj, int *pr, int *cb) {
+  TRY(check_disk(h, d));
+  const DiskDev &D = h->hD[d];
+  if (ti) *ti = D.ti;
+  if (tj) *tj = D.tj;
+  if (pr) *pr = D.pr;
+  if (cb) *cb = D.cb;
+  return DFK_OK;
+}
+
+int dfk_last_advance_ms(dfk_handle *h, double *ms) {
+  if (!h || !ms) { set_err("null argument"); return DFK_EINVAL; }
+  *ms = h->last_ms;
+  return DFK_OK;
+}
+
+int dfk_set_graph_steps(dfk_handle *h, int nsteps) {
+  if (!h || nsteps < 1) { set_err("bad argument"); return DFK_EINVAL; }
+  if (nsteps != h->chunk) drop_graphs(h);
+  h->chunk = nsteps;
+  return DFK_OK;
+}
+
+int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
+  if (!h || !ms || nrep < 1) { set_err("bad argument"); return DFK_EINVAL; }
+  if (!h->run_ready) { set_err("dfk_profile_step needs dfk_run_init"); return DFK_ESTATE; }
+  HIPCHK(hipSetDevice(h->device));
+  std::vector<DiskCtl> saved(h->nd);
+  HIPCHK(hipMemcpy(saved.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
+  // a control block that lets every kernel run: not done, src = cur
+  std::vector<DiskCtl> c = saved;
+  for (int d = 0; d < h->nd; ++d) {
+    c[d].done = 0;
+    c[d].src = c[d].cur;
+    c[d].dst = 1 - c[d].cur;
+    if (!(c[d].dt > 0)) c[d].dt = 1e-6;
+    c[d].tlim = 1e300;
+    c[d].max_steps = 0;
+  }
+  const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
+  hipEvent_t e0 = h->ev0, e1 = h->ev1;
+  for (int k = 0; k < 5; ++k) {
+    HIPCHK(hipMemcpy(h->ctl, c.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
+    HIPCHK(hipEventRecord(e0, h->stream));
+    for (int r = 0; r < nrep; ++r) {
+      switch (k) {
+        case 0:
+          HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap));
+          break;
+        case 1:
+          HIPCHK(dfk_launch_r2c(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
+                                h->geom, h->tw, h->spec));
+          break;
+        case 2:
+          HIPCHK(dfk_launch_conv(h->stream, nc, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
+                                 h->tw, h->spec));
+          break;
+        case 3:
+          HIPCHK(dfk_launch_c2r(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
+                                h->geom, h->tw, h->phi));
+          break;
+        case 4:
+          HIPCHK(dfk_launch_update(h->stream, nu, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
+                                   h->ntot, h->phi, nullptr, nullptr, 0, 0, h->geom, h->trig,
+                                   h->pcfl, h->pcm));
+          break;
+      }
+    }
+    HIPCHK(hipEventRecord(e1, h->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    ms[k] = (double)t / nrep;
+  }
+  // the control kernel advanced t / nsamples: restore the loop position and
+  // recompute the partials of the (unchanged) current state
+  HIPCHK(hipMemcpy(h->ctl, saved.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
+  HIPCHK(dfk_launch_diag(h->stream, nu, h->dD, h->map_upd, 0, h->ctl, h->st, h->ntot, h->geom,
+                         h->trig, h->pcfl, h->pcm));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DFK_OK;
+}
+
+}  // extern "C"

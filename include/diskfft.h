@@ -1,0 +1,149 @@
+/* diskfft on MI355X -- C ABI of the HIP time-step library (libdiskfft.so).
+ *
+ * Plain C: opaque handle, plain pointers and sizes, int status (DFK_OK = 0),
+ * no exceptions or torch types across the boundary.  Host pointers are copied
+ * in or out during the call and never retained.  One handle owns one HIP
+ * stream on one device and holds one or more disks ("an ensemble"; a single
+ * disk is an ensemble of one).  A handle must not be used from two host
+ * threads at once.  Every compute call needs a HIP device; without one the
+ * call fails with DFK_EHIP and dfk_last_error() says why (there is no CPU
+ * fallback).
+ *
+ * Reference interfaces replaced (paths relative to the oklo/heavy-disks tree):
+ *   DiskHydro                 diskfft/src/hydro.hpp:20-180
+ *   KalnajsSolver             diskfft/src/kalnajs.hpp:19-117
+ *   the diskdisk driver loop  diskfft/src/diskmain.cpp:16-115
+ */
+#ifndef DISKFFT_H
+#define DISKFFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFK_OK 0
+#define DFK_EINVAL 1        /* bad argument (sizes, null pointers, disk index) */
+#define DFK_EHIP 2          /* HIP runtime error or no device                  */
+#define DFK_ENOMEM 3        /* device allocation failed                        */
+#define DFK_EUNSUPPORTED 4  /* transform length with a prime factor > 13       */
+#define DFK_ESTATE 5        /* call out of order (e.g. advance before run_init)*/
+
+/* Constructor arguments of DiskHydro (hydro.hpp:28-29) and KalnajsSolver
+ * (kalnajs.hpp:25-26) for one disk. */
+typedef struct {
+  int Nu, Nphi;          /* radial (log-spaced) and azimuthal zones; Nu >= 3, Nphi >= 4 */
+  double Rin, Rout;      /* grid radii */
+  double K, gamma;       /* barotropic EOS P = K Sigma^gamma */
+  double gmstar;         /* G M_star */
+  double eps, G;         /* Kalnajs softening (scale-free) and G */
+  int selfgrav;          /* 0: star only; 1: Kalnajs self-gravity (diskmain.cpp:22,103-108) */
+} dfk_disk_params;
+
+/* Loop parameters for one disk (diskmain.cpp:88,92-101). */
+typedef struct {
+  double tlim;           /* loop ends when t >= tlim; last step clipped to land on tlim */
+  double dtout;          /* output interval (reference: 0.25) */
+  double cfl;            /* CFL number (reference: 0.3) */
+  double mD;             /* mass normaliser of |C_m| (diskmain.cpp:65-70, 82) */
+  long long max_steps;   /* stop after this many steps, <= 0: no cap */
+  int stop_on_nonfinite; /* 1: end the disk's loop when a NaN/Inf zone appears */
+} dfk_run_params;
+
+/* One output record, as diskmain.cpp:93-98 prints it: t, |C0|, |C2|, arg C2. */
+typedef struct { double t, c0, c2, ph2; } dfk_sample;
+
+typedef struct {
+  double t;              /* current time */
+  long long steps;       /* steps taken */
+  int done;              /* loop ended */
+  int flags;             /* bit 0: a non-finite zone was seen */
+  double c2_final;       /* |C2| at loop end (diskmain.cpp:111) */
+  double ph2_final;
+} dfk_status;
+
+typedef struct dfk_handle dfk_handle;
+
+/* Last error message of the calling thread ("" if none). */
+const char *dfk_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never fails). */
+int dfk_device_count(void);
+
+/* Creates ndisks disks on `device` (DiskHydro ctor hydro.hpp:28-43 and
+ * KalnajsSolver ctor kalnajs.hpp:25-42, including the K-hat build
+ * kalnajs.hpp:105-116).  State is zeroed. */
+int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle **out);
+void dfk_destroy(dfk_handle *h);
+int dfk_num_disks(const dfk_handle *h);
+
+/* Geometry tables of DiskHydro: R (Nu), Rf (Nu+1), Vol (Nu) (hydro.hpp:31-38). */
+int dfk_geometry(dfk_handle *h, int disk, double *R, double *Rf, double *Vol);
+
+/* Public state vectors Sig, vR, vp of DiskHydro (hydro.hpp:24), Nu*Nphi each. */
+int dfk_set_state(dfk_handle *h, int disk, const double *Sig, const double *vR, const double *vp);
+int dfk_get_state(dfk_handle *h, int disk, double *Sig, double *vR, double *vp);
+
+/* DiskHydro::timestep (hydro.hpp:48-61). */
+int dfk_timestep(dfk_handle *h, int disk, double cfl, double *dt);
+
+/* KalnajsSolver::potential (kalnajs.hpp:48-63): Phi from Sigma (Nu*Nphi). */
+int dfk_potential(dfk_handle *h, int disk, const double *Sigma, double *Phi);
+
+/* KalnajsSolver::forces (kalnajs.hpp:65-80). */
+int dfk_forces(dfk_handle *h, int disk, const double *Phi, double *gR, double *gphi);
+
+/* DiskHydro::step (hydro.hpp:63-70); gR / gphi may be NULL as in the reference. */
+int dfk_step(dfk_handle *h, int disk, double dt, const double *gR, const double *gphi);
+
+/* One stage of DiskHydro::step: 1 source (hydro.hpp:73-96, gR/gphi may be NULL),
+ * 2 transportR (:99-140), 3 transportPhi (:143-179), 4 floor (:68-69). */
+int dfk_stage(dfk_handle *h, int disk, int stage, double dt, const double *gR, const double *gphi);
+
+/* Complex K-hat (M x (Nphi/2+1), row-major, re/im split) as built by
+ * KalnajsSolver::build_kernel (kalnajs.hpp:105-116); for checks. */
+int dfk_khat(dfk_handle *h, int disk, double *re, double *im);
+
+/* The C_m diagnostic of diskmain.cpp:72-84 on the current state. */
+int dfk_modes(dfk_handle *h, int disk, int m, double mD, double *mag, double *phase);
+
+/* Disk mass sum Sigma R dR dphi of the current state (diskmain.cpp:65-70). */
+int dfk_disk_mass(dfk_handle *h, int disk, double *mD);
+
+/* The time loop of diskmain.cpp:92-113, device resident.  run_init resets
+ * t = 0 and the output schedule for every disk (params: one per disk).
+ * advance runs up to `nsteps` loop iterations on every unfinished disk, as a
+ * captured HIP graph, and sets *all_done.  samples drains the records produced
+ * since the last call (at most one per iteration, so drain after every
+ * advance of more than `cap_hint` steps; see dfk_run_init). */
+int dfk_run_init(dfk_handle *h, const dfk_run_params *params, int sample_cap);
+int dfk_advance(dfk_handle *h, int nsteps, int *all_done);
+int dfk_samples(dfk_handle *h, int disk, dfk_sample *buf, int cap, int *n);
+int dfk_get_status(dfk_handle *h, int disk, dfk_status *st);
+
+/* Tile / transform-batch shapes (the autotuner's knobs): update tile ti x tj,
+ * complex rows per workgroup in the row transforms (pr) and columns per
+ * workgroup in the column convolution (cb).  0 keeps the current value.
+ * Takes effect for the next dfk_run_init / unit call. */
+int dfk_set_shapes(dfk_handle *h, int disk, int ti, int tj, int pr, int cb);
+int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
+
+/* Device time in ms of the last dfk_advance (HIP events around the graph
+ * launch on the handle's stream). */
+int dfk_last_advance_ms(dfk_handle *h, double *ms);
+
+/* Loop iterations captured per HIP graph in dfk_advance (default 128). */
+int dfk_set_graph_steps(dfk_handle *h, int nsteps);
+
+/* Profiling aid (no counterpart in the reference): launches each kernel of
+ * one loop iteration `nrep` times back to back for every disk, bracketed by
+ * HIP events on the handle's stream, and returns the average duration in ms
+ * of one launch of: ms[0] control, ms[1] row r2c, ms[2] column convolution,
+ * ms[3] row c2r, ms[4] fused update.  The state and loop position are left
+ * as they were (the update rewrites the same output buffer each time).
+ * Needs dfk_run_init first. */
+int dfk_profile_step(dfk_handle *h, int nrep, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+/* diskfft on MI355X -- disk models and initial conditions (host code, no GPU).
+ *
+ * Two models:
+ *   DFK_MODEL_LKA98     the reference driver's Gaussian ring (diskmain.cpp:28-63):
+ *                       Rin 0.25, Rout 1, Sigma0 = 0.372 exp(-(R-0.45)^2/0.05),
+ *                       K 0.25, gamma 2, GM* 0.6, seed Sigma0 (1 + amp cos 2phi).
+ *   DFK_MODEL_POWERLAW  a power-law disk Sigma0 ~ R^slope with tanh tapers,
+ *                       normalised to a disk/star mass ratio, K set by a
+ *                       bracket + bisection so that min Toomre Q over the disk
+ *                       body [q_in, q_out] equals Qmin (the smallest such K;
+ *                       if no K reaches Qmin, the K maximising min Q, with
+ *                       Qmin_got < Qmin), and seeded 1e-3 rms white noise in
+ *                       azimuthal modes m = 1..8 (SplitMix64 + Box-Muller).
+ *                       No reference counterpart (the reference only runs LKA98).
+ *
+ * The equilibrium needs the azimuthally averaged self-gravity g_R of Sigma0
+ * (diskmain.cpp:36-48), which the caller computes with the solver of its
+ * choice (dfk_potential/dfk_forces on the GPU) and passes in:
+ *   1. dfk_model_sigma0     -> Sigma0(R_i)
+ *   2. caller: gRdisk_i = phi-average of g_R of the axisymmetric Sigma0
+ *   3. dfk_model_state      -> K (and Q profile), Sigma, vR, vphi
+ */
+#ifndef DISKFFT_MODEL_H
+#define DISKFFT_MODEL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFK_MODEL_LKA98 0
+#define DFK_MODEL_POWERLAW 1
+
+typedef struct {
+  int kind;
+  double Rin, Rout, gmstar, G, gamma;
+  double K;                  /* LKA98: fixed; POWERLAW: output of the Q solve */
+  double amp;                /* LKA98 cos(2 phi) seed amplitude */
+  /* POWERLAW */
+  double mass_ratio;         /* M_disk / M_star */
+  double Qmin;               /* target min Toomre Q over [q_in, q_out] */
+  double slope;              /* Sigma0 ~ R^slope */
+  double taper_in, taper_out, taper_w;
+  double q_in, q_out;        /* radial window of the Q minimum (the disk body) */
+  double Qmin_got;           /* output: min Q actually reached (< Qmin if unreachable) */
+  double noise_rms;          /* delta Sigma / Sigma rms */
+  int mmin, mmax;            /* noise modes */
+  unsigned long long seed;
+} dfk_model;
+
+/* Reference LKA98 defaults (diskmain.cpp:28-29) and the power-law defaults
+ * (M_d/M* 1, Qmin 1.3 over [0.2, 0.8], slope -1.5, tapers 0.15/0.9 width 0.02,
+ * noise 1e-3 in m = 1..8, seed 42, r in [0.1, 1], GM* = 1). */
+void dfk_model_defaults(int kind, dfk_model *m);
+
+/* Sigma0 on the cell centres R (length Nu); log-grid geometry as hydro.hpp. */
+int dfk_model_sigma0(const dfk_model *m, int Nu, int Nphi, double *sig0);
+
+/* Equilibrium state.  gRdisk (length Nu, may be NULL = no self-gravity).  For
+ * POWERLAW, solves K and writes it to m->K; Q (length Nu, may be NULL)
+ * receives the Toomre Q profile.  Sig/vR/vp are Nu*Nphi row-major. */
+int dfk_model_state(dfk_model *m, int Nu, int Nphi, const double *gRdisk, double *Sig,
+                    double *vR, double *vp, double *Q);
+
+/* Disk mass sum Sigma R dR dphi (diskmain.cpp:65-70). */
+double dfk_model_mass(const dfk_model *m, int Nu, int Nphi, const double *Sig);
+
+/* SplitMix64 stream: next 64-bit value (state updated), and a standard normal
+ * by Box-Muller (one value per call; the pair's sine half is discarded). */
+unsigned long long dfk_splitmix64(unsigned long long *state);
+double dfk_normal(unsigned long long *state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,260 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Bar (north star): Sigma and velocities after 10 steps within 1e-10 relative
+(max-norm over the field, relative to the field's max).  The kernels use the
+reference's formulas with reciprocal tables instead of some divisions and FMA
+contraction, so agreement is round-off level; the tests assert 1e-10 for the
+loop and tighter bounds (stated per test) for single operators.
+"""
+import numpy as np
+import pytest
+
+from helpers import grid, relerr, rough_state
+
+pytestmark = pytest.mark.gpu
+
+LOOP_TOL = 1e-10      # north-star tolerance, Sigma/vR/vphi after 10 steps
+OP_TOL = 1e-12        # single operator (one transform, one stage, one step)
+
+
+@pytest.fixture(scope="module")
+def dfk():
+    import diskfft_amd
+    if diskfft_amd.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+    return diskfft_amd
+
+
+def _pair(orc, dfk, Nu, Nphi, eps=0.02, Rin=0.25, Rout=1.0, selfgrav=1, **kw):
+    p = dfk.disk_params(Nu, Nphi, Rin=Rin, Rout=Rout, eps=eps, selfgrav=selfgrav, **kw)
+    e = dfk.Ensemble(p)
+    h = orc.Hydro(Nu, Nphi, Rin, Rout, p["K"], p["gamma"], p["gmstar"])
+    k = orc.Kalnajs(Nu, Nphi, Rin, Rout, eps, p["G"]) if selfgrav else None
+    return e, h, k
+
+
+# ------------------------------------------------------------------ set-up tables
+@pytest.mark.parametrize("Nu,Nphi", [(32, 32), (24, 20), (12, 15), (64, 96)])
+def test_geometry_and_khat(orc, dfk, Nu, Nphi):
+    print(f"case geometry/khat {Nu}x{Nphi}", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01)
+    R, Rf, V = e.geometry()
+    assert np.array_equal(R, h.R) and np.array_equal(Rf, h.Rf) and np.array_equal(V, h.Vol)
+    Kg = e.khat()
+    Ko = k.khat()
+    assert relerr(Kg, Ko) < 1e-13
+    assert np.max(np.abs(Kg.imag)) / np.max(np.abs(Kg.real)) < 1e-13
+
+
+# ------------------------------------------------------------------ gravity
+def test_potential_known_answer_32x32(orc, dfk):
+    """test_kalnajs.cpp:10-42: FFT potential vs direct convolution < 1e-10."""
+    print("case potential 32x32 known answer", flush=True)
+    Nu = Nphi = 32
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.02)
+    R, phi, _ = grid(Nu, Nphi)
+    Sig = 0.372 * np.exp(-(R - 0.45) ** 2 / 0.05)[:, None] * (1 + 0.3 * np.cos(2 * phi))[None, :]
+    pg = e.potential(Sig)
+    pd = k.potential_direct(Sig)
+    assert np.max(np.abs(pg - pd)) / np.max(np.abs(pd)) < 1e-10
+    assert relerr(pg, k.potential(Sig)) < 1e-13
+
+
+@pytest.mark.parametrize("Nu,Nphi,eps", [(16, 24, 0.02), (12, 15, 0.005), (64, 64, 0.01),
+                                         (48, 80, 0.04), (128, 192, 0.0025), (256, 256, 0.01),
+                                         (96, 28, 0.02)])
+def test_potential_matches_oracle(orc, dfk, Nu, Nphi, eps):
+    print(f"case potential {Nu}x{Nphi} eps={eps}", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=eps, Rin=0.1)
+    rng = np.random.default_rng(Nu + Nphi)
+    Sig = rng.random((Nu, Nphi))
+    pg = e.potential(Sig)
+    assert relerr(pg, k.potential(Sig)) < 1e-13
+    if Nu * Nphi <= 4096:
+        assert relerr(pg, k.potential_direct(Sig)) < 1e-12
+
+
+@pytest.mark.parametrize("Nu,Nphi", [(16, 16), (35, 40), (128, 128), (33, 26)])
+def test_forces_match_oracle(orc, dfk, Nu, Nphi):
+    print(f"case forces {Nu}x{Nphi}", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi)
+    rng = np.random.default_rng(1)
+    Phi = rng.standard_normal((Nu, Nphi))
+    gRg, gpg = e.forces(Phi)
+    gRo, gpo = k.forces(Phi)
+    assert relerr(gRg, gRo) < 1e-14 and relerr(gpg, gpo) < 1e-14
+
+
+# ------------------------------------------------------------------ hydro operators
+@pytest.mark.parametrize("Nu,Nphi", [(16, 24), (37, 52), (128, 128)])
+def test_timestep_matches_oracle(orc, dfk, Nu, Nphi):
+    print(f"case timestep {Nu}x{Nphi}", flush=True)
+    e, h, _ = _pair(orc, dfk, Nu, Nphi, selfgrav=0)
+    st = rough_state(Nu, Nphi, seed=5)
+    e.set_state(*st)
+    h.set_state(*st)
+    dg, do = e.timestep(0.3), h.timestep(0.3)
+    assert abs(dg - do) / do < 1e-14
+
+
+@pytest.mark.parametrize("stage", [1, 2, 3, 4])
+@pytest.mark.parametrize("Nu,Nphi", [(16, 24), (37, 52)])
+def test_single_stage_matches_oracle(orc, dfk, stage, Nu, Nphi):
+    print(f"case stage {stage} {Nu}x{Nphi}", flush=True)
+    e, h, _ = _pair(orc, dfk, Nu, Nphi, selfgrav=0)
+    st = rough_state(Nu, Nphi, seed=stage)
+    if stage == 4:
+        st[0][3, :5] = -1.0   # below the floor
+    e.set_state(*st)
+    h.set_state(*st)
+    rng = np.random.default_rng(3)
+    gR = -0.3 + 0.05 * rng.standard_normal((Nu, Nphi))
+    gp = 0.05 * rng.standard_normal((Nu, Nphi))
+    dt = 0.5 * h.timestep(0.3)
+    e.stage(stage, dt, gR, gp)
+    {1: lambda: h.source(dt, gR, gp), 2: lambda: h.transportR(dt),
+     3: lambda: h.transportPhi(dt), 4: lambda: h.floor()}[stage]()
+    for a, b in zip(e.get_state(), h.state()):
+        assert relerr(a, b) < OP_TOL
+
+
+@pytest.mark.parametrize("shape", [(8, 64, 0, 0), (4, 16, 0, 0), (3, 7, 0, 0), (16, 32, 0, 0),
+                                   (1, 5, 0, 0)])
+def test_fused_step_matches_oracle(orc, dfk, shape):
+    """DiskHydro::step with given g_R/g_phi through the fused tile kernel, for
+    tile shapes that do and do not divide the grid."""
+    Nu, Nphi = 37, 52
+    print(f"case fused step tiles={shape}", flush=True)
+    e, h, _ = _pair(orc, dfk, Nu, Nphi, selfgrav=0)
+    e.set_shapes(*shape)
+    st = rough_state(Nu, Nphi, seed=9)
+    e.set_state(*st)
+    h.set_state(*st)
+    rng = np.random.default_rng(4)
+    gR = -0.3 + 0.05 * rng.standard_normal((Nu, Nphi))
+    gp = 0.05 * rng.standard_normal((Nu, Nphi))
+    for it in range(3):
+        dt = h.timestep(0.3)
+        g = (gR, gp) if it != 1 else (None, None)
+        e.step(dt, *g)
+        h.step(dt, *g)
+    for a, b in zip(e.get_state(), h.state()):
+        assert relerr(a, b) < OP_TOL
+
+
+# ------------------------------------------------------------------ the loop
+def _lka98(orc, Nu, Nphi, eps, amp=1e-3, selfgrav=1):
+    h = orc.Hydro(Nu, Nphi)
+    k = orc.Kalnajs(Nu, Nphi, 0.25, 1.0, eps)
+    orc.ic_lka98(h, k, selfgrav, amp)
+    return h, k
+
+
+@pytest.mark.parametrize("Nu,Nphi,eps", [(64, 64, 0.02), (128, 128, 0.02), (96, 80, 0.01),
+                                         (256, 256, 0.005), (48, 36, 0.02)])
+def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps):
+    """The north-star gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative."""
+    print(f"case 10-step loop {Nu}x{Nphi} eps={eps}", flush=True)
+    h, k = _lka98(orc, Nu, Nphi, eps)
+    e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
+    e.set_state(*h.state())
+    mD = h.disk_mass()
+    so, tf, c2f, steps = orc.run(h, k, 1, 1e9, dtout=0.25, cfl=0.3, max_steps=10, mD=mD)
+    e.run_init(1e9, dtout=0.25, cfl=0.3, mD=mD, max_steps=10)
+    e.advance(12)
+    st = e.status()
+    assert st["done"] and st["steps"] == 10 and steps == 10
+    assert abs(st["t"] - tf) / tf < 1e-13
+    errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
+    print("  rel errors Sigma, vR, vphi:", errs, flush=True)
+    assert max(errs) <= LOOP_TOL
+    sg = e.samples()
+    assert sg.shape == so.shape
+    assert abs(sg[0, 2] - so[0, 2]) / so[0, 2] < 1e-12
+
+
+def test_loop_samples_and_clip(orc, dfk):
+    """Output schedule and last-step clip of diskmain.cpp:92-110 over ~300 steps."""
+    Nu, Nphi, eps, tlim = 64, 64, 0.02, 0.6
+    print("case loop samples 64x64 tlim=0.6", flush=True)
+    h, k = _lka98(orc, Nu, Nphi, eps)
+    e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
+    e.set_state(*h.state())
+    mD = h.disk_mass()
+    so, tf, c2f, steps = orc.run(h, k, 1, tlim, dtout=0.25, cfl=0.3, mD=mD)
+    ((sg, st),) = e.run(tlim, dtout=0.25, cfl=0.3, mD=mD, chunk=100)
+    assert st["steps"] == steps
+    assert st["t"] == pytest.approx(tf, rel=1e-14)
+    assert sg.shape == so.shape
+    assert np.max(np.abs(sg[:, 0] - so[:, 0])) < 1e-12
+    assert relerr(sg[:, 2], so[:, 2]) < 1e-9
+    assert abs(st["c2_final"] - c2f) / c2f < 1e-9
+    errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
+    print(f"  {steps} steps, rel errors:", errs, flush=True)
+    assert max(errs) < 1e-9
+
+
+def test_mass_conservation_gpu(dfk):
+    Nu, Nphi = 64, 96
+    print("case mass conservation", flush=True)
+    e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, selfgrav=1))
+    e.set_state(*rough_state(Nu, Nphi, seed=21))
+    m0 = e.disk_mass()
+    e.run_init(1e9, cfl=0.3, mD=m0, max_steps=50)
+    e.advance(50)
+    assert abs(e.disk_mass() - m0) / m0 < 1e-13
+
+
+def test_ensemble_matches_single_disks(orc, dfk):
+    """Disks of different shapes in one handle give the same bits as each disk
+    run alone (identical kernels per disk)."""
+    print("case ensemble vs single", flush=True)
+    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 30, 0.04)]
+    params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
+    ens = dfk.Ensemble(params)
+    ics, mDs = [], []
+    for d, (a, b, c) in enumerate(shapes):
+        h, _ = _lka98(orc, a, b, c)
+        ics.append(h.state())
+        mDs.append(h.disk_mass())
+        ens.set_state(*ics[-1], d=d)
+    ens.run_init(1e9, mD=mDs, max_steps=[20, 35, 27])
+    ens.advance(40)
+    for d, p in enumerate(params):
+        one = dfk.Ensemble(p)
+        one.set_state(*ics[d])
+        one.run_init(1e9, mD=mDs[d], max_steps=[20, 35, 27][d])
+        one.advance(40)
+        assert ens.status(d)["steps"] == [20, 35, 27][d]
+        for a, b in zip(ens.get_state(d), one.get_state()):
+            assert np.array_equal(a, b)
+
+
+def test_nonfinite_flag_stops_disk(dfk):
+    Nu, Nphi = 32, 32
+    print("case nonfinite flag", flush=True)
+    e = dfk.Ensemble([dfk.disk_params(Nu, Nphi), dfk.disk_params(Nu, Nphi)])
+    st = rough_state(Nu, Nphi, seed=2)
+    e.set_state(*st, d=1)
+    bad = [x.copy() for x in st]
+    bad[0][5, 5] = np.nan
+    e.set_state(*bad, d=0)
+    e.run_init(1e9, mD=1.0, max_steps=20, stop_on_nonfinite=1)
+    e.advance(25)
+    s0, s1 = e.status(0), e.status(1)
+    assert s0["flags"] & 1 and s0["done"] and s0["steps"] == 0
+    assert s1["flags"] == 0 and s1["steps"] == 20
+
+
+def test_error_paths(dfk):
+    print("case error paths", flush=True)
+    with pytest.raises(dfk.DiskfftError) as ei:
+        dfk.Ensemble(dfk.disk_params(2, 16))
+    assert ei.value.code == dfk.DFK_EINVAL
+    with pytest.raises(dfk.DiskfftError) as ei:
+        dfk.Ensemble(dfk.disk_params(16, 34))      # 34 = 2 * 17
+    assert ei.value.code == dfk.DFK_EUNSUPPORTED
+    e = dfk.Ensemble(dfk.disk_params(16, 16))
+    with pytest.raises(dfk.DiskfftError) as ei:
+        e.advance(1)
+    assert ei.value.code == dfk.DFK_ESTATE
