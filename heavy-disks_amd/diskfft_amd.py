@@ -379,7 +379,7 @@ def model_ic(m, Nu, Nphi, eps, selfgrav=1, potential_forces=None, device=0):
             tmp.close()
         else:
             gR, _ = potential_forces(Sax)
-        gRdisk = np.array([np.sum(gR[i]) / Nphi for i in range(Nu)])   # diskmain.cpp:44-46
+        gRdisk = np.cumsum(gR, axis=1)[:, -1] / Nphi   # serial sum as diskmain.cpp:44-46
     S, U, W, Q = model_state(m, Nu, Nphi, gRdisk)
     p = disk_params(Nu, Nphi, Rin=m.Rin, Rout=m.Rout, K=m.K, gamma=m.gamma, gmstar=m.gmstar,
                     eps=eps, G=m.G, selfgrav=selfgrav)
