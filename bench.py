@@ -27,6 +27,8 @@ sys.path.insert(0, os.path.join(ROOT, "heavy-disks_amd"))
 
 import numpy as np  # noqa: E402
 
+import autotune  # noqa: E402
+
 METRIC = "zone-updates/s incl. FFT gravity, 256² single vs 64-disk batch; % HBM peak"
 HBM_PEAK = 8.0e12          # B/s, MI355X
 REF_PUBLISHED = 6.12e7     # reference README: 256^2, FFTW + OpenMP 8 threads (CPU)
@@ -56,6 +58,7 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk):
     m = dfk.model_defaults(dfk.MODEL_POWERLAW)
     p, S, U, W, mD, _ = dfk.model_ic(m, Nu, Nphi, eps)
     e = dfk.Ensemble(p)
+    tuned = autotune.apply(e, regime="latency")
     e.set_state(S, U, W)
     e.set_graph_steps(chunk)
     e.run_init(1e30, dtout=0.25, cfl=cfl, mD=mD, sample_cap=chunk + 8)
@@ -67,8 +70,11 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk):
     st = e.status()
     assert st["steps"] == warmup + steps and not st["flags"], st
     prof = e.profile_step(200)
+    shapes = dict(zip(("ti", "tj", "pr", "cb"), e.get_shapes(0)))
+    shapes["threads"] = e.get_threads()
+    shapes["tuned"] = bool(tuned)
     e.close()
-    return dict(p=p, S=S, U=U, W=W, mD=mD, wall=wall, dev_ms=dev_ms, prof=prof)
+    return dict(p=p, S=S, U=U, W=W, mD=mD, wall=wall, dev_ms=dev_ms, prof=prof, shapes=shapes)
 
 
 def cpu_baseline(p, S, U, W, cfl, seconds, threads):
@@ -95,6 +101,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
     import ensemble_cfg
     disks = ensemble_cfg.draw(64, seed=2026)
     ens, ics = ensemble_cfg.build(dfk, disks)
+    ntuned = autotune.apply(ens, regime="throughput")
     zones = sum(d["Nu"] * d["Nphi"] for d in disks)
     ens.set_graph_steps(50)
     ens.run_init(1e30, cfl=0.4, mD=[ic[3] for ic in ics], sample_cap=64)
@@ -108,6 +115,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
     seq_time, seq_zones = 0.0, 0
     for d, ic in zip(disks, ics):
         e = dfk.Ensemble(ic[4])
+        autotune.apply(e, regime="latency")
         e.set_state(ic[0], ic[1], ic[2])
         e.set_graph_steps(50)
         e.run_init(1e30, cfl=0.4, mD=ic[3], sample_cap=64)
@@ -122,7 +130,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
     return {"disks": 64, "zones": zones, "steps": steps_batch,
             "batched_zone_updates_per_s": batched,
             "sequential_zone_updates_per_s": seq,
-            "speedup_vs_sequential": batched / seq,
+            "speedup_vs_sequential": batched / seq, "tuned_disks": ntuned,
             "hbm_frac_alg": batched * balg / HBM_PEAK}
 
 
@@ -176,6 +184,7 @@ def main():
         "hbm_frac_step": zps * b_alg(Nu, Nphi) / (Nu * Nphi) / HBM_PEAK,
         "device_ms_per_step": r["dev_ms"] / args.steps,
         "kernel_ms": kms,
+        "shapes": r["shapes"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": ach / (HBM_PEAK / 1e9), "traffic": None,
                      "alg_bytes_per_launch": kb[dom]},

@@ -1,0 +1,154 @@
+"""Shape autotuner for the loop kernels.
+
+The knobs (see dfk_set_shapes in include/diskfft.h) act on separate kernels:
+  (ti, tj)  output tile of the fused update kernel
+  pr        complex rows (two real rows each) per workgroup of the row r2c/c2r
+  cb        columns per workgroup of the column convolution
+so each is tuned on its own kernel's time, measured with HIP events on the
+library's stream (dfk_profile_step).  Two regimes:
+  "latency"     one disk alone (few workgroups; launch/latency bound)
+  "throughput"  enough copies of the disk to hold >= 2M zones (bandwidth bound),
+                which is what a disk sees inside a large batched ensemble.
+Results are cached in a JSON file keyed by (grid, regime, device).
+
+    python heavy-disks_amd/autotune.py 256x256 512x512 --regime throughput
+"""
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CACHE = os.path.join(HERE, "tuned_shapes.json")
+LDS_MAX = 64 * 1024
+
+
+def candidates(Nu, Nphi):
+    M = 2 * Nu
+    tiles = [(ti, tj) for ti in (2, 4, 8, 16, 32) for tj in (16, 32, 64, 128, 256)
+             if tj <= Nphi and ti <= Nu
+             and 8 * (3 * (ti + 4) * (tj + 4) + 3 * (ti + 1) * (tj + 4)) <= LDS_MAX]
+    prs = [p for p in (1, 2, 4, 8, 16) if 2 * p * Nphi * 16 <= LDS_MAX and 2 * p <= Nu]
+    cbs = [c for c in (1, 2, 4, 8, 16) if 2 * c * M * 16 <= LDS_MAX and c <= Nphi // 2 + 1]
+    return tiles, prs, cbs
+
+
+def _key(Nu, Nphi, regime, device_name):
+    return f"{Nu}x{Nphi}:{regime}:{device_name}"
+
+
+def device_name():
+    """This build targets gfx950 only; the cache key keeps the field so tuned
+    shapes from another architecture are never applied by mistake."""
+    return "gfx950"
+
+
+def load_cache(path=CACHE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(Nu, Nphi, regime="throughput", path=CACHE, device="gfx950"):
+    """Tuned (ti, tj, pr, cb) for a grid, or None."""
+    c = load_cache(path).get(_key(Nu, Nphi, regime, device))
+    return None if c is None else (c["ti"], c["tj"], c["pr"], c["cb"])
+
+
+def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
+    """Set every disk of an Ensemble to its cached shapes (where present), and
+    the handle's threads per workgroup to the most common tuned choice."""
+    cache = load_cache(path)
+    n, votes = 0, {}
+    for d in range(ens.n):
+        c = cache.get(_key(*ens.shape(d), regime, device))
+        if c:
+            ens.set_shapes(c["ti"], c["tj"], c["pr"], c["cb"], d=d)
+            if "threads" in c:
+                t = tuple(c["threads"])
+                votes[t] = votes.get(t, 0) + ens.shape(d)[0] * ens.shape(d)[1]
+            n += 1
+    if votes:
+        ens.set_threads(*max(votes, key=votes.get))
+    return n
+
+
+def _handle(dfk, Nu, Nphi, copies):
+    m = dfk.model_defaults(dfk.MODEL_LKA98)
+    sig0 = dfk.model_sigma0(m, Nu, Nphi)
+    S, U, W, _ = dfk.model_state(m, Nu, Nphi, None)
+    p = dfk.disk_params(Nu, Nphi, eps=0.01)
+    e = dfk.Ensemble([p] * copies)
+    for d in range(copies):
+        e.set_state(S, U, W, d=d)
+    mD = dfk.model_mass(m, S)
+    del sig0
+    return e, mD
+
+
+def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
+    copies = 1 if regime == "latency" else max(1, (2 << 20) // (Nu * Nphi))
+    e, mD = _handle(dfk, Nu, Nphi, copies)
+    tiles, prs, cbs = candidates(Nu, Nphi)
+    base = e.get_shapes(0)
+
+    def prof(ti, tj, pr, cb):
+        e.set_shapes(ti, tj, pr, cb)
+        e.run_init(1e30, cfl=0.3, mD=mD)
+        return e.profile_step(nrep)
+
+    res = {"tile": {}, "pr": {}, "cb": {}}
+    for ti, tj in tiles:
+        res["tile"][f"{ti}x{tj}"] = float(prof(ti, tj, base[2], base[3])[4])
+    for pr in prs:
+        ms = prof(base[0], base[1], pr, base[3])
+        res["pr"][str(pr)] = float(ms[1] + ms[3])
+    for cb in cbs:
+        res["cb"][str(cb)] = float(prof(base[0], base[1], base[2], cb)[2])
+    bt = min(res["tile"], key=res["tile"].get)
+    bpr = int(min(res["pr"], key=res["pr"].get))
+    bcb = int(min(res["cb"], key=res["cb"].get))
+    ti, tj = (int(x) for x in bt.split("x"))
+    # threads per workgroup of each kernel, at the chosen shapes
+    res["threads"] = {}
+    for t in (64, 128, 256):
+        e.set_threads(t, t, t)
+        res["threads"][str(t)] = [float(x) for x in prof(ti, tj, bpr, bcb)]
+    pick = lambda idx: int(min(res["threads"], key=lambda k: res["threads"][k][idx]))  # noqa: E731
+    thr = (pick(1), pick(2), pick(4))
+    e.set_threads(*thr)
+    best = prof(ti, tj, bpr, bcb)
+    e.close()
+    out = {"ti": ti, "tj": tj, "pr": bpr, "cb": bcb, "threads": list(thr), "copies": copies,
+           "ms_per_kernel": [float(x) for x in best],
+           "default": {"shape": list(base), "tile_ms": res["tile"].get(f"{base[0]}x{base[1]}")},
+           "sweep": res}
+    if verbose:
+        print(f"{Nu}x{Nphi} {regime} x{copies}: tile {ti}x{tj} pr {bpr} cb {bcb} threads {thr} "
+              f"ms {np.round(best, 4).tolist()}", flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("grids", nargs="+", help="NuxNphi")
+    ap.add_argument("--regime", default="throughput", choices=["latency", "throughput"])
+    ap.add_argument("--cache", default=CACHE)
+    ap.add_argument("--nrep", type=int, default=30)
+    args = ap.parse_args()
+    sys.path.insert(0, HERE)
+    import diskfft_amd as dfk
+    cache = load_cache(args.cache)
+    dev = device_name()
+    for g in args.grids:
+        Nu, Nphi = (int(x) for x in g.lower().split("x"))
+        cache[_key(Nu, Nphi, args.regime, dev)] = tune(dfk, Nu, Nphi, args.regime, args.nrep)
+        with open(args.cache, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

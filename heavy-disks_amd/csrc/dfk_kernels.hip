@@ -13,7 +13,8 @@
 //               MUSCL sweep (:99-140) + azimuthal sweep (:143-179) + floor
 //               (:68-69) fused on an LDS tile; epilogue writes the next step's
 //               CFL maximum and the C_0/C_2 partial sums per tile.
-// All FFTs are this file's own LDS Stockham transforms (radix 4/2/3/5/7/11/13).
+// All FFTs are this file's own LDS Stockham transforms (radix 4, 2, 3 unrolled;
+// 5, 7, 11, 13 by a direct small DFT).
 #include <hip/hip_runtime.h>
 
 #include "dfk_internal.h"
@@ -24,9 +25,19 @@ typedef long long i64;
 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
-// hydro.hpp:16-18
+// 1/x to about 1 ulp: the hardware reciprocal estimate refined by two Newton
+// steps (x finite, nonzero and normal where it is used).
+__device__ __forceinline__ double frcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// hydro.hpp:16-18.  a*b > 0 keeps a+b away from zero and from denormals.
 __device__ __forceinline__ double vanleer(double a, double b) {
-  return (a * b > 0.0) ? 2.0 * a * b / (a + b) : 0.0;
+  return (a * b > 0.0) ? (2.0 * a * b) * frcp(a + b) : 0.0;
 }
 
 __device__ __forceinline__ double pres(const DiskDev &D, double s) {
@@ -54,9 +65,50 @@ __device__ __forceinline__ const double *gtab(const double *geom, const DiskDev 
 }
 
 // ------------------------------------------------------------------ FFT
-// One Stockham stage of radix R over `batch` transforms of length n stored
-// contiguously in LDS (a -> b).  Ns = product of the radices already applied.
-// Twiddles: tw[t] = exp(-2 pi i t / n); SIGN > 0 uses the conjugate.
+// Quotient of small non-negative ints (a < 2^22, b <= 2^12) through a float
+// reciprocal: (a + 0.5) / b is at least 0.5/b away from an integer, which the
+// float rounding error (< 2^-22 relative) cannot cross.
+__device__ __forceinline__ int qdiv(int a, float rb) { return (int)(((float)a + 0.5f) * rb); }
+
+__device__ __forceinline__ double2 twid(const double2 *__restrict__ tw, int t, int sign) {
+  double2 w = tw[t];
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+// In-register butterflies (forward for SIGN < 0).
+template <int SIGN>
+__device__ __forceinline__ void bfly2(double2 &a, double2 &b) {
+  const double2 t = csub(a, b);
+  a = cadd(a, b);
+  b = t;
+}
+template <int SIGN>
+__device__ __forceinline__ void bfly4(double2 (&v)[4]) {
+  const double sg = (SIGN < 0) ? 1.0 : -1.0;
+  const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  const double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+  const double2 t = make_double2(sg * b1.y, -sg * b1.x);   // -i*b1 forward, +i*b1 inverse
+  v[0] = cadd(a0, b0);
+  v[1] = cadd(a1, t);
+  v[2] = csub(a0, b0);
+  v[3] = csub(a1, t);
+}
+template <int SIGN>
+__device__ __forceinline__ void bfly3(double2 (&v)[3]) {
+  // W = exp(SIGN 2 pi i / 3) = -1/2 + i s,  s = SIGN sqrt(3)/2
+  const double s = (SIGN < 0) ? -0.86602540378443864676 : 0.86602540378443864676;
+  const double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+  const double2 c = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+  const double2 is = make_double2(-s * t2.y, s * t2.x);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(c, is);
+  v[2] = csub(c, is);
+}
+
+// One Stockham stage of radix R in {2, 3, 4} over `batch` transforms of
+// length n stored contiguously in LDS (a -> b).  Ns = product of the radices
+// already applied.  Twiddles tw[t] = exp(-2 pi i t / n), conjugated for SIGN > 0.
 template <int R, int SIGN>
 __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2 *__restrict__ b,
                                           int n, int batch, int Ns,
@@ -64,71 +116,75 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
   const int nb = n / R;
   const int tstep = n / (Ns * R);
   const int total = nb * batch;
-  const int rstep = n / R;
+  const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int p = e / nb;
+    const int p = qdiv(e, rnb);
     const int j = e - p * nb;
-    const int k = j % Ns;
+    const int jq = qdiv(j, rNs);
+    const int k = j - jq * Ns;
     const double2 *src = a + p * n + j;
-    double2 *dst = b + p * n + (j / Ns) * Ns * R + k;
+    double2 *dst = b + p * n + jq * Ns * R + k;
     double2 v[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       double2 x = src[m * nb];
-      if (m > 0 && k > 0) {
-        double2 w = tw[k * m * tstep];
-        if (SIGN > 0) w.y = -w.y;
-        x = cmul(x, w);
-      }
+      if (m > 0 && k > 0) x = cmul(x, twid(tw, k * m * tstep, SIGN));
       v[m] = x;
     }
-    if (R == 2) {
-      dst[0] = cadd(v[0], v[1]);
-      dst[Ns] = csub(v[0], v[1]);
-    } else if (R == 4) {
-      const double sg = (SIGN < 0) ? 1.0 : -1.0;
-      double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-      double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
-      double2 t = make_double2(sg * b1.y, -sg * b1.x);   // -i*b1 (forward), +i*b1 (inverse)
-      dst[0] = cadd(a0, b0);
-      dst[Ns] = cadd(a1, t);
-      dst[2 * Ns] = csub(a0, b0);
-      dst[3 * Ns] = csub(a1, t);
-    } else {
+    if constexpr (R == 2) bfly2<SIGN>(v[0], v[1]);
+    if constexpr (R == 3) bfly3<SIGN>(v);
+    if constexpr (R == 4) bfly4<SIGN>(v);
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        double2 s = v[0];
-#pragma unroll
-        for (int m = 1; m < R; ++m) {
-          double2 w = tw[((m * q) % R) * rstep];
-          if (SIGN > 0) w.y = -w.y;
-          double2 c = cmul(v[m], w);
-          s.x += c.x;
-          s.y += c.y;
-        }
-        dst[q * Ns] = s;
+    for (int q = 0; q < R; ++q) dst[q * Ns] = v[q];
+  }
+}
+
+// Stage of any other prime radix R: direct R-point DFT reading its inputs
+// from LDS (few registers; used only for lengths with factors 5, 7, 11, 13).
+template <int SIGN>
+__device__ void fft_stage_generic(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
+                                  int batch, int Ns, int R, const double2 *__restrict__ tw) {
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  const int rstep = n / R;
+  const int total = nb * batch;
+  const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int p = qdiv(e, rnb);
+    const int j = e - p * nb;
+    const int jq = qdiv(j, rNs);
+    const int k = j - jq * Ns;
+    const double2 *src = a + p * n + j;
+    double2 *dst = b + p * n + jq * Ns * R + k;
+    for (int q = 0; q < R; ++q) {
+      const int base = k * tstep + q * rstep;   // exponent step per input m
+      double2 s = src[0];
+      int t = 0;
+      for (int m = 1; m < R; ++m) {
+        t += base;
+        if (t >= n) t -= n;
+        if (t >= n) t -= n;
+        const double2 c = cmul(src[m * nb], twid(tw, t, SIGN));
+        s.x += c.x;
+        s.y += c.y;
       }
+      dst[q * Ns] = s;
     }
   }
 }
 
-// Full transform: returns the LDS buffer holding the result (a or b).
+// Full transform of `batch` rows of length n in LDS: returns the buffer that
+// holds the result (a or b).  One barrier per stage.
 template <int SIGN>
 __device__ double2 *lds_fft(double2 *a, double2 *b, int n, int batch, int nst,
                             const int *__restrict__ rad, const double2 *__restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s < nst; ++s) {
     const int r = rad[s];
-    switch (r) {
-      case 4: fft_stage<4, SIGN>(a, b, n, batch, Ns, tw); break;
-      case 2: fft_stage<2, SIGN>(a, b, n, batch, Ns, tw); break;
-      case 3: fft_stage<3, SIGN>(a, b, n, batch, Ns, tw); break;
-      case 5: fft_stage<5, SIGN>(a, b, n, batch, Ns, tw); break;
-      case 7: fft_stage<7, SIGN>(a, b, n, batch, Ns, tw); break;
-      case 11: fft_stage<11, SIGN>(a, b, n, batch, Ns, tw); break;
-      case 13: fft_stage<13, SIGN>(a, b, n, batch, Ns, tw); break;
-      default: break;
-    }
+    if (r == 4) fft_stage<4, SIGN>(a, b, n, batch, Ns, tw);
+    else if (r == 2) fft_stage<2, SIGN>(a, b, n, batch, Ns, tw);
+    else if (r == 3) fft_stage<3, SIGN>(a, b, n, batch, Ns, tw);
+    else fft_stage_generic<SIGN>(a, b, n, batch, Ns, r, tw);
     __syncthreads();
     double2 *t = a;
     a = b;
@@ -202,7 +258,7 @@ k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
   if (C.done) return;
   const int nt = D.ntile_i * D.ntile_j;
   double mx = 1e-30, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int t = threadIdx.x; t < nt; t += WG) {
+  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
     double v = pcfl[D.part + t];
     mx = (v > mx) ? v : mx;
     s0 += pcm[4 * (D.part + t) + 0];
@@ -266,7 +322,7 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
                *dA = gtab(geom, D, GT_DA);
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  for (int e = threadIdx.x; e < D.ti * D.tj; e += WG) {
+  for (int e = threadIdx.x; e < D.ti * D.tj; e += blockDim.x) {
     const int li = e / D.tj, lj = e - li * D.tj;
     const int i = i0 + li, j = j0 + lj;
     if (i >= Nu || j >= Nphi) continue;
@@ -305,7 +361,7 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   double2 *A = lds2, *B = lds2 + pr * n;
   const double *S = stp(st, ntot, C.src, 0, D);
   const double *R32 = gtab(geom, D, GT_R32);
-  for (int e = threadIdx.x; e < pr * n; e += WG) {
+  for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     double xa = 0.0, xb = 0.0;
@@ -317,7 +373,7 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
   double2 *out = spec + D.spo;
   const int w2 = 2 * pr;
-  for (int e = threadIdx.x; e < w2 * Nc; e += WG) {
+  for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
     const int k = e / w2, rr = e - k * w2;
     const int p = rr >> 1, isb = rr & 1;
     const int row = row0 + rr;
@@ -346,7 +402,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   const int k0 = wm.y * cb;
   double2 *A = lds2, *B = lds2 + cb * M;
   double2 *io = spec + D.spo;
-  for (int e = threadIdx.x; e < cb * M; e += WG) {
+  for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
     double2 v = make_double2(0.0, 0.0);
@@ -357,7 +413,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
   double2 *Wb = (Z == A) ? B : A;
   const double *kh = khat + D.kho;
-  for (int e = threadIdx.x; e < cb * M; e += WG) {
+  for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
     const int pp = (p <= Nu) ? p : M - p;
@@ -369,7 +425,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   }
   __syncthreads();
   const double2 *Y = lds_fft<+1>(Z, Wb, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
-  for (int e = threadIdx.x; e < cb * Nu; e += WG) {
+  for (int e = threadIdx.x; e < cb * Nu; e += blockDim.x) {
     const int c = e / Nu, p = e - c * Nu;
     const int k = k0 + c;
     if (k < Nc) io[(i64)k * Nu + p] = Y[c * M + p];
@@ -391,7 +447,7 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   const int row0 = 2 * pr * wm.y;
   double2 *A = lds2, *B = lds2 + pr * n;
   const double2 *in = spec + D.spo;
-  for (int e = threadIdx.x; e < pr * Nc; e += WG) {
+  for (int e = threadIdx.x; e < pr * Nc; e += blockDim.x) {
     const int k = e / pr, p = e - k * pr;
     const int ra = row0 + 2 * p, rb = ra + 1;
     double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
@@ -405,7 +461,7 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
   const double *sc = gtab(geom, D, GT_PHISC);
   double *P = phi + D.so;
-  for (int e = threadIdx.x; e < 2 * pr * n; e += WG) {
+  for (int e = threadIdx.x; e < 2 * pr * n; e += blockDim.x) {
     const int rr = e / n, j = e - rr * n;
     const int row = row0 + rr;
     if (row >= Nu) continue;
@@ -426,7 +482,7 @@ k_khat_rows(const DiskDev *__restrict__ D_, const double *__restrict__ ktab,
   const int n = D.Nphi, M = D.M, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * blockIdx.x;
   double2 *A = lds2, *B = lds2 + pr * n;
-  for (int e = threadIdx.x; e < pr * n; e += WG) {
+  for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     A[e] = make_double2(ra < M ? ktab[(i64)ra * n + j] : 0.0, rb < M ? ktab[(i64)rb * n + j] : 0.0);
@@ -434,7 +490,7 @@ k_khat_rows(const DiskDev *__restrict__ D_, const double *__restrict__ ktab,
   __syncthreads();
   const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
   const int w2 = 2 * pr;
-  for (int e = threadIdx.x; e < w2 * Nc; e += WG) {
+  for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
     const int k = e / w2, rr = e - k * w2;
     const int p = rr >> 1, isb = rr & 1;
     const int row = row0 + rr;
@@ -457,14 +513,14 @@ k_khat_cols(const DiskDev *__restrict__ D_, const double2 *__restrict__ tw,
   const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
   const int k0 = blockIdx.x * cb;
   double2 *A = lds2, *B = lds2 + cb * M;
-  for (int e = threadIdx.x; e < cb * M; e += WG) {
+  for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
     A[e] = (k < Nc) ? tmp[(i64)k * M + p] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   const double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
-  for (int e = threadIdx.x; e < cb * M; e += WG) {
+  for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
     if (k >= Nc) continue;
@@ -511,7 +567,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   const bool sg = D.selfgrav != 0;
 
   // ---- phase 1: source (+ forces) on rows i0-2..i0+TI+1, cols j0-2..j0+TJ+1
-  for (int e = threadIdx.x; e < SH * SW; e += WG) {
+  for (int e = threadIdx.x; e < SH * SW; e += blockDim.x) {
     const int lr = e / SW, lc = e - lr * SW;
     const int i = i0 - 2 + lr;
     if (i < 0 || i >= Nu) continue;
@@ -547,7 +603,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   const double *tRf = gtab(geom, D, GT_RF), *tInvdR = gtab(geom, D, GT_INVDR),
                *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL);
   (void)tRf;
-  for (int e = threadIdx.x; e < (TI + 1) * SW; e += WG) {
+  for (int e = threadIdx.x; e < (TI + 1) * SW; e += blockDim.x) {
     const int lf = e / SW, lc = e - lf * SW;
     const int f = i0 + lf;
     double fm = 0.0, fu = 0.0, fw = 0.0;
@@ -579,7 +635,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   __syncthreads();
 
   // ---- phase 3: radial gather for rows i0..i0+TI-1 (in place into s*)
-  for (int e = threadIdx.x; e < TI * SW; e += WG) {
+  for (int e = threadIdx.x; e < TI * SW; e += blockDim.x) {
     const int lr = e / SW, lc = e - lr * SW;
     const int i = i0 + lr;
     if (i >= Nu) continue;
@@ -598,7 +654,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   // ---- phase 4: azimuthal faces j = j0..j0+TJ (between j-1 and j), rows i0..i0+TI-1
   const double *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE);
   const int FW = TJ + 1;
-  for (int e = threadIdx.x; e < TI * FW; e += WG) {
+  for (int e = threadIdx.x; e < TI * FW; e += blockDim.x) {
     const int lr = e / FW, lf = e - lr * FW;
     const int i = i0 + lr;
     double fm = 0.0, fu = 0.0, fw = 0.0;
@@ -632,7 +688,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   const double *dA = gtab(geom, D, GT_DA);
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-  for (int e = threadIdx.x; e < TI * TJ; e += WG) {
+  for (int e = threadIdx.x; e < TI * TJ; e += blockDim.x) {
     const int lr = e / TJ, lc = e - lr * TJ;
     const int i = i0 + lr, j = j0 + lc;
     if (i >= Nu || j >= Nphi) continue;
@@ -814,43 +870,44 @@ hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const 
   return hipGetLastError();
 }
 
-hipError_t dfk_launch_r2c(hipStream_t s, int nblk, size_t lds, const DiskDev *disks, const int2 *map,
-                          int base, const DiskCtl *ctl, const double *st, i64 ntot,
+hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
+                          const int2 *map, int base, const DiskCtl *ctl, const double *st, i64 ntot,
                           const double *geom, const double2 *tw, double2 *spec) {
   if (nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rows_r2c, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, st, ntot, geom,
+  hipLaunchKernelGGL(k_rows_r2c, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot, geom,
                      tw, spec);
   return hipGetLastError();
 }
 
-hipError_t dfk_launch_conv(hipStream_t s, int nblk, size_t lds, const DiskDev *disks, const int2 *map,
-                           int base, const DiskCtl *ctl, const double *khat, const double2 *tw,
-                           double2 *spec) {
+hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
+                           const int2 *map, int base, const DiskCtl *ctl, const double *khat,
+                           const double2 *tw, double2 *spec) {
   if (nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cols_conv, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, khat, tw, spec);
+  hipLaunchKernelGGL(k_cols_conv, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec);
   return hipGetLastError();
 }
 
-hipError_t dfk_launch_c2r(hipStream_t s, int nblk, size_t lds, const DiskDev *disks, const int2 *map,
-                          int base, const DiskCtl *ctl, const double2 *spec, const double *geom,
-                          const double2 *tw, double *phi) {
+hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
+                          const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
+                          const double *geom, const double2 *tw, double *phi) {
   if (nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rows_c2r, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, spec, geom, tw,
+  hipLaunchKernelGGL(k_rows_c2r, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, spec, geom, tw,
                      phi);
   return hipGetLastError();
 }
 
-hipError_t dfk_launch_update(hipStream_t s, int nblk, size_t lds, int garr, const DiskDev *disks,
+hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int garr,
+                             const DiskDev *disks,
                              const int2 *map, int base, const DiskCtl *ctl, double *st, i64 ntot,
                              const double *phi, const double *gR, const double *gp, int has_gR,
                              int has_gp, const double *geom, const double *trig, double *pcfl,
                              double *pcm) {
   if (nblk <= 0) return hipSuccess;
   if (garr)
-    hipLaunchKernelGGL(k_update<true>, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, st, ntot,
+    hipLaunchKernelGGL(k_update<true>, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot,
                        phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
   else
-    hipLaunchKernelGGL(k_update<false>, dim3(nblk), dim3(WG), lds, s, disks, map, base, ctl, st, ntot,
+    hipLaunchKernelGGL(k_update<false>, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot,
                        phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
   return hipGetLastError();
 }

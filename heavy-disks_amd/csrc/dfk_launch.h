@@ -12,16 +12,16 @@ hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, Di
 hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const int2 *map,
                            int base, const DiskCtl *ctl, const double *st, long long ntot,
                            const double *geom, const double *trig, double *pcfl, double *pcm);
-hipError_t dfk_launch_r2c(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double *st,
                           long long ntot, const double *geom, const double2 *tw, double2 *spec);
-hipError_t dfk_launch_conv(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                            const int2 *map, int base, const DiskCtl *ctl, const double *khat,
                            const double2 *tw, double2 *spec);
-hipError_t dfk_launch_c2r(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
                           const double *geom, const double2 *tw, double *phi);
-hipError_t dfk_launch_update(hipStream_t s, int nblk, size_t lds, int garr, const DiskDev *disks,
+hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int garr, const DiskDev *disks,
                              const int2 *map, int base, const DiskCtl *ctl, double *st,
                              long long ntot, const double *phi, const double *gR,
                              const double *gp, int has_gR, int has_gp, const double *geom,

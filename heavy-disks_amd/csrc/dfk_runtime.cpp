@@ -69,6 +69,23 @@ bool factor(int n, int *rad, int *nst) {
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Reorder a
+// work list so that XCD x receives one contiguous chunk of it: neighbouring
+// tiles (which share halo rows) then meet in the same L2.
+const int kXcds = 8;
+std::vector<int2> xcd_order(const std::vector<int2> &v) {
+  const int L = (int)v.size();
+  std::vector<int2> out(L);
+  const int q = L / kXcds, r = L % kXcds;
+  for (int b = 0; b < L; ++b) {
+    const int x = b % kXcds;
+    const int start = x * q + (x < r ? x : r);
+    out[b] = v[start + b / kXcds];
+  }
+  return out;
+}
+
 }  // namespace
 
 struct dfk_handle {
@@ -103,6 +120,7 @@ struct dfk_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   int chunk = 128;               // loop iterations per captured graph
+  int thr_rows = 256, thr_cols = 256, thr_upd = 256;   // threads per workgroup
 };
 
 namespace {
@@ -185,6 +203,24 @@ int build_layout(dfk_handle *h) {
   TRY(dalloc(&h->map_upd, (i64)mu.size()));
   TRY(dalloc(&h->map_r2c, (i64)mr.size()));
   TRY(dalloc(&h->map_conv, (i64)mc.size()));
+  // unit calls launch one disk's range [upd0[d], upd0[d+1]) of the map, so
+  // each disk's range is reordered on its own
+  auto per_disk = [&](std::vector<int2> &m, const std::vector<int> &first) {
+    for (int d = 0; d < h->nd; ++d) {
+      std::vector<int2> part(m.begin() + first[d], m.begin() + first[d + 1]);
+      part = xcd_order(part);
+      std::copy(part.begin(), part.end(), m.begin() + first[d]);
+    }
+  };
+  if (h->nd == 1) {
+    mu = xcd_order(mu);
+    mr = xcd_order(mr);
+    mc = xcd_order(mc);
+  } else {
+    per_disk(mu, h->upd0);
+    per_disk(mr, h->r2c0);
+    per_disk(mc, h->conv0);
+  }
   HIPCHK(hipMemcpy(h->map_upd, mu.data(), sizeof(int2) * mu.size(), hipMemcpyHostToDevice));
   if (!mr.empty()) HIPCHK(hipMemcpy(h->map_r2c, mr.data(), sizeof(int2) * mr.size(), hipMemcpyHostToDevice));
   if (!mc.empty()) HIPCHK(hipMemcpy(h->map_conv, mc.data(), sizeof(int2) * mc.size(), hipMemcpyHostToDevice));
@@ -223,11 +259,11 @@ double *field(dfk_handle *h, int buf, int f, int d) {
 // Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk.
 int launch_gravity_disk(dfk_handle *h, int d) {
   const int nr = h->r2c0[d + 1] - h->r2c0[d], nc = h->conv0[d + 1] - h->conv0[d];
-  HIPCHK(dfk_launch_r2c(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->st,
+  HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->st,
                         h->ntot, h->geom, h->tw, h->spec));
-  HIPCHK(dfk_launch_conv(h->stream, nc, h->lds_conv, h->dD, h->map_conv, h->conv0[d], h->ctl,
+  HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[d], h->ctl,
                          h->khat, h->tw, h->spec));
-  HIPCHK(dfk_launch_c2r(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
+  HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
                         h->geom, h->tw, h->phi));
   return DFK_OK;
 }
@@ -237,14 +273,14 @@ int launch_step_all(dfk_handle *h) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
   HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap));
   if (nr > 0) {
-    HIPCHK(dfk_launch_r2c(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
+    HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
                           h->geom, h->tw, h->spec));
-    HIPCHK(dfk_launch_conv(h->stream, nc, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
+    HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
                            h->tw, h->spec));
-    HIPCHK(dfk_launch_c2r(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
+    HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
                           h->geom, h->tw, h->phi));
   }
-  HIPCHK(dfk_launch_update(h->stream, nu, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
+  HIPCHK(dfk_launch_update(h->stream, nu, h->thr_upd, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
                            h->ntot, h->phi, nullptr, nullptr, 0, 0, h->geom, h->trig, h->pcfl,
                            h->pcm));
   return DFK_OK;
@@ -504,15 +540,24 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
     D.kho = kho; kho += (i64)(p.Nu + 1) * D.Nc;
     D.go = go; go += (i64)GT_COUNT * (p.Nu + 1);
     D.cos_off = (int)co; co += 2 * (i64)p.Nphi;
-    // default shapes
-    D.ti = 8;
-    D.tj = 64;
-    D.pr = std::max(1, 1024 / D.Nphi);
-    D.cb = std::max(1, 1024 / D.M);
     h->maxN = std::max(h->maxN, (i64)p.Nu * p.Nphi);
     h->maxF = std::max(h->maxF, (i64)(p.Nu + 1) * p.Nphi);
   }
   h->ntot = so; h->nspec = spo; h->nkhat = kho; h->ngeom = go; h->ntrig = co; h->ntw = (i64)twall.size();
+  // Default shapes from the autotuner's sweeps on MI355X: a handle with few
+  // zones is launch/latency bound and wants many small workgroups; a large
+  // (batched) one is bandwidth bound and wants bigger tiles and batches.
+  const bool small = h->ntot < (1LL << 19);
+  for (int d = 0; d < ndisks; ++d) {
+    DiskDev &D = h->hD[d];
+    if (small) {
+      D.ti = 8; D.tj = 16; D.pr = 1; D.cb = 1;
+    } else {
+      D.ti = 16; D.tj = 32;
+      D.pr = (D.Nphi <= 192) ? 4 : (D.Nphi <= 512 ? 2 : 1);
+      D.cb = (D.M <= 384) ? 2 : 1;
+    }
+  }
   int rc;
   if ((rc = dalloc(&h->dD, ndisks)) || (rc = dalloc(&h->st, 6 * h->ntot)) ||
       (rc = dalloc(&h->phi, h->ntot)) || (rc = dalloc(&h->gR, h->ntot)) ||
@@ -674,7 +719,7 @@ int dfk_step(dfk_handle *h, int d, double dt, const double *gR, const double *gp
   const int c = h->hcur[d];
   TRY(unit_ctl(h, d, c, 1 - c, dt));
   const int nb = h->upd0[d + 1] - h->upd0[d];
-  HIPCHK(dfk_launch_update(h->stream, nb, h->lds_upd, 1, h->dD, h->map_upd, h->upd0[d], h->ctl,
+  HIPCHK(dfk_launch_update(h->stream, nb, h->thr_upd, h->lds_upd, 1, h->dD, h->map_upd, h->upd0[d], h->ctl,
                            h->st, h->ntot, h->phi, h->gR, h->gp, gR != nullptr, gp != nullptr,
                            h->geom, h->trig, h->pcfl, h->pcm));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -837,6 +882,28 @@ int dfk_last_advance_ms(dfk_handle *h, double *ms) {
   return DFK_OK;
 }
 
+int dfk_set_threads(dfk_handle *h, int rows, int cols, int update) {
+  if (!h) { set_err("null handle"); return DFK_EINVAL; }
+  for (int v : {rows, cols, update})
+    if (v != 0 && (v < 64 || v > 256 || v % 64)) {
+      set_err("threads per workgroup must be 64, 128, 192 or 256");
+      return DFK_EINVAL;
+    }
+  if (rows) h->thr_rows = rows;
+  if (cols) h->thr_cols = cols;
+  if (update) h->thr_upd = update;
+  drop_graphs(h);
+  return DFK_OK;
+}
+
+int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update) {
+  if (!h) { set_err("null handle"); return DFK_EINVAL; }
+  if (rows) *rows = h->thr_rows;
+  if (cols) *cols = h->thr_cols;
+  if (update) *update = h->thr_upd;
+  return DFK_OK;
+}
+
 int dfk_set_graph_steps(dfk_handle *h, int nsteps) {
   if (!h || nsteps < 1) { set_err("bad argument"); return DFK_EINVAL; }
   if (nsteps != h->chunk) drop_graphs(h);
@@ -871,19 +938,19 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
           HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap));
           break;
         case 1:
-          HIPCHK(dfk_launch_r2c(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
+          HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
                                 h->geom, h->tw, h->spec));
           break;
         case 2:
-          HIPCHK(dfk_launch_conv(h->stream, nc, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
+          HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
                                  h->tw, h->spec));
           break;
         case 3:
-          HIPCHK(dfk_launch_c2r(h->stream, nr, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
+          HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
                                 h->geom, h->tw, h->phi));
           break;
         case 4:
-          HIPCHK(dfk_launch_update(h->stream, nu, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
+          HIPCHK(dfk_launch_update(h->stream, nu, h->thr_upd, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
                                    h->ntot, h->phi, nullptr, nullptr, 0, 0, h->geom, h->trig,
                                    h->pcfl, h->pcm));
           break;

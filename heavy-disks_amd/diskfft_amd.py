@@ -32,6 +32,7 @@ SYMBOLS = [
     "dfk_forces", "dfk_step", "dfk_stage", "dfk_khat", "dfk_modes", "dfk_disk_mass",
     "dfk_run_init", "dfk_advance", "dfk_samples", "dfk_get_status", "dfk_set_shapes",
     "dfk_get_shapes", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
+    "dfk_set_threads", "dfk_get_threads",
 ]
 
 
@@ -96,6 +97,8 @@ def lib():
             "dfk_last_advance_ms": (_I, [_P, _DP]),
             "dfk_set_graph_steps": (_I, [_P, _I]),
             "dfk_profile_step": (_I, [_P, _I, _DP]),
+            "dfk_set_threads": (_I, [_P, _I, _I, _I]),
+            "dfk_get_threads": (_I, [_P, _IP, _IP, _IP]),
         }
         for k, (res, args) in sig.items():
             f = getattr(L, k)
@@ -268,6 +271,14 @@ class Ensemble:
         ms = _D()
         _check(lib().dfk_last_advance_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+    def set_threads(self, rows=0, cols=0, update=0):
+        _check(lib().dfk_set_threads(self.h, rows, cols, update))
+
+    def get_threads(self):
+        v = [_I() for _ in range(3)]
+        _check(lib().dfk_get_threads(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def set_graph_steps(self, n):
         _check(lib().dfk_set_graph_steps(self.h, n))

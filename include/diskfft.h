@@ -127,6 +127,12 @@ int dfk_get_status(dfk_handle *h, int disk, dfk_status *st);
 int dfk_set_shapes(dfk_handle *h, int disk, int ti, int tj, int pr, int cb);
 int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
 
+/* Threads per workgroup (64, 128, 192 or 256; 0 keeps the current value) of
+ * the row transforms, the column convolution and the update kernel, for all
+ * disks of the handle (a batched launch has one workgroup size). */
+int dfk_set_threads(dfk_handle *h, int rows, int cols, int update);
+int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update);
+
 /* Device time in ms of the last dfk_advance (HIP events around the graph
  * launch on the handle's stream). */
 int dfk_last_advance_ms(dfk_handle *h, double *ms);
