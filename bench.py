@@ -165,6 +165,14 @@ def main():
     kb = kernel_bytes(Nu, Nphi)
     dom = max(kms, key=kms.get)
     ach = kb[dom] / (kms[dom] * 1e-3) / 1e9 if kms[dom] > 0 else 0.0
+    # HBM bytes per launch of that kernel from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (calibrated on a copy kernel), when they
+    # cover this workload
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_single_{Nu}x{Nphi}.json")
+    if os.path.exists(tpath):
+        kname = {"update": "k_update<false>"}.get(dom, "k_" + dom)
+        traffic = json.load(open(tpath))["kernels"].get(kname, {}).get("traffic_bytes")
     out = {
         "metric": METRIC,
         "value": zps,
@@ -186,7 +194,7 @@ def main():
         "kernel_ms": kms,
         "shapes": r["shapes"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": ach / (HBM_PEAK / 1e9), "traffic": None,
+                     "unit": "GB/s", "frac": ach / (HBM_PEAK / 1e9), "traffic": traffic,
                      "alg_bytes_per_launch": kb[dom]},
     }
     if not args.no_cpu:
