@@ -62,9 +62,15 @@ def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
     """Set every disk of an Ensemble to its cached shapes (where present), and
     the handle's threads per workgroup to the most common tuned choice."""
     cache = load_cache(path)
-    n, votes = 0, {}
-    for d in range(ens.n):
-        c = cache.get(_key(*ens.shape(d), regime, device))
+    n, votes, kinds = 0, {}, {}
+    found = [cache.get(_key(*ens.shape(d), regime, device)) for d in range(ens.n)]
+    for d, c in enumerate(found):
+        if c:
+            k = c.get("kind", 0)
+            kinds[k] = kinds.get(k, 0) + ens.shape(d)[0] * ens.shape(d)[1]
+    if kinds:
+        ens.set_update_kernel(max(kinds, key=kinds.get))
+    for d, c in enumerate(found):
         if c:
             ens.set_shapes(c["ti"], c["tj"], c["pr"], c["cb"], d=d)
             if "threads" in c:
@@ -100,18 +106,29 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
         e.run_init(1e30, cfl=0.3, mD=mD)
         return e.profile_step(nrep)
 
-    res = {"tile": {}, "pr": {}, "cb": {}}
+    res = {"tile": {}, "stream": {}, "pr": {}, "cb": {}}
+    e.set_update_kernel(0)
     for ti, tj in tiles:
         res["tile"][f"{ti}x{tj}"] = float(prof(ti, tj, base[2], base[3])[4])
+    e.set_update_kernel(1)
+    for ts in (4, 8, 16, 32, 64, 128):
+        if ts <= Nu:
+            res["stream"][str(ts)] = float(prof(ts, 0, base[2], base[3])[4])
+    e.set_update_kernel(0)
     for pr in prs:
         ms = prof(base[0], base[1], pr, base[3])
         res["pr"][str(pr)] = float(ms[1] + ms[3])
     for cb in cbs:
         res["cb"][str(cb)] = float(prof(base[0], base[1], base[2], cb)[2])
     bt = min(res["tile"], key=res["tile"].get)
+    bst = min(res["stream"], key=res["stream"].get) if res["stream"] else None
     bpr = int(min(res["pr"], key=res["pr"].get))
     bcb = int(min(res["cb"], key=res["cb"].get))
     ti, tj = (int(x) for x in bt.split("x"))
+    kind = 0
+    if bst is not None and res["stream"][bst] < res["tile"][bt]:
+        kind, ti, tj = 1, int(bst), 60
+    e.set_update_kernel(kind)
     # threads per workgroup of each kernel, at the chosen shapes
     res["threads"] = {}
     for t in (64, 128, 256):
@@ -122,12 +139,14 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     e.set_threads(*thr)
     best = prof(ti, tj, bpr, bcb)
     e.close()
-    out = {"ti": ti, "tj": tj, "pr": bpr, "cb": bcb, "threads": list(thr), "copies": copies,
+    out = {"kind": kind, "ti": ti, "tj": tj, "pr": bpr, "cb": bcb, "threads": list(thr),
+           "copies": copies,
            "ms_per_kernel": [float(x) for x in best],
            "default": {"shape": list(base), "tile_ms": res["tile"].get(f"{base[0]}x{base[1]}")},
            "sweep": res}
     if verbose:
-        print(f"{Nu}x{Nphi} {regime} x{copies}: tile {ti}x{tj} pr {bpr} cb {bcb} threads {thr} "
+        print(f"{Nu}x{Nphi} {regime} x{copies}: kernel {kind} tile {ti}x{tj} pr {bpr} cb {bcb} "
+              f"threads {thr} "
               f"ms {np.round(best, 4).tolist()}", flush=True)
     return out
 

@@ -720,6 +720,193 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   }
 }
 
+// ------------------------------------------------------------------ streaming update
+// Register-streaming variant of k_update for bandwidth-bound (large or
+// batched) runs.  One wavefront per work unit = (disk, column group g, radial
+// strip k).  Lane L owns azimuthal column c = 60 g - 2 + L (periodic); lanes
+// 2..61 produce output.  The wave walks source rows s = i0-2 .. i1+1 of its
+// strip [i0, i1): the radial stencil (Sigma/Phi windows, the last four
+// post-source rows, the previous radial face) stays in registers, azimuthal
+// neighbours come from cross-lane shuffles, and row tables are wave-uniform.
+// Same arithmetic as k_update.
+#define STREAM_OUT 60
+
+__device__ __forceinline__ double shup(double v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ double shdn(double v, int d) { return __shfl_down(v, d, 64); }
+
+template <bool GARR>
+__global__ void __launch_bounds__(64)
+k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+                const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
+                const double *__restrict__ phi, const double *__restrict__ gRa,
+                const double *__restrict__ gpa, int has_gR, int has_gp,
+                const double *__restrict__ geom, const double *__restrict__ trig,
+                double *__restrict__ pcfl, double *__restrict__ pcm) {
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  const DiskCtl &C = ctl[wm.x];
+  if (C.done) return;
+  const double dt = C.dt;
+  const int Nu = D.Nu, Nphi = D.Nphi, TS = D.ti;
+  const int nrs = D.ntile_i;
+  const int cg = wm.y / nrs, ks = wm.y - cg * nrs;
+  const int i0 = ks * TS, i1 = min(i0 + TS, Nu);
+  const int j0 = cg * STREAM_OUT;
+  const int lane = threadIdx.x;
+  int c = (j0 - 2 + lane) % Nphi;
+  if (c < 0) c += Nphi;
+  const int cm = (c == 0) ? Nphi - 1 : c - 1, cp = (c == Nphi - 1) ? 0 : c + 1;
+  const bool out_ok = lane >= 2 && lane < 2 + STREAM_OUT && (j0 + lane - 2) < Nphi;
+
+  const double *S = stp(st, ntot, C.src, 0, D), *U = stp(st, ntot, C.src, 1, D),
+               *W = stp(st, ntot, C.src, 2, D);
+  double *So = stpw(st, ntot, C.dst, 0, D), *Uo = stpw(st, ntot, C.dst, 1, D),
+         *Wo = stpw(st, ntot, C.dst, 2, D);
+  const double *Ph = phi + D.so;
+  const double *tInvdRc = gtab(geom, D, GT_INVDRC), *tInvR = gtab(geom, D, GT_INVR),
+               *tGstar = gtab(geom, D, GT_GSTAR), *tGrc = gtab(geom, D, GT_GRC),
+               *tGpc = gtab(geom, D, GT_GPC), *tInvdR = gtab(geom, D, GT_INVDR),
+               *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL),
+               *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE),
+               *tdA = gtab(geom, D, GT_DA);
+  const double c2j = trig[D.cos_off + c], s2j = trig[D.cos_off + Nphi + c];
+  const bool sg = !GARR && D.selfgrav != 0;
+
+  auto clampr = [&](int r) { return r < 0 ? 0 : (r >= Nu ? Nu - 1 : r); };
+  const int sbeg = i0 - 2, send = i1 + 1;
+  double Sm = S[(i64)clampr(sbeg - 1) * Nphi + c], S0 = S[(i64)clampr(sbeg) * Nphi + c];
+  double Pm = 0.0, P0 = 0.0;
+  if (sg) {
+    Pm = Ph[(i64)clampr(sbeg - 1) * Nphi + c];
+    P0 = Ph[(i64)clampr(sbeg) * Nphi + c];
+  }
+  // post-source ring: slot k holds row s-3+k after row s is pushed
+  double qs0 = 0, qs1 = 0, qs2 = 0, qs3 = 0, qu0 = 0, qu1 = 0, qu2 = 0, qu3 = 0;
+  double qw0 = 0, qw1 = 0, qw2 = 0, qw3 = 0;
+  double pfm = 0, pfu = 0, pfw = 0;     // previous radial face (f-1)
+  double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+
+  for (int s = sbeg; s <= send; ++s) {
+    const double Sp = S[(i64)clampr(s + 1) * Nphi + c];
+    double Pp = 0.0;
+    if (sg) Pp = Ph[(i64)clampr(s + 1) * Nphi + c];
+    // ---- source at row s (hydro.hpp:73-96, forces kalnajs.hpp:65-80)
+    double ns_s = 0.0, ns_u = 0.0, ns_w = 0.0;
+    if (s >= 0 && s < Nu) {
+      const i64 ro = (i64)s * Nphi;
+      const double u = U[ro + c], w = W[ro + c];
+      const double dPdR = (pres(D, Sp) - pres(D, Sm)) * tInvdRc[s];
+      const double dPdphi = (pres(D, S[ro + cp]) - pres(D, S[ro + cm])) * D.inv2dphi;
+      double gR = tGstar[s], gp = 0.0;
+      if (GARR) {
+        if (has_gR) gR += gRa[D.so + ro + c];
+        if (has_gp) gp += gpa[D.so + ro + c];
+      } else if (sg) {
+        gR += -(Pp - Pm) * tGrc[s];
+        gp += -(Ph[ro + cp] - Ph[ro + cm]) * tGpc[s];
+      }
+      const double rs = 1.0 / S0;
+      const double invR = tInvR[s];
+      const double aR = -dPdR * rs + gR + w * w * invR;
+      const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
+      ns_s = S0;
+      ns_u = u + dt * aR;
+      ns_w = w + dt * ap;
+    }
+    qs0 = qs1; qs1 = qs2; qs2 = qs3; qs3 = ns_s;
+    qu0 = qu1; qu1 = qu2; qu2 = qu3; qu3 = ns_u;
+    qw0 = qw1; qw1 = qw2; qw2 = qw3; qw3 = ns_w;
+    Sm = S0; S0 = Sp;
+    Pm = P0; P0 = Pp;
+    // ---- radial face f = s-1 between cells f-1 (slot 1) and f (slot 2) (hydro.hpp:111-130)
+    const int f = s - 1;
+    double fm = 0.0, fu = 0.0, fw = 0.0;
+    if (f >= i0 && f <= i1) {
+      if (f >= 1 && f <= Nu - 1) {
+        const double vf = 0.5 * (qu1 + qu2);
+        const bool up = vf > 0;
+        const int cell = up ? f - 1 : f;
+        const double bs = up ? qs1 : qs2, bu = up ? qu1 : qu2, bw = up ? qw1 : qw2;
+        const double ls = up ? qs0 : qs1, lu = up ? qu0 : qu1, lw = up ? qw0 : qw1;
+        const double hs = up ? qs2 : qs3, hu = up ? qu2 : qu3, hw = up ? qw2 : qw3;
+        const double fac = up ? 0.5 * (1.0 - vf * dt * tInvdR[f - 1])
+                              : -0.5 * (1.0 + vf * dt * tInvdR[f]);
+        double ss = bs, su = bu, sw = bw;
+        if (cell > 0 && cell < Nu - 1) {
+          ss += vanleer(hs - bs, bs - ls) * fac;
+          su += vanleer(hu - bu, bu - lu) * fac;
+          sw += vanleer(hw - bw, bw - lw) * fac;
+        }
+        fm = vf * ss * tAf[f];
+        fu = fm * su;
+        fw = fm * sw;
+      }
+      // ---- radial gather of row g = f-1 (slot 1), then the azimuthal sweep
+      const int g = f - 1;
+      if (g >= i0 && g < i1) {
+        const double s_ = qs1;
+        const double iv = dt * tInvVol[g];
+        const double ns = s_ + (pfm - fm) * iv;
+        const double nmR = s_ * qu1 + (pfu - fu) * iv;
+        const double nmP = s_ * qw1 + (pfw - fw) * iv;
+        const double rn = 1.0 / smax(ns, 1e-10);
+        const double Xs = ns, Xu = nmR * rn, Xw = nmP * rn;
+        // azimuthal face at this lane's left edge (hydro.hpp:153-168)
+        const double sm1 = shup(Xs, 1), um1 = shup(Xu, 1), wm1 = shup(Xw, 1);
+        const double sm2 = shup(Xs, 2), um2 = shup(Xu, 2), wm2 = shup(Xw, 2);
+        const double sp1 = shdn(Xs, 1), up1 = shdn(Xu, 1), wp1 = shdn(Xw, 1);
+        const double vf = 0.5 * (wm1 + Xw);
+        const bool upw = vf > 0;
+        const double bs = upw ? sm1 : Xs, bu = upw ? um1 : Xu, bw = upw ? wm1 : Xw;
+        const double ls = upw ? sm2 : sm1, lu = upw ? um2 : um1, lw = upw ? wm2 : wm1;
+        const double hs = upw ? Xs : sp1, hu = upw ? Xu : up1, hw = upw ? Xw : wp1;
+        const double fac = upw ? 0.5 * (1.0 - vf * dt * tInvdl[g])
+                               : -0.5 * (1.0 + vf * dt * tInvdl[g]);
+        const double ss = bs + vanleer(hs - bs, bs - ls) * fac;
+        const double su = bu + vanleer(hu - bu, bu - lu) * fac;
+        const double sw = bw + vanleer(hw - bw, bw - lw) * fac;
+        const double Fm = vf * ss * tAface[g];
+        const double FU = Fm * su, FW = Fm * sw;
+        const double Fm1 = shdn(Fm, 1), FU1 = shdn(FU, 1), FW1 = shdn(FW, 1);
+        // azimuthal gather + floor (hydro.hpp:170-178, 68-69)
+        const double ns2 = Xs + (Fm - Fm1) * iv;
+        const double nmR2 = Xs * Xu + (FU - FU1) * iv;
+        const double nmP2 = Xs * Xw + (FW - FW1) * iv;
+        const double rn2 = 1.0 / smax(ns2, 1e-10);
+        const double so = smax(ns2, 1e-10), uo = nmR2 * rn2, wo = nmP2 * rn2;
+        if (out_ok) {
+          const i64 o = (i64)g * Nphi + c;
+          So[o] = so;
+          Uo[o] = uo;
+          Wo[o] = wo;
+          const double r = cfl_rate(D, so, uo, wo, tInvdR[g], tInvdl[g]);
+          mx = (r > mx) ? r : mx;
+          a0 += so * tdA[g];
+          a1 += so * c2j * tdA[g];
+          a2 -= so * s2j * tdA[g];
+          a3 += nonfinite(so, uo, wo);
+        }
+      }
+    }
+    if (f >= i0 && f <= i1) { pfm = fm; pfu = fu; pfw = fw; }
+  }
+  // wave reduction of the partials (fixed order)
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(mx, off);
+    mx = (o > mx) ? o : mx;
+    a0 += __shfl_xor(a0, off);
+    a1 += __shfl_xor(a1, off);
+    a2 += __shfl_xor(a2, off);
+    a3 += __shfl_xor(a3, off);
+  }
+  if (lane == 0) {
+    const i64 t = D.part + wm.y;
+    pcfl[t] = mx;
+    pcm[4 * t + 0] = a0; pcm[4 * t + 1] = a1; pcm[4 * t + 2] = a2; pcm[4 * t + 3] = a3;
+  }
+}
+
 // ------------------------------------------------------------------ unit stages
 // Single-stage kernels that follow the reference's loop structure one to one
 // (source, radial faces/gather, azimuthal faces/gather, floor, forces).  They
@@ -909,6 +1096,21 @@ hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int 
   else
     hipLaunchKernelGGL(k_update<false>, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot,
                        phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, const DiskDev *disks,
+                                    const int2 *map, int base, const DiskCtl *ctl, double *st,
+                                    i64 ntot, const double *phi, const double *gR, const double *gp,
+                                    int has_gR, int has_gp, const double *geom, const double *trig,
+                                    double *pcfl, double *pcm) {
+  if (nblk <= 0) return hipSuccess;
+  if (garr)
+    hipLaunchKernelGGL(k_update_stream<true>, dim3(nblk), dim3(64), 0, s, disks, map, base, ctl, st,
+                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+  else
+    hipLaunchKernelGGL(k_update_stream<false>, dim3(nblk), dim3(64), 0, s, disks, map, base, ctl, st,
+                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
   return hipGetLastError();
 }
 

@@ -26,6 +26,11 @@ hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int 
                              long long ntot, const double *phi, const double *gR,
                              const double *gp, int has_gR, int has_gp, const double *geom,
                              const double *trig, double *pcfl, double *pcm);
+hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, const DiskDev *disks,
+                                    const int2 *map, int base, const DiskCtl *ctl, double *st,
+                                    long long ntot, const double *phi, const double *gR,
+                                    const double *gp, int has_gR, int has_gp, const double *geom,
+                                    const double *trig, double *pcfl, double *pcm);
 hipError_t dfk_launch_khat(hipStream_t s, const DiskDev *D, int nrow_blk, int ncol_blk,
                            size_t lds_rows, size_t lds_cols, const double *ktab,
                            const double2 *tw, double2 *tmp, double *khat);

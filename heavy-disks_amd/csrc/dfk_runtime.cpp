@@ -74,6 +74,7 @@ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // work list so that XCD x receives one contiguous chunk of it: neighbouring
 // tiles (which share halo rows) then meet in the same L2.
 const int kXcds = 8;
+const int kStreamCols = 60;   // output columns per wave of the streaming update (kernel's STREAM_OUT)
 std::vector<int2> xcd_order(const std::vector<int2> &v) {
   const int L = (int)v.size();
   std::vector<int2> out(L);
@@ -121,6 +122,7 @@ struct dfk_handle {
   double last_ms = 0.0;
   int chunk = 128;               // loop iterations per captured graph
   int thr_rows = 256, thr_cols = 256, thr_upd = 256;   // threads per workgroup
+  int upd_kind = 0;              // 0: LDS-tile update kernel, 1: register-streaming
 };
 
 namespace {
@@ -161,6 +163,7 @@ int build_layout(dfk_handle *h) {
   size_t lu = 0, lr = 0, lc = 0;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
+    if (h->upd_kind == 1) D.tj = kStreamCols;   // one wave = 60 output columns
     D.tj = std::max(1, std::min(D.tj, D.Nphi));
     D.ti = std::max(1, std::min(D.ti, D.Nu));
     D.ntile_i = ceil_div(D.Nu, D.ti);
@@ -171,7 +174,7 @@ int build_layout(dfk_handle *h) {
     h->upd0[d] = (int)mu.size();
     for (int t = 0; t < nt; ++t) mu.push_back(make_int2(d, t));
     size_t l = (size_t)(3 * (D.ti + 4) * (D.tj + 4) + 3 * (D.ti + 1) * (D.tj + 4)) * sizeof(double);
-    lu = std::max(lu, l);
+    if (h->upd_kind == 0) lu = std::max(lu, l);
     D.pr = std::max(1, std::min(D.pr, ceil_div(D.Nu, 2)));
     D.cb = std::max(1, std::min(D.cb, D.Nc));
     h->r2c0[d] = (int)mr.size();
@@ -268,6 +271,16 @@ int launch_gravity_disk(dfk_handle *h, int d) {
   return DFK_OK;
 }
 
+hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR, int has_gp) {
+  if (h->upd_kind == 1)
+    return dfk_launch_update_stream(h->stream, nblk, garr, h->dD, h->map_upd, base, h->ctl, h->st,
+                                    h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
+                                    h->trig, h->pcfl, h->pcm);
+  return dfk_launch_update(h->stream, nblk, h->thr_upd, h->lds_upd, garr, h->dD, h->map_upd, base,
+                           h->ctl, h->st, h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
+                           h->trig, h->pcfl, h->pcm);
+}
+
 // One whole loop iteration for every disk (captured into the graph).
 int launch_step_all(dfk_handle *h) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
@@ -280,9 +293,7 @@ int launch_step_all(dfk_handle *h) {
     HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
                           h->geom, h->tw, h->phi));
   }
-  HIPCHK(dfk_launch_update(h->stream, nu, h->thr_upd, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
-                           h->ntot, h->phi, nullptr, nullptr, 0, 0, h->geom, h->trig, h->pcfl,
-                           h->pcm));
+  HIPCHK(launch_update(h, nu, 0, 0, 0, 0));
   return DFK_OK;
 }
 
@@ -719,9 +730,7 @@ int dfk_step(dfk_handle *h, int d, double dt, const double *gR, const double *gp
   const int c = h->hcur[d];
   TRY(unit_ctl(h, d, c, 1 - c, dt));
   const int nb = h->upd0[d + 1] - h->upd0[d];
-  HIPCHK(dfk_launch_update(h->stream, nb, h->thr_upd, h->lds_upd, 1, h->dD, h->map_upd, h->upd0[d], h->ctl,
-                           h->st, h->ntot, h->phi, h->gR, h->gp, gR != nullptr, gp != nullptr,
-                           h->geom, h->trig, h->pcfl, h->pcm));
+  HIPCHK(launch_update(h, nb, h->upd0[d], 1, gR != nullptr, gp != nullptr));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->hcur[d] = 1 - c;
   return DFK_OK;
@@ -896,6 +905,23 @@ int dfk_set_threads(dfk_handle *h, int rows, int cols, int update) {
   return DFK_OK;
 }
 
+int dfk_set_update_kernel(dfk_handle *h, int kind) {
+  if (!h || kind < 0 || kind > 1) { set_err("update kernel must be 0 (LDS tile) or 1 (streaming)"); return DFK_EINVAL; }
+  if (kind != h->upd_kind) {
+    h->upd_kind = kind;
+    for (auto &D : h->hD) D.tj = (kind == 1) ? kStreamCols : 32;
+    h->layout_ok = false;
+    return build_layout(h);
+  }
+  return DFK_OK;
+}
+
+int dfk_get_update_kernel(dfk_handle *h, int *kind) {
+  if (!h || !kind) { set_err("null argument"); return DFK_EINVAL; }
+  *kind = h->upd_kind;
+  return DFK_OK;
+}
+
 int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update) {
   if (!h) { set_err("null handle"); return DFK_EINVAL; }
   if (rows) *rows = h->thr_rows;
@@ -950,9 +976,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
                                 h->geom, h->tw, h->phi));
           break;
         case 4:
-          HIPCHK(dfk_launch_update(h->stream, nu, h->thr_upd, h->lds_upd, 0, h->dD, h->map_upd, 0, h->ctl, h->st,
-                                   h->ntot, h->phi, nullptr, nullptr, 0, 0, h->geom, h->trig,
-                                   h->pcfl, h->pcm));
+          HIPCHK(launch_update(h, nu, 0, 0, 0, 0));
           break;
       }
     }

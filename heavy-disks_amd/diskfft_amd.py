@@ -32,7 +32,7 @@ SYMBOLS = [
     "dfk_forces", "dfk_step", "dfk_stage", "dfk_khat", "dfk_modes", "dfk_disk_mass",
     "dfk_run_init", "dfk_advance", "dfk_samples", "dfk_get_status", "dfk_set_shapes",
     "dfk_get_shapes", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
-    "dfk_set_threads", "dfk_get_threads",
+    "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
 ]
 
 
@@ -99,6 +99,8 @@ def lib():
             "dfk_profile_step": (_I, [_P, _I, _DP]),
             "dfk_set_threads": (_I, [_P, _I, _I, _I]),
             "dfk_get_threads": (_I, [_P, _IP, _IP, _IP]),
+            "dfk_set_update_kernel": (_I, [_P, _I]),
+            "dfk_get_update_kernel": (_I, [_P, _IP]),
         }
         for k, (res, args) in sig.items():
             f = getattr(L, k)
@@ -279,6 +281,15 @@ class Ensemble:
         v = [_I() for _ in range(3)]
         _check(lib().dfk_get_threads(self.h, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
+
+    def set_update_kernel(self, kind):
+        """0: LDS tile kernel, 1: register-streaming kernel."""
+        _check(lib().dfk_set_update_kernel(self.h, kind))
+
+    def get_update_kernel(self):
+        k = _I()
+        _check(lib().dfk_get_update_kernel(self.h, ctypes.byref(k)))
+        return k.value
 
     def set_graph_steps(self, n):
         _check(lib().dfk_set_graph_steps(self.h, n))

@@ -133,6 +133,13 @@ int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
 int dfk_set_threads(dfk_handle *h, int rows, int cols, int update);
 int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update);
 
+/* Update-kernel variant for all disks of the handle: 0 = LDS tile (ti x tj
+ * zones per workgroup; suits small, latency-bound runs), 1 = register
+ * streaming (one wavefront per 60 columns x ti rows; suits large or batched,
+ * bandwidth-bound runs).  Both compute the same step. */
+int dfk_set_update_kernel(dfk_handle *h, int kind);
+int dfk_get_update_kernel(dfk_handle *h, int *kind);
+
 /* Device time in ms of the last dfk_advance (HIP events around the graph
  * launch on the handle's stream). */
 int dfk_last_advance_ms(dfk_handle *h, double *ms);

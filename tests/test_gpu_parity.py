@@ -118,14 +118,18 @@ def test_single_stage_matches_oracle(orc, dfk, stage, Nu, Nphi):
         assert relerr(a, b) < OP_TOL
 
 
-@pytest.mark.parametrize("shape", [(8, 64, 0, 0), (4, 16, 0, 0), (3, 7, 0, 0), (16, 32, 0, 0),
-                                   (1, 5, 0, 0)])
-def test_fused_step_matches_oracle(orc, dfk, shape):
-    """DiskHydro::step with given g_R/g_phi through the fused tile kernel, for
-    tile shapes that do and do not divide the grid."""
-    Nu, Nphi = 37, 52
-    print(f"case fused step tiles={shape}", flush=True)
+@pytest.mark.parametrize("kind,shape,Nu,Nphi", [
+    (0, (8, 64, 0, 0), 37, 52), (0, (4, 16, 0, 0), 37, 52), (0, (3, 7, 0, 0), 37, 52),
+    (0, (16, 32, 0, 0), 37, 52), (0, (1, 5, 0, 0), 37, 52),
+    (1, (8, 0, 0, 0), 37, 52), (1, (5, 0, 0, 0), 37, 52), (1, (64, 0, 0, 0), 37, 52),
+    (1, (16, 0, 0, 0), 40, 130), (1, (3, 0, 0, 0), 21, 7), (1, (32, 0, 0, 0), 70, 256)])
+def test_fused_step_matches_oracle(orc, dfk, kind, shape, Nu, Nphi):
+    """DiskHydro::step with given g_R/g_phi through the fused update kernels
+    (0 LDS tile, 1 register streaming), for tile / strip shapes that do and do
+    not divide the grid, and column counts below, at and above one wave."""
+    print(f"case fused step kind={kind} tiles={shape} grid {Nu}x{Nphi}", flush=True)
     e, h, _ = _pair(orc, dfk, Nu, Nphi, selfgrav=0)
+    e.set_update_kernel(kind)
     e.set_shapes(*shape)
     st = rough_state(Nu, Nphi, seed=9)
     e.set_state(*st)
@@ -150,13 +154,15 @@ def _lka98(orc, Nu, Nphi, eps, amp=1e-3, selfgrav=1):
     return h, k
 
 
+@pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("Nu,Nphi,eps", [(64, 64, 0.02), (128, 128, 0.02), (96, 80, 0.01),
                                          (256, 256, 0.005), (48, 36, 0.02)])
-def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps):
+def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps, kind):
     """The north-star gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative."""
-    print(f"case 10-step loop {Nu}x{Nphi} eps={eps}", flush=True)
+    print(f"case 10-step loop {Nu}x{Nphi} eps={eps} update kernel {kind}", flush=True)
     h, k = _lka98(orc, Nu, Nphi, eps)
     e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
+    e.set_update_kernel(kind)
     e.set_state(*h.state())
     mD = h.disk_mass()
     so, tf, c2f, steps = orc.run(h, k, 1, 1e9, dtout=0.25, cfl=0.3, max_steps=10, mD=mD)
@@ -205,27 +211,32 @@ def test_mass_conservation_gpu(dfk):
     assert abs(e.disk_mass() - m0) / m0 < 1e-13
 
 
-def test_ensemble_matches_single_disks(orc, dfk):
+@pytest.mark.parametrize("kind", [0, 1])
+def test_ensemble_matches_single_disks(orc, dfk, kind):
     """Disks of different shapes in one handle give the same bits as each disk
     run alone (identical kernels per disk)."""
-    print("case ensemble vs single", flush=True)
-    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 30, 0.04)]
+    print(f"case ensemble vs single, update kernel {kind}", flush=True)
+    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 30, 0.04), (40, 130, 0.01)]
     params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
     ens = dfk.Ensemble(params)
+    ens.set_update_kernel(kind)
     ics, mDs = [], []
     for d, (a, b, c) in enumerate(shapes):
         h, _ = _lka98(orc, a, b, c)
         ics.append(h.state())
         mDs.append(h.disk_mass())
         ens.set_state(*ics[-1], d=d)
-    ens.run_init(1e9, mD=mDs, max_steps=[20, 35, 27])
+    caps = [20, 35, 27, 31]
+    ens.run_init(1e9, mD=mDs, max_steps=caps)
     ens.advance(40)
     for d, p in enumerate(params):
         one = dfk.Ensemble(p)
+        one.set_update_kernel(kind)
+        one.set_shapes(*ens.get_shapes(d))
         one.set_state(*ics[d])
-        one.run_init(1e9, mD=mDs[d], max_steps=[20, 35, 27][d])
+        one.run_init(1e9, mD=mDs[d], max_steps=caps[d])
         one.advance(40)
-        assert ens.status(d)["steps"] == [20, 35, 27][d]
+        assert ens.status(d)["steps"] == caps[d]
         for a, b in zip(ens.get_state(d), one.get_state()):
             assert np.array_equal(a, b)
 
