@@ -909,7 +909,10 @@ int dfk_set_update_kernel(dfk_handle *h, int kind) {
   if (!h || kind < 0 || kind > 1) { set_err("update kernel must be 0 (LDS tile) or 1 (streaming)"); return DFK_EINVAL; }
   if (kind != h->upd_kind) {
     h->upd_kind = kind;
-    for (auto &D : h->hD) D.tj = (kind == 1) ? kStreamCols : 32;
+    for (auto &D : h->hD) {
+      D.tj = (kind == 1) ? kStreamCols : 32;
+      if (kind == 0) D.ti = std::min(D.ti, 16);   // a strip height may not fit an LDS tile
+    }
     h->layout_ok = false;
     return build_layout(h);
   }
