@@ -227,4 +227,59 @@ double dfk_model_mass(const dfk_model *m, int Nu, int Nphi, const double *Sig) {
   return mD;
 }
 
+int dfk_extract_growth(const double *t, const double *c0, const double *c2, const double *ph2,
+                       int n, double lo_factor, double hi_frac, double grow_factor,
+                       dfk_growth *out) {
+  if (!t || !c0 || !c2 || !ph2 || !out || n < 4) return 1;
+  std::vector<double> a(n), ph(n);
+  double amax = 0.0;
+  for (int k = 0; k < n; ++k) {
+    a[k] = c2[k] / c0[k];
+    if (a[k] > amax) amax = a[k];
+  }
+  // unwrap the phase
+  ph[0] = ph2[0];
+  for (int k = 1; k < n; ++k) {
+    double d = ph2[k] - ph2[k - 1];
+    d -= 2.0 * M_PI * std::floor((d + M_PI) / (2.0 * M_PI));
+    ph[k] = ph[k - 1] + d;
+  }
+  int lo = 0, hi = n - 1, grew = 0;
+  if (amax >= grow_factor * a[0]) {
+    int l = -1, h = -1;
+    for (int k = 0; k < n; ++k)
+      if (l < 0 && a[k] >= lo_factor * a[0]) l = k;
+    for (int k = 0; k < n; ++k)
+      if (h < 0 && a[k] >= hi_frac * amax) h = k;
+    if (l >= 0 && h >= 0 && h - l + 1 >= 4) {
+      lo = l;
+      hi = h;
+      grew = 1;
+    }
+  }
+  // least squares over [lo, hi]
+  const int m = hi - lo + 1;
+  double st = 0, sa = 0, sp = 0;
+  for (int k = lo; k <= hi; ++k) { st += t[k]; sa += std::log(a[k]); sp += ph[k]; }
+  const double tm = st / m, am = sa / m, pm = sp / m;
+  double stt = 0, sta = 0, stp = 0;
+  for (int k = lo; k <= hi; ++k) {
+    const double dt = t[k] - tm;
+    stt += dt * dt;
+    sta += dt * (std::log(a[k]) - am);
+    stp += dt * (ph[k] - pm);
+  }
+  out->gamma = (stt > 0) ? sta / stt : 0.0;
+  out->omega_p = (stt > 0) ? -0.5 * stp / stt : 0.0;
+  const int q0 = n - std::max(1, n / 4);
+  double ss = 0;
+  for (int k = q0; k < n; ++k) ss += a[k];
+  out->sat = ss / (n - q0);
+  out->t_lo = t[lo];
+  out->t_hi = t[hi];
+  out->n_fit = m;
+  out->grew = grew;
+  return 0;
+}
+
 }  // extern "C"

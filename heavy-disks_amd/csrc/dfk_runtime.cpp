@@ -864,6 +864,90 @@ int dfk_get_status(dfk_handle *h, int d, dfk_status *st) {
   return DFK_OK;
 }
 
+namespace {
+struct SnapHeader {
+  char magic[8];             // "DFKSNAP1"
+  int Nu, Nphi;
+  dfk_disk_params params;
+  double t, tnext, tlim, dtout, cfl;
+  long long step;
+  int flags, pad_;
+};
+}  // namespace
+
+int dfk_snapshot_save(dfk_handle *h, int d, const char *path) {
+  TRY(check_disk(h, d));
+  if (!path) { set_err("null path"); return DFK_EINVAL; }
+  if (!h->run_ready) { set_err("dfk_snapshot_save needs a run (dfk_run_init)"); return DFK_ESTATE; }
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  DiskCtl c;
+  TRY(read_ctl(h, d, &c));
+  SnapHeader hd;
+  memset(&hd, 0, sizeof hd);
+  memcpy(hd.magic, "DFKSNAP1", 8);
+  hd.Nu = h->hD[d].Nu;
+  hd.Nphi = h->hD[d].Nphi;
+  hd.params = h->P[d];
+  hd.t = c.t; hd.tnext = c.tnext; hd.tlim = c.tlim; hd.dtout = c.dtout; hd.cfl = c.cfl;
+  hd.step = c.step;
+  hd.flags = c.flags;
+  const size_t N = (size_t)hd.Nu * hd.Nphi;
+  std::vector<double> buf(3 * N);
+  h->hcur[d] = c.cur;
+  TRY(dfk_get_state(h, d, buf.data(), buf.data() + N, buf.data() + 2 * N));
+  FILE *f = fopen(path, "wb");
+  if (!f) { set_err("cannot open %s for writing", path); return DFK_EINVAL; }
+  const bool ok = fwrite(&hd, sizeof hd, 1, f) == 1 && fwrite(buf.data(), sizeof(double), 3 * N, f) == 3 * N;
+  fclose(f);
+  if (!ok) { set_err("short write to %s", path); return DFK_EINVAL; }
+  return DFK_OK;
+}
+
+int dfk_snapshot_load(dfk_handle *h, int d, const char *path) {
+  TRY(check_disk(h, d));
+  if (!path) { set_err("null path"); return DFK_EINVAL; }
+  if (!h->run_ready) { set_err("dfk_snapshot_load needs dfk_run_init first"); return DFK_ESTATE; }
+  FILE *f = fopen(path, "rb");
+  if (!f) { set_err("cannot open %s", path); return DFK_EINVAL; }
+  SnapHeader hd;
+  bool ok = fread(&hd, sizeof hd, 1, f) == 1 && memcmp(hd.magic, "DFKSNAP1", 8) == 0;
+  const dfk_disk_params &p = h->P[d];
+  if (ok && (hd.Nu != p.Nu || hd.Nphi != p.Nphi || hd.params.K != p.K || hd.params.eps != p.eps ||
+             hd.params.gamma != p.gamma || hd.params.gmstar != p.gmstar || hd.params.Rin != p.Rin ||
+             hd.params.Rout != p.Rout || hd.params.G != p.G || hd.params.selfgrav != p.selfgrav)) {
+    fclose(f);
+    set_err("snapshot %s is for a different disk (grid or parameters differ)", path);
+    return DFK_EINVAL;
+  }
+  const size_t N = (size_t)p.Nu * p.Nphi;
+  std::vector<double> buf(3 * N);
+  ok = ok && fread(buf.data(), sizeof(double), 3 * N, f) == 3 * N;
+  fclose(f);
+  if (!ok) { set_err("%s is not a complete snapshot", path); return DFK_EINVAL; }
+  HIPCHK(hipSetDevice(h->device));
+  DiskCtl c;
+  TRY(read_ctl(h, d, &c));
+  h->hcur[d] = c.cur;
+  const int cur = c.cur;
+  const size_t nb = N * sizeof(double);
+  HIPCHK(hipMemcpy(field(h, cur, 0, d), buf.data(), nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(field(h, cur, 1, d), buf.data() + N, nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(field(h, cur, 2, d), buf.data() + 2 * N, nb, hipMemcpyHostToDevice));
+  c.t = hd.t;
+  c.tnext = hd.tnext;
+  c.step = hd.step;
+  c.flags = hd.flags;
+  c.done = 0;
+  TRY(write_ctl(h, d, &c));
+  // partials (CFL maximum, C_m sums) of the restored state
+  const int nb_ = h->upd0[d + 1] - h->upd0[d];
+  HIPCHK(dfk_launch_diag(h->stream, nb_, h->dD, h->map_upd, h->upd0[d], h->ctl, h->st, h->ntot,
+                         h->geom, h->trig, h->pcfl, h->pcm));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return DFK_OK;
+}
+
 int dfk_set_shapes(dfk_handle *h, int d, int ti, int tj, int pr, int cb) {
   TRY(check_disk(h, d));
   DiskDev &D = h->hD[d];

@@ -33,6 +33,7 @@ SYMBOLS = [
     "dfk_run_init", "dfk_advance", "dfk_samples", "dfk_get_status", "dfk_set_shapes",
     "dfk_get_shapes", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
     "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
+    "dfk_snapshot_save", "dfk_snapshot_load",
 ]
 
 
@@ -101,6 +102,8 @@ def lib():
             "dfk_get_threads": (_I, [_P, _IP, _IP, _IP]),
             "dfk_set_update_kernel": (_I, [_P, _I]),
             "dfk_get_update_kernel": (_I, [_P, _IP]),
+            "dfk_snapshot_save": (_I, [_P, _I, ctypes.c_char_p]),
+            "dfk_snapshot_load": (_I, [_P, _I, ctypes.c_char_p]),
         }
         for k, (res, args) in sig.items():
             f = getattr(L, k)
@@ -291,6 +294,12 @@ class Ensemble:
         _check(lib().dfk_get_update_kernel(self.h, ctypes.byref(k)))
         return k.value
 
+    def snapshot_save(self, path, d=0):
+        _check(lib().dfk_snapshot_save(self.h, d, os.fsencode(path)))
+
+    def snapshot_load(self, path, d=0):
+        _check(lib().dfk_snapshot_load(self.h, d, os.fsencode(path)))
+
     def set_graph_steps(self, n):
         _check(lib().dfk_set_graph_steps(self.h, n))
 
@@ -323,7 +332,12 @@ class Ensemble:
 # disk models (include/diskfft_model.h; host code, no GPU needed)
 MODEL_LKA98, MODEL_POWERLAW = 0, 1
 MODEL_SYMBOLS = ["dfk_model_defaults", "dfk_model_sigma0", "dfk_model_state", "dfk_model_mass",
-                 "dfk_splitmix64", "dfk_normal"]
+                 "dfk_splitmix64", "dfk_normal", "dfk_extract_growth"]
+
+
+class Growth(ctypes.Structure):
+    _fields_ = [("gamma", _D), ("omega_p", _D), ("sat", _D), ("t_lo", _D), ("t_hi", _D),
+                ("n_fit", _I), ("grew", _I)]
 
 
 class Model(ctypes.Structure):
@@ -348,6 +362,9 @@ def _model_lib():
         L.dfk_splitmix64.restype = ctypes.c_ulonglong
         L.dfk_normal.argtypes = [ctypes.POINTER(ctypes.c_ulonglong)]
         L.dfk_normal.restype = _D
+        L.dfk_extract_growth.argtypes = [_DP, _DP, _DP, _DP, _I, _D, _D, _D,
+                                         ctypes.POINTER(Growth)]
+        L.dfk_extract_growth.restype = _I
         L._model_sig = True
     return L
 
@@ -406,3 +423,17 @@ def model_ic(m, Nu, Nphi, eps, selfgrav=1, potential_forces=None, device=0):
     p = disk_params(Nu, Nphi, Rin=m.Rin, Rout=m.Rout, K=m.K, gamma=m.gamma, gmstar=m.gmstar,
                     eps=eps, G=m.G, selfgrav=selfgrav)
     return p, S, U, W, model_mass(m, S), Q
+
+
+def extract_growth(samples, lo_factor=3.0, hi_frac=0.3, grow_factor=10.0):
+    """gamma, Omega_p and saturated |C2|/|C0| from an (n, 4) array of
+    (t, |C0|, |C2|, arg C2) samples (see dfk_extract_growth)."""
+    a = np.ascontiguousarray(np.asarray(samples, dtype=np.float64))
+    cols = [np.ascontiguousarray(a[:, k]) for k in range(4)]
+    g = Growth()
+    rc = _model_lib().dfk_extract_growth(*[_dp(c) for c in cols], len(a), lo_factor, hi_frac,
+                                         grow_factor, ctypes.byref(g))
+    if rc:
+        raise ValueError("dfk_extract_growth needs at least 4 samples")
+    return dict(gamma=g.gamma, omega_p=g.omega_p, sat=g.sat, t_lo=g.t_lo, t_hi=g.t_hi,
+                n_fit=g.n_fit, grew=bool(g.grew))

@@ -120,6 +120,16 @@ int dfk_advance(dfk_handle *h, int nsteps, int *all_done);
 int dfk_samples(dfk_handle *h, int disk, dfk_sample *buf, int cap, int *n);
 int dfk_get_status(dfk_handle *h, int disk, dfk_status *st);
 
+/* Snapshot / resume (no counterpart in the reference, which writes only the
+ * diagnostic series).  save writes disk `disk`'s constructor parameters, loop
+ * clock (t, next output time, steps) and current state (Sigma, vR, vphi) to
+ * a binary file.  load, called after dfk_run_init, restores state and clock of
+ * a disk with the same grid and parameters (DFK_EINVAL otherwise); the run
+ * then continues exactly as the saved one would have (dt is a function of the
+ * state only). */
+int dfk_snapshot_save(dfk_handle *h, int disk, const char *path);
+int dfk_snapshot_load(dfk_handle *h, int disk, const char *path);
+
 /* Tile / transform-batch shapes (the autotuner's knobs): update tile ti x tj,
  * complex rows per workgroup in the row transforms (pr) and columns per
  * workgroup in the column convolution (cb).  0 keeps the current value.

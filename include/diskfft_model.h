@@ -64,6 +64,27 @@ int dfk_model_state(dfk_model *m, int Nu, int Nphi, const double *gRdisk, double
 /* Disk mass sum Sigma R dR dphi (diskmain.cpp:65-70). */
 double dfk_model_mass(const dfk_model *m, int Nu, int Nphi, const double *Sig);
 
+/* Growth-rate / pattern-speed / saturation extractor for the m = 2 mode from
+ * the driver's output series (t, |C0|, |C2|, arg C2) of diskmain.cpp:93-98.
+ * The reference only writes the series; this definition is the build's own:
+ *   a(t) = |C2|/|C0|;  growth window = from the first sample with
+ *   a >= lo_factor * a(0) to the first sample with a >= hi_frac * max a;
+ *   gamma  = least-squares slope of ln a over the window;
+ *   omega_p = -1/2 least-squares slope of unwrap(arg C2) over the window
+ *            (C_m ~ exp(-i m Omega_p t) for a prograde pattern);
+ *   sat    = mean of a over the last quarter of the samples.
+ * If the mode never grows past grow_factor * a(0) (or the window has fewer
+ * than 4 samples) the window is the whole series and grew = 0. */
+typedef struct {
+  double gamma, omega_p, sat;
+  double t_lo, t_hi;         /* fit window */
+  int n_fit;                 /* samples in the window */
+  int grew;
+} dfk_growth;
+int dfk_extract_growth(const double *t, const double *c0, const double *c2, const double *ph2,
+                       int n, double lo_factor, double hi_frac, double grow_factor,
+                       dfk_growth *out);
+
 /* SplitMix64 stream: next 64-bit value (state updated), and a standard normal
  * by Box-Muller (one value per call; the pair's sine half is discarded). */
 unsigned long long dfk_splitmix64(unsigned long long *state);

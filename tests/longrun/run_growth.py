@@ -1,0 +1,91 @@
+"""Long runs of the loop on the GPU and on the CPU oracle from the same state;
+gamma, Omega_p and saturated |C2|/|C0| of the m = 2 mode from each.
+
+    python tests/longrun/run_growth.py CASE [CASE ...] --out FILE.json
+
+Cases (name: model, grid, eps, tlim or steps, CFL):
+  lka128_e<eps>   LKA98 ring 128x128, tlim 9, CFL 0.3 (diskfft/README.md:27-30)
+  lka256          LKA98 ring 256x256, eps 0.005, tlim 16 (diskfft/README.md:36-43)
+  pl128           power-law model 128x128, eps 0.01, 2000 steps, CFL 0.4
+  pl256           power-law model 256x256, eps 0.01, 20 outer orbits (t = 89), CFL 0.4
+The oracle runs with OpenMP on this host's cores (<= 16).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+for p in (os.path.join(ROOT, "heavy-disks_amd"), os.path.join(ROOT, "oracle")):
+    sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+import diskfft_amd as dfk  # noqa: E402
+import oracle  # noqa: E402
+
+
+def case(name):
+    if name.startswith("lka128_e"):
+        return dict(model="lka98", N=(128, 128), eps=float(name[8:]), tlim=9.0, steps=0, cfl=0.3)
+    return {"lka256": dict(model="lka98", N=(256, 256), eps=0.005, tlim=16.0, steps=0, cfl=0.3),
+            "pl128": dict(model="powerlaw", N=(128, 128), eps=0.01, tlim=1e9, steps=2000, cfl=0.4),
+            "pl256": dict(model="powerlaw", N=(256, 256), eps=0.01, tlim=89.0, steps=0, cfl=0.4),
+            }[name]
+
+
+def run_case(name):
+    c = case(name)
+    Nu, Nphi = c["N"]
+    kind = dfk.MODEL_LKA98 if c["model"] == "lka98" else dfk.MODEL_POWERLAW
+    m = dfk.model_defaults(kind)
+    k = oracle.Kalnajs(Nu, Nphi, m.Rin, m.Rout, c["eps"], m.G, fast=True)
+    # the oracle's gravity sets up the equilibrium; both sides start from the same arrays
+    p, S, U, W, mD, _ = dfk.model_ic(m, Nu, Nphi, c["eps"],
+                                     potential_forces=lambda x: k.forces(k.potential(x)))
+    e = dfk.Ensemble(p)
+    e.set_state(S, U, W)
+    t0 = time.perf_counter()
+    ((sg, stg),) = e.run(c["tlim"], dtout=0.25, cfl=c["cfl"], mD=mD, max_steps=c["steps"],
+                         chunk=5000)
+    tg = time.perf_counter() - t0
+    h = oracle.Hydro(Nu, Nphi, p["Rin"], p["Rout"], p["K"], p["gamma"], p["gmstar"], fast=True)
+    h.set_state(S, U, W)
+    t0 = time.perf_counter()
+    so, tf, c2f, steps = oracle.run(h, k, 1, c["tlim"], dtout=0.25, cfl=c["cfl"],
+                                    max_steps=c["steps"], mD=mD, cap=1 << 20)
+    to = time.perf_counter() - t0
+    gg, go = dfk.extract_growth(sg), dfk.extract_growth(so)
+    rel = {q: abs(gg[q] - go[q]) / abs(go[q]) if go[q] else None
+           for q in ("gamma", "omega_p", "sat")}
+    Sg, _, _ = e.get_state()
+    out = dict(case=name, config=c, K=p["K"], gpu=gg, oracle=go, rel_diff=rel,
+               steps_gpu=stg["steps"], steps_oracle=steps, t_gpu=stg["t"], t_oracle=tf,
+               c2_final_gpu=stg["c2_final"], c2_final_oracle=c2f,
+               mass_drift_gpu=float(sg[-1, 1] - 1.0), mass_drift_oracle=float(so[-1, 1] - 1.0),
+               final_sigma_maxrel=float(np.max(np.abs(Sg - h.Sig)) / np.max(np.abs(h.Sig))),
+               wall_gpu_s=tg, wall_oracle_s=to, oracle_threads=oracle.lib(True).orc_get_threads(),
+               samples_gpu=sg.tolist(), samples_oracle=so.tolist())
+    print(f"{name}: gamma gpu {gg['gamma']:.5f} oracle {go['gamma']:.5f} | Omega_p gpu "
+          f"{gg['omega_p']:.5f} oracle {go['omega_p']:.5f} | sat gpu {gg['sat']:.4f} oracle "
+          f"{go['sat']:.4f} | steps {stg['steps']}/{steps} | {tg:.1f}s gpu {to:.1f}s cpu", flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cases", nargs="+")
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args()
+    oracle.set_threads(max(1, min(16, os.cpu_count() or 1)))
+    res = []
+    for name in args.cases:
+        res.append(run_case(name))
+        with open(args.out, "w") as f:
+            json.dump(res, f)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,6 +26,14 @@ def test_library_exports_every_declared_symbol():
     assert sorted(diskfft_amd.SYMBOLS) == names
 
 
+def test_model_library_exports():
+    import diskfft_amd
+    L = ctypes.CDLL(diskfft_amd.LIBPATH)
+    names = _declared("diskfft_model.h")
+    assert sorted(diskfft_amd.MODEL_SYMBOLS) == names
+    assert all(hasattr(L, n) for n in names)
+
+
 def test_no_device_means_loud_failure():
     import diskfft_amd
     if diskfft_amd.device_count() > 0:

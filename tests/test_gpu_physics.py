@@ -1,0 +1,88 @@
+"""GPU loop over many steps: snapshot/resume and the m = 2 growth rate.
+
+The north star asks gamma and Omega_p within 1% of the FP64 CPU path from the
+same state.  The reference's own 128 x 128 setup (diskfft/README.md:27-30:
+LKA98 ring, tlim 9, amp 1e-3) runs in a couple of seconds on both sides.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dfk():
+    import diskfft_amd
+    if diskfft_amd.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return diskfft_amd
+
+
+def _lka(orc, Nu, Nphi, eps, amp=1e-3):
+    h = orc.Hydro(Nu, Nphi)
+    k = orc.Kalnajs(Nu, Nphi, 0.25, 1.0, eps)
+    orc.ic_lka98(h, k, 1, amp)
+    return h, k
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_snapshot_resume_is_bitwise(orc, dfk, tmp_path, kind):
+    print(f"case snapshot resume kind {kind}", flush=True)
+    Nu, Nphi, eps = 64, 64, 0.02
+    h, _ = _lka(orc, Nu, Nphi, eps)
+    st, mD = h.state(), h.disk_mass()
+    p = dfk.disk_params(Nu, Nphi, eps=eps)
+    a = dfk.Ensemble(p)
+    a.set_update_kernel(kind)
+    a.set_state(*st)
+    a.run_init(1e9, dtout=0.01, mD=mD, max_steps=40)
+    a.advance(45)
+    sa = a.samples()
+    b = dfk.Ensemble(p)
+    b.set_update_kernel(kind)
+    b.set_state(*st)
+    b.run_init(1e9, dtout=0.01, mD=mD, max_steps=40)
+    b.advance(20)
+    s1 = b.samples()
+    path = os.path.join(tmp_path, "disk.snap")
+    b.snapshot_save(path)
+    b.close()
+    c = dfk.Ensemble(p)
+    c.set_update_kernel(kind)
+    c.run_init(1e9, dtout=0.01, mD=mD, max_steps=40)
+    c.snapshot_load(path)
+    c.advance(25)
+    s2 = c.samples()
+    assert c.status()["steps"] == 40 and c.status()["t"] == a.status()["t"]
+    for x, y in zip(a.get_state(), c.get_state()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(np.concatenate([s1, s2]), sa)
+    with pytest.raises(dfk.DiskfftError):
+        d = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=0.03))
+        d.run_init(1.0, mD=mD)
+        d.snapshot_load(path)
+
+
+def test_growth_rate_matches_oracle(orc, dfk):
+    """LKA98 ring, 128 x 128, eps 0.0025, tlim 9: gamma and Omega_p of the m=2
+    mode from the GPU loop within 1% of the oracle's."""
+    print("case growth rate 128x128 eps=0.0025 tlim=9", flush=True)
+    Nu = Nphi = 128
+    eps, tlim = 0.0025, 9.0
+    h, k = _lka(orc, Nu, Nphi, eps)
+    mD = h.disk_mass()
+    e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
+    e.set_state(*h.state())
+    ((sg, stg),) = e.run(tlim, dtout=0.25, cfl=0.3, mD=mD)
+    so, tf, c2f, steps = orc.run(h, k, 1, tlim, dtout=0.25, cfl=0.3, mD=mD)
+    gg = dfk.extract_growth(sg)
+    go = dfk.extract_growth(so)
+    print("  gpu", gg, "\n  oracle", go, f"\n  steps {stg['steps']} vs {steps}", flush=True)
+    assert gg["grew"] and go["grew"]
+    assert abs(gg["gamma"] - go["gamma"]) / abs(go["gamma"]) < 0.01
+    assert abs(gg["omega_p"] - go["omega_p"]) / abs(go["omega_p"]) < 0.01
+    # reference result for this setup (diskfft/README.md:27): gamma 0.757, Omega_p 1.49
+    assert abs(gg["gamma"] - 0.757) / 0.757 < 0.10
+    assert abs(gg["omega_p"] - 1.49) / 1.49 < 0.10
