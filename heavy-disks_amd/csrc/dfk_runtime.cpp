@@ -621,6 +621,18 @@ void dfk_destroy(dfk_handle *h) {
 
 int dfk_num_disks(const dfk_handle *h) { return h ? h->nd : 0; }
 
+int dfk_set_eos(dfk_handle *h, int d, double K, double gamma) {
+  TRY(check_disk(h, d));
+  DiskDev &D = h->hD[d];
+  D.K = K;
+  D.gamma = gamma;
+  D.gamma2 = (gamma == 2.0);
+  h->P[d].K = K;
+  h->P[d].gamma = gamma;
+  h->layout_ok = false;   // re-uploads the descriptors
+  return build_layout(h);
+}
+
 int dfk_geometry(dfk_handle *h, int d, double *R, double *Rf, double *Vol) {
   TRY(check_disk(h, d));
   const dfk_disk_params &p = h->P[d];

@@ -33,7 +33,7 @@ SYMBOLS = [
     "dfk_run_init", "dfk_advance", "dfk_samples", "dfk_get_status", "dfk_set_shapes",
     "dfk_get_shapes", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
     "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
-    "dfk_snapshot_save", "dfk_snapshot_load",
+    "dfk_snapshot_save", "dfk_snapshot_load", "dfk_set_eos",
 ]
 
 
@@ -104,6 +104,7 @@ def lib():
             "dfk_get_update_kernel": (_I, [_P, _IP]),
             "dfk_snapshot_save": (_I, [_P, _I, ctypes.c_char_p]),
             "dfk_snapshot_load": (_I, [_P, _I, ctypes.c_char_p]),
+            "dfk_set_eos": (_I, [_P, _I, _D, _D]),
         }
         for k, (res, args) in sig.items():
             f = getattr(L, k)
@@ -293,6 +294,11 @@ class Ensemble:
         k = _I()
         _check(lib().dfk_get_update_kernel(self.h, ctypes.byref(k)))
         return k.value
+
+    def set_eos(self, K, gamma=2.0, d=0):
+        _check(lib().dfk_set_eos(self.h, d, K, gamma))
+        self.params[d]["K"] = K
+        self.params[d]["gamma"] = gamma
 
     def snapshot_save(self, path, d=0):
         _check(lib().dfk_snapshot_save(self.h, d, os.fsencode(path)))

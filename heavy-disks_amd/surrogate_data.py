@@ -1,0 +1,118 @@
+"""Training data for the surrogate: batched hydro runs of the power-law disk.
+
+Each run draws (M_d/M*, Q_min, softening s, slope) from the ensemble
+distribution (ensemble_cfg.draw), builds its equilibrium with this library's
+GPU gravity, and advances ~20 outer orbits (t = 20 * 2 pi / sqrt(G (M* + M_d))
+at R = 1) in one batched handle with every other run of the shard.  The
+extractor (dfk_extract_growth) turns each run's (t, |C0|, |C2|, arg C2)
+series into the targets (gamma, Omega_p, saturated |C2|/|C0|).
+
+All runs use one grid (default 128 x 128) so a shard is a uniform batch; the
+grid is a numerical choice, not a surrogate input.
+
+    python heavy-disks_amd/surrogate_data.py --first 0 --count 1024 --out shard0.npz
+"""
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import autotune  # noqa: E402
+import diskfft_amd as dfk  # noqa: E402
+import ensemble_cfg  # noqa: E402
+
+ORBITS = 20
+
+
+def tlim_for(d):
+    return ORBITS * 2.0 * np.pi / np.sqrt(1.0 + d["mass_ratio"])
+
+
+def run_shard(first, count, Nu=128, Nphi=128, seed=2026, total=4352, chunk=4000,
+              max_wall=None, verbose=True):
+    disks = ensemble_cfg.draw(total, seed=seed, grids=[(Nu, Nphi)])[first:first + count]
+    t0 = time.perf_counter()
+    # one handle; gravity of the axisymmetric disks first (K does not enter it)
+    params = [dfk.disk_params(Nu, Nphi, Rin=0.1, Rout=1.0, K=1.0, gmstar=1.0, eps=d["eps"])
+              for d in disks]
+    ens = dfk.Ensemble(params)
+    mDs, Ks, Qgot = [], [], []
+    for k, d in enumerate(disks):
+        m = ensemble_cfg.model_for(dfk, d)
+        sig0 = dfk.model_sigma0(m, Nu, Nphi)
+        Sax = np.repeat(sig0[:, None], Nphi, axis=1)
+        gR, _ = ens.forces(ens.potential(Sax, d=k), d=k)
+        gRdisk = np.cumsum(gR, axis=1)[:, -1] / Nphi
+        S, U, W, _ = dfk.model_state(m, Nu, Nphi, gRdisk)
+        ens.set_eos(m.K, 2.0, d=k)
+        ens.set_state(S, U, W, d=k)
+        mDs.append(dfk.model_mass(m, S))
+        Ks.append(m.K)
+        Qgot.append(m.Qmin_got)
+    autotune.apply(ens, regime="throughput")
+    t_setup = time.perf_counter() - t0
+    tl = [tlim_for(d) for d in disks]
+    ens.set_graph_steps(200)
+    ens.run_init(tl, dtout=0.25, cfl=0.4, mD=mDs, stop_on_nonfinite=1, sample_cap=256)
+    series = [[] for _ in disks]
+    t1 = time.perf_counter()
+    done = False
+    while not done:
+        done = ens.advance(chunk)
+        for k in range(len(disks)):
+            s = ens.samples(k)
+            if len(s):
+                series[k].append(s)
+        if verbose:
+            st = [ens.status(k) for k in range(0, len(disks), max(1, len(disks) // 4))]
+            print(f"  shard {first}: {time.perf_counter() - t1:.0f}s, t = "
+                  f"{[round(x['t'], 1) for x in st]}", flush=True)
+        if max_wall and time.perf_counter() - t1 > max_wall:
+            break
+    wall = time.perf_counter() - t1
+    steps = np.array([ens.status(k)["steps"] for k in range(len(disks))])
+    flags = np.array([ens.status(k)["flags"] for k in range(len(disks))])
+    tend = np.array([ens.status(k)["t"] for k in range(len(disks))])
+    X, Y, ok = [], [], []
+    nmax = max(sum(len(x) for x in s) for s in series)
+    ser = np.full((len(disks), nmax, 4), np.nan, dtype=np.float32)
+    for k, d in enumerate(disks):
+        s = np.concatenate(series[k]) if series[k] else np.zeros((0, 4))
+        ser[k, :len(s)] = s
+        X.append([d["mass_ratio"], d["Qmin"], np.sqrt(d["eps"]), d["slope"]])
+        good = len(s) >= 8 and flags[k] == 0 and tend[k] >= tl[k] * (1 - 1e-9)
+        g = dfk.extract_growth(s) if len(s) >= 4 else dict(gamma=np.nan, omega_p=np.nan, sat=np.nan)
+        Y.append([g["gamma"], g["omega_p"], g["sat"]])
+        ok.append(good)
+    zones = float(Nu * Nphi) * steps.sum()
+    ens.close()
+    return dict(X=np.array(X), Y=np.array(Y), ok=np.array(ok), K=np.array(Ks),
+                Qgot=np.array(Qgot), steps=steps, flags=flags, tlim=np.array(tl),
+                series=ser, wall=wall, setup=t_setup, zone_updates=zones,
+                first=first, grid=np.array([Nu, Nphi]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--first", type=int, default=0)
+    ap.add_argument("--count", type=int, default=1024)
+    ap.add_argument("--total", type=int, default=4352)
+    ap.add_argument("--nu", type=int, default=128)
+    ap.add_argument("--nphi", type=int, default=128)
+    ap.add_argument("--max-wall", type=float, default=None)
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args()
+    r = run_shard(a.first, a.count, a.nu, a.nphi, total=a.total, max_wall=a.max_wall)
+    np.savez_compressed(a.out, **r)
+    print(f"shard {a.first}+{a.count}: {r['ok'].sum()} good runs, {r['zone_updates']:.3e} "
+          f"zone-updates in {r['wall']:.1f}s ({r['zone_updates'] / r['wall']:.3e}/s), setup "
+          f"{r['setup']:.1f}s", flush=True)
+
+
+if __name__ == "__main__":
+    main()

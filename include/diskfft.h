@@ -76,6 +76,11 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
 void dfk_destroy(dfk_handle *h);
 int dfk_num_disks(const dfk_handle *h);
 
+/* Changes the EOS constants (P = K Sigma^gamma) of one disk after creation;
+ * used when K comes out of an equilibrium solve that needed this handle's
+ * gravity first.  Takes effect for the next dfk_run_init / unit call. */
+int dfk_set_eos(dfk_handle *h, int disk, double K, double gamma);
+
 /* Geometry tables of DiskHydro: R (Nu), Rf (Nu+1), Vol (Nu) (hydro.hpp:31-38). */
 int dfk_geometry(dfk_handle *h, int disk, double *R, double *Rf, double *Vol);
 
