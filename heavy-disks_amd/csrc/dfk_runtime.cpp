@@ -320,6 +320,13 @@ int get_graph(dfk_handle *h, int nsteps, hipGraphExec_t *out) {
 int drain(dfk_handle *h) {
   std::vector<DiskCtl> c(h->nd);
   HIPCHK(hipMemcpy(c.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
+  bool any = false;
+  for (int d = 0; d < h->nd; ++d) any |= c[d].nsamples > h->drained[d];
+  std::vector<DfkSampleDev> ring;
+  if (any) {
+    ring.resize((size_t)h->nd * h->cap);
+    HIPCHK(hipMemcpy(ring.data(), h->samples, sizeof(DfkSampleDev) * ring.size(), hipMemcpyDeviceToHost));
+  }
   for (int d = 0; d < h->nd; ++d) {
     h->hcur[d] = c[d].cur;
     const int n = c[d].nsamples, from = h->drained[d];
@@ -328,8 +335,7 @@ int drain(dfk_handle *h) {
       return DFK_ESTATE;
     }
     for (int q = from; q < n; ++q) {
-      DfkSampleDev s;
-      HIPCHK(hipMemcpy(&s, h->samples + (i64)d * h->cap + (q % h->cap), sizeof s, hipMemcpyDeviceToHost));
+      const DfkSampleDev s = ring[(size_t)d * h->cap + (q % h->cap)];
       dfk_sample o;
       const double mD = h->mD[d];
       o.t = s.t;

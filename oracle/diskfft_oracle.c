@@ -654,10 +654,28 @@ void orc_ic_lka98(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double amp) {
 }
 
 /* diskmain.cpp:86-113 (max_steps is the added --steps cap; <=0 = none) */
+int orc_run_from(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double tlim, double dtout,
+                 double cfl, long max_steps, double mD, orc_sample *samples, int cap,
+                 double *t_io, double *tnext_io, long *step_io, double *c2_final);
 int orc_run(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double tlim,
             double dtout, double cfl, long max_steps, double mD,
             orc_sample *samples, int cap, double *t_final, double *c2_final,
             long *steps_out) {
+  double t = 0.0, tnext = 0.0;
+  long step = 0;
+  int ns = orc_run_from(h, grav, selfgrav, tlim, dtout, cfl, max_steps, mD, samples, cap,
+                        &t, &tnext, &step, c2_final);
+  if (t_final) *t_final = t;
+  if (steps_out) *steps_out = step;
+  return ns;
+}
+
+/* The same loop continued from (t, tnext, step); they are updated in place.
+ * max_steps caps the total step count (<= 0: none). */
+int orc_run_from(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double tlim,
+                 double dtout, double cfl, long max_steps, double mD,
+                 orc_sample *samples, int cap, double *t_io, double *tnext_io,
+                 long *step_io, double *c2_final) {
   const size_t N = (size_t)h->Nu * h->Nphi;
   double *Phi = NULL, *gR = NULL, *gp = NULL;
   if (selfgrav) {
@@ -665,8 +683,8 @@ int orc_run(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double tlim,
     gR = (double *)malloc(sizeof(double) * N);
     gp = (double *)malloc(sizeof(double) * N);
   }
-  double t = 0.0, tnext = 0.0, c0, c2, ph2, ph0;
-  long step = 0;
+  double t = *t_io, tnext = *tnext_io, c0, c2, ph2, ph0;
+  long step = *step_io;
   int ns = 0;
   while (t < tlim && (max_steps <= 0 || step < max_steps)) {
     if (t >= tnext - 1e-12) {
@@ -689,9 +707,10 @@ int orc_run(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double tlim,
     t += dt; ++step;
   }
   orc_cm(h, 2, mD, &c2, &ph2);
-  if (t_final) *t_final = t;
+  *t_io = t;
+  *tnext_io = tnext;
+  *step_io = step;
   if (c2_final) *c2_final = c2;
-  if (steps_out) *steps_out = step;
   free(Phi); free(gR); free(gp);
   return ns;
 }

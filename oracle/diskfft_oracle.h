@@ -106,6 +106,12 @@ int orc_run(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double tlim,
             orc_sample *samples, int cap, double *t_final, double *c2_final,
             long *steps_out);
 
+/* The same loop continued from (*t_io, *tnext_io, *step_io), updated in place. */
+int orc_run_from(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double tlim,
+                 double dtout, double cfl, long max_steps, double mD,
+                 orc_sample *samples, int cap, double *t_io, double *tnext_io,
+                 long *step_io, double *c2_final);
+
 void orc_set_threads(int n);
 int orc_get_threads(void);
 

@@ -54,6 +54,9 @@ def load(fast=False):
         "orc_run": (_I, [_P, _P, _I, _D, _D, _D, ctypes.c_long, _D,
                          ctypes.POINTER(Sample), _I, _DP, _DP,
                          ctypes.POINTER(ctypes.c_long)]),
+        "orc_run_from": (_I, [_P, _P, _I, _D, _D, _D, ctypes.c_long, _D,
+                              ctypes.POINTER(Sample), _I, _DP, _DP,
+                              ctypes.POINTER(ctypes.c_long), _DP]),
         "orc_set_threads": (None, [_I]),
         "orc_get_threads": (_I, []),
     }
@@ -244,6 +247,30 @@ def run(h, k, selfgrav, tlim, dtout=0.25, cfl=0.3, max_steps=0, mD=None, cap=100
     samples = np.array([(buf[i].t, buf[i].c0, buf[i].c2, buf[i].ph2)
                         for i in range(min(n, cap))]).reshape(-1, 4)
     return samples, tf.value, c2f.value, st.value
+
+
+def run_chunked(h, k, selfgrav, tlim, dtout=0.25, cfl=0.3, max_steps=0, mD=None, chunk=20000,
+                progress=None):
+    """orc_run in pieces of `chunk` steps (same result); progress(t, steps)
+    is called after each piece."""
+    if mD is None:
+        mD = h.disk_mass()
+    t, tn, st = _D(0.0), _D(0.0), ctypes.c_long(0)
+    c2f = _D()
+    out = []
+    cap = 1 << 16
+    buf = (Sample * cap)()
+    while True:
+        cap_steps = st.value + chunk if max_steps <= 0 else min(max_steps, st.value + chunk)
+        n = h.L.orc_run_from(h.h, k.k if k is not None else None, selfgrav, tlim, dtout, cfl,
+                             cap_steps, mD, buf, cap, ctypes.byref(t), ctypes.byref(tn),
+                             ctypes.byref(st), ctypes.byref(c2f))
+        out.extend((buf[i].t, buf[i].c0, buf[i].c2, buf[i].ph2) for i in range(min(n, cap)))
+        if progress:
+            progress(t.value, st.value)
+        if not (t.value < tlim) or (max_steps > 0 and st.value >= max_steps):
+            break
+    return np.array(out).reshape(-1, 4), t.value, c2f.value, st.value
 
 
 def set_threads(n):

@@ -49,13 +49,16 @@ def run_case(name):
     e.set_state(S, U, W)
     t0 = time.perf_counter()
     ((sg, stg),) = e.run(c["tlim"], dtout=0.25, cfl=c["cfl"], mD=mD, max_steps=c["steps"],
-                         chunk=5000)
+                         chunk=20000)
+    print(f"  {name} gpu: {stg['steps']} steps to t = {stg['t']:.2f}", flush=True)
     tg = time.perf_counter() - t0
     h = oracle.Hydro(Nu, Nphi, p["Rin"], p["Rout"], p["K"], p["gamma"], p["gmstar"], fast=True)
     h.set_state(S, U, W)
     t0 = time.perf_counter()
-    so, tf, c2f, steps = oracle.run(h, k, 1, c["tlim"], dtout=0.25, cfl=c["cfl"],
-                                    max_steps=c["steps"], mD=mD, cap=1 << 20)
+    so, tf, c2f, steps = oracle.run_chunked(
+        h, k, 1, c["tlim"], dtout=0.25, cfl=c["cfl"], max_steps=c["steps"], mD=mD, chunk=20000,
+        progress=lambda t, n: print(f"  {name} oracle: t = {t:.2f} after {n} steps "
+                                    f"({time.perf_counter() - t0:.0f} s)", flush=True))
     to = time.perf_counter() - t0
     gg, go = dfk.extract_growth(sg), dfk.extract_growth(so)
     rel = {q: abs(gg[q] - go[q]) / abs(go[q]) if go[q] else None
