@@ -72,7 +72,6 @@ struct DiskCtl {
   double fin_re2, fin_im2;   // C_2 sums at loop end
   long long step, max_steps;
   int cur;                   // state buffer (0/1) holding the current state
-  int src, dst;              // buffers read / written by the step in flight
   int done;                  // loop finished (t >= tlim, step cap, or failure)
   int nsamples;              // samples recorded so far (ring buffer index)
   int flags;                 // DFK_FLAG_*

@@ -106,7 +106,36 @@ __device__ __forceinline__ void bfly3(double2 (&v)[3]) {
   v[2] = csub(c, is);
 }
 
-// One Stockham stage of radix R in {2, 3, 4} over `batch` transforms of
+// Radix 8 as radix 2 then radix 4: a_m = v_m + v_{m+4}, b_m = (v_m - v_{m+4}) W8^m,
+// X_{2k} = DFT4(a)_k, X_{2k+1} = DFT4(b)_k, W8 = exp(SIGN 2 pi i / 8).
+template <int SIGN>
+__device__ __forceinline__ void bfly8(double2 (&v)[8]) {
+  const double h = 0.70710678118654752440;
+  double2 a[4], b[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    a[m] = cadd(v[m], v[m + 4]);
+    b[m] = csub(v[m], v[m + 4]);
+  }
+  if (SIGN < 0) {   // W8 = (1 - i)/sqrt2
+    b[1] = make_double2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+    b[2] = make_double2(b[2].y, -b[2].x);
+    b[3] = make_double2(-h * (b[3].x - b[3].y), -h * (b[3].x + b[3].y));
+  } else {          // W8 = (1 + i)/sqrt2
+    b[1] = make_double2(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y));
+    b[2] = make_double2(-b[2].y, b[2].x);
+    b[3] = make_double2(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y));
+  }
+  bfly4<SIGN>(a);
+  bfly4<SIGN>(b);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[2 * k] = a[k];
+    v[2 * k + 1] = b[k];
+  }
+}
+
+// One Stockham stage of radix R in {2, 3, 4, 8} over `batch` transforms of
 // length n stored contiguously in LDS (a -> b).  Ns = product of the radices
 // already applied.  Twiddles tw[t] = exp(-2 pi i t / n), conjugated for SIGN > 0.
 template <int R, int SIGN>
@@ -134,6 +163,7 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
     if constexpr (R == 2) bfly2<SIGN>(v[0], v[1]);
     if constexpr (R == 3) bfly3<SIGN>(v);
     if constexpr (R == 4) bfly4<SIGN>(v);
+    if constexpr (R == 8) bfly8<SIGN>(v);
 #pragma unroll
     for (int q = 0; q < R; ++q) dst[q * Ns] = v[q];
   }
@@ -173,6 +203,12 @@ __device__ void fft_stage_generic(const double2 *__restrict__ a, double2 *__rest
   }
 }
 
+// Copy the n twiddles of a transform into LDS (read many times per stage).
+__device__ __forceinline__ void stage_twiddles(double2 *__restrict__ dst,
+                                               const double2 *__restrict__ src, int n) {
+  for (int e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
+}
+
 // Full transform of `batch` rows of length n in LDS: returns the buffer that
 // holds the result (a or b).  One barrier per stage.
 template <int SIGN>
@@ -181,7 +217,8 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int n, int batch, int nst,
   int Ns = 1;
   for (int s = 0; s < nst; ++s) {
     const int r = rad[s];
-    if (r == 4) fft_stage<4, SIGN>(a, b, n, batch, Ns, tw);
+    if (r == 8) fft_stage<8, SIGN>(a, b, n, batch, Ns, tw);
+    else if (r == 4) fft_stage<4, SIGN>(a, b, n, batch, Ns, tw);
     else if (r == 2) fft_stage<2, SIGN>(a, b, n, batch, Ns, tw);
     else if (r == 3) fft_stage<3, SIGN>(a, b, n, batch, Ns, tw);
     else fft_stage_generic<SIGN>(a, b, n, batch, Ns, r, tw);
@@ -246,11 +283,14 @@ __device__ __forceinline__ double *stpw(double *st, i64 ntot, int buf, int f, co
 // ------------------------------------------------------------------ control
 // One workgroup per disk.  Reduces the per-tile partials of the state that
 // the previous update (or the initial diag) produced, then performs the top
-// of one iteration of diskmain.cpp:92-101.
+// of one iteration of diskmain.cpp:92-101.  It touches only the control block
+// and the sample ring, so it runs concurrently with the gravity kernels of
+// the same iteration (they read `done` only, and a stale `done` costs at most
+// one wasted transform).  `par` is the iteration's state buffer.
 __global__ void __launch_bounds__(WG)
 k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
           const double *__restrict__ pcfl, const double *__restrict__ pcm,
-          DfkSampleDev *__restrict__ samples, int cap) {
+          DfkSampleDev *__restrict__ samples, int cap, int par) {
   __shared__ double red[5 * (WG / 64)];
   const int d = blockIdx.x;
   const DiskDev &D = disks[d];
@@ -268,10 +308,12 @@ k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
   }
   wg_reduce5(mx, s0, s1, s2, s3, red);
   if (threadIdx.x != 0) return;
+  // the current state of every disk still running is in buffer `par`
   if (s3 > 0.0) {
     C.flags |= DFK_FLAG_NONFINITE;
     if (C.stop_on_nonfinite) {
       C.done = 1;
+      C.cur = par;
       C.fin_re2 = s1;
       C.fin_im2 = s2;
       return;
@@ -280,6 +322,7 @@ k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
   const double t = C.t;
   if (!(t < C.tlim) || (C.max_steps > 0 && C.step >= C.max_steps)) {
     C.done = 1;
+    C.cur = par;
     C.fin_re2 = s1;
     C.fin_im2 = s2;
     return;
@@ -296,9 +339,7 @@ k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
   C.dt = dt;
   C.t = t + dt;
   C.step += 1;
-  C.src = C.cur;
-  C.dst = 1 - C.cur;
-  C.cur = C.dst;
+  C.cur = 1 - par;   // where this iteration's update writes
 }
 
 // ------------------------------------------------------------------ diag
@@ -350,7 +391,7 @@ __global__ void __launch_bounds__(WG)
 k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
            const DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,
            const double *__restrict__ geom, const double2 *__restrict__ tw,
-           double2 *__restrict__ spec) {
+           double2 *__restrict__ spec, int par) {
   extern __shared__ double2 lds2[];
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
@@ -358,8 +399,9 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   if (C.done) return;
   const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * wm.y;
-  double2 *A = lds2, *B = lds2 + pr * n;
-  const double *S = stp(st, ntot, C.src, 0, D);
+  double2 *A = lds2, *B = lds2 + pr * n, *TW = lds2 + 2 * pr * n;
+  stage_twiddles(TW, tw + D.two_row, n);
+  const double *S = stp(st, ntot, par, 0, D);
   const double *R32 = gtab(geom, D, GT_R32);
   for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
     const int p = e / n, j = e - p * n;
@@ -370,7 +412,7 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
     A[e] = make_double2(xa, xb);
   }
   __syncthreads();
-  const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
+  const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
   double2 *out = spec + D.spo;
   const int w2 = 2 * pr;
   for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
@@ -400,7 +442,8 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   if (ctl[wm.x].done) return;
   const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
   const int k0 = wm.y * cb;
-  double2 *A = lds2, *B = lds2 + cb * M;
+  double2 *A = lds2, *B = lds2 + cb * M, *TW = lds2 + 2 * cb * M;
+  stage_twiddles(TW, tw + D.two_col, M);
   double2 *io = spec + D.spo;
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
@@ -410,7 +453,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
     A[e] = v;
   }
   __syncthreads();
-  double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
+  double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, TW);
   double2 *Wb = (Z == A) ? B : A;
   const double *kh = khat + D.kho;
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
@@ -424,7 +467,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
     Z[e] = z;
   }
   __syncthreads();
-  const double2 *Y = lds_fft<+1>(Z, Wb, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
+  const double2 *Y = lds_fft<+1>(Z, Wb, M, cb, D.nst_col, D.rad_col, TW);
   for (int e = threadIdx.x; e < cb * Nu; e += blockDim.x) {
     const int c = e / Nu, p = e - c * Nu;
     const int k = k0 + c;
@@ -445,7 +488,8 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   if (ctl[wm.x].done) return;
   const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * wm.y;
-  double2 *A = lds2, *B = lds2 + pr * n;
+  double2 *A = lds2, *B = lds2 + pr * n, *TW = lds2 + 2 * pr * n;
+  stage_twiddles(TW, tw + D.two_row, n);
   const double2 *in = spec + D.spo;
   for (int e = threadIdx.x; e < pr * Nc; e += blockDim.x) {
     const int k = e / pr, p = e - k * pr;
@@ -458,7 +502,7 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
     if (k > 0 && n - k >= Nc) A[p * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
   }
   __syncthreads();
-  const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
+  const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
   const double *sc = gtab(geom, D, GT_PHISC);
   double *P = phi + D.so;
   for (int e = threadIdx.x; e < 2 * pr * n; e += blockDim.x) {
@@ -481,14 +525,15 @@ k_khat_rows(const DiskDev *__restrict__ D_, const double *__restrict__ ktab,
   const DiskDev &D = *D_;
   const int n = D.Nphi, M = D.M, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * blockIdx.x;
-  double2 *A = lds2, *B = lds2 + pr * n;
+  double2 *A = lds2, *B = lds2 + pr * n, *TW = lds2 + 2 * pr * n;
+  stage_twiddles(TW, tw + D.two_row, n);
   for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     A[e] = make_double2(ra < M ? ktab[(i64)ra * n + j] : 0.0, rb < M ? ktab[(i64)rb * n + j] : 0.0);
   }
   __syncthreads();
-  const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, tw + D.two_row);
+  const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
   const int w2 = 2 * pr;
   for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
     const int k = e / w2, rr = e - k * w2;
@@ -512,14 +557,15 @@ k_khat_cols(const DiskDev *__restrict__ D_, const double2 *__restrict__ tw,
   const DiskDev &D = *D_;
   const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
   const int k0 = blockIdx.x * cb;
-  double2 *A = lds2, *B = lds2 + cb * M;
+  double2 *A = lds2, *B = lds2 + cb * M, *TW = lds2 + 2 * cb * M;
+  stage_twiddles(TW, tw + D.two_col, M);
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
     A[e] = (k < Nc) ? tmp[(i64)k * M + p] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, tw + D.two_col);
+  const double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, TW);
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
@@ -541,7 +587,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
          const double *__restrict__ phi, const double *__restrict__ gRa,
          const double *__restrict__ gpa, int has_gR, int has_gp,
          const double *__restrict__ geom, const double *__restrict__ trig,
-         double *__restrict__ pcfl, double *__restrict__ pcm) {
+         double *__restrict__ pcfl, double *__restrict__ pcm, int par) {
   extern __shared__ double lds[];
   __shared__ double red[5 * (WG / 64)];
   const int2 wm = map[map_base + blockIdx.x];
@@ -556,8 +602,8 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   double *sS = lds, *sU = sS + SH * SW, *sW = sU + SH * SW;
   double *fM = sW + SH * SW, *fU = fM + (TI + 1) * SW, *fW = fU + (TI + 1) * SW;
 
-  const double *S = stp(st, ntot, C.src, 0, D), *U = stp(st, ntot, C.src, 1, D),
-               *W = stp(st, ntot, C.src, 2, D);
+  const double *S = stp(st, ntot, par, 0, D), *U = stp(st, ntot, par, 1, D),
+               *W = stp(st, ntot, par, 2, D);
   const double *Ph = phi + D.so;
   const double *tR = gtab(geom, D, GT_R);
   (void)tR;
@@ -683,8 +729,8 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   __syncthreads();
 
   // ---- phase 5: azimuthal gather + floor, write, next-step CFL and C_m partials
-  double *So = stpw(st, ntot, C.dst, 0, D), *Uo = stpw(st, ntot, C.dst, 1, D),
-         *Wo = stpw(st, ntot, C.dst, 2, D);
+  double *So = stpw(st, ntot, 1 - par, 0, D), *Uo = stpw(st, ntot, 1 - par, 1, D),
+         *Wo = stpw(st, ntot, 1 - par, 2, D);
   const double *dA = gtab(geom, D, GT_DA);
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
@@ -741,7 +787,7 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
                 const double *__restrict__ phi, const double *__restrict__ gRa,
                 const double *__restrict__ gpa, int has_gR, int has_gp,
                 const double *__restrict__ geom, const double *__restrict__ trig,
-                double *__restrict__ pcfl, double *__restrict__ pcm) {
+                double *__restrict__ pcfl, double *__restrict__ pcm, int par) {
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
   const DiskCtl &C = ctl[wm.x];
@@ -758,10 +804,10 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   const int cm = (c == 0) ? Nphi - 1 : c - 1, cp = (c == Nphi - 1) ? 0 : c + 1;
   const bool out_ok = lane >= 2 && lane < 2 + STREAM_OUT && (j0 + lane - 2) < Nphi;
 
-  const double *S = stp(st, ntot, C.src, 0, D), *U = stp(st, ntot, C.src, 1, D),
-               *W = stp(st, ntot, C.src, 2, D);
-  double *So = stpw(st, ntot, C.dst, 0, D), *Uo = stpw(st, ntot, C.dst, 1, D),
-         *Wo = stpw(st, ntot, C.dst, 2, D);
+  const double *S = stp(st, ntot, par, 0, D), *U = stp(st, ntot, par, 1, D),
+               *W = stp(st, ntot, par, 2, D);
+  double *So = stpw(st, ntot, 1 - par, 0, D), *Uo = stpw(st, ntot, 1 - par, 1, D),
+         *Wo = stpw(st, ntot, 1 - par, 2, D);
   const double *Ph = phi + D.so;
   const double *tInvdRc = gtab(geom, D, GT_INVDRC), *tInvR = gtab(geom, D, GT_INVR),
                *tGstar = gtab(geom, D, GT_GSTAR), *tGrc = gtab(geom, D, GT_GRC),
@@ -1042,9 +1088,11 @@ __global__ void k_floor(const DiskDev *__restrict__ Dp, double *__restrict__ st,
 #define CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
 
 hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
-                              const double *pcfl, const double *pcm, DfkSampleDev *samples, int cap) {
+                              const double *pcfl, const double *pcm, DfkSampleDev *samples, int cap,
+                              int par) {
   if (ndisk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_control, dim3(ndisk), dim3(WG), 0, s, disks, ctl, pcfl, pcm, samples, cap);
+  hipLaunchKernelGGL(k_control, dim3(ndisk), dim3(WG), 0, s, disks, ctl, pcfl, pcm, samples, cap,
+                     par);
   return hipGetLastError();
 }
 
@@ -1059,10 +1107,10 @@ hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const 
 
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double *st, i64 ntot,
-                          const double *geom, const double2 *tw, double2 *spec) {
+                          const double *geom, const double2 *tw, double2 *spec, int par) {
   if (nblk <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_rows_r2c, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot, geom,
-                     tw, spec);
+                     tw, spec, par);
   return hipGetLastError();
 }
 
@@ -1088,14 +1136,14 @@ hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int 
                              const int2 *map, int base, const DiskCtl *ctl, double *st, i64 ntot,
                              const double *phi, const double *gR, const double *gp, int has_gR,
                              int has_gp, const double *geom, const double *trig, double *pcfl,
-                             double *pcm) {
+                             double *pcm, int par) {
   if (nblk <= 0) return hipSuccess;
   if (garr)
     hipLaunchKernelGGL(k_update<true>, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot,
-                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
   else
     hipLaunchKernelGGL(k_update<false>, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot,
-                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
   return hipGetLastError();
 }
 
@@ -1103,14 +1151,14 @@ hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, const Dis
                                     const int2 *map, int base, const DiskCtl *ctl, double *st,
                                     i64 ntot, const double *phi, const double *gR, const double *gp,
                                     int has_gR, int has_gp, const double *geom, const double *trig,
-                                    double *pcfl, double *pcm) {
+                                    double *pcfl, double *pcm, int par) {
   if (nblk <= 0) return hipSuccess;
   if (garr)
     hipLaunchKernelGGL(k_update_stream<true>, dim3(nblk), dim3(64), 0, s, disks, map, base, ctl, st,
-                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
   else
     hipLaunchKernelGGL(k_update_stream<false>, dim3(nblk), dim3(64), 0, s, disks, map, base, ctl, st,
-                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm);
+                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
   return hipGetLastError();
 }
 

@@ -8,13 +8,14 @@
 
 hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
                               const double *pcfl, const double *pcm, DfkSampleDev *samples,
-                              int cap);
+                              int cap, int par);
 hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const int2 *map,
                            int base, const DiskCtl *ctl, const double *st, long long ntot,
                            const double *geom, const double *trig, double *pcfl, double *pcm);
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double *st,
-                          long long ntot, const double *geom, const double2 *tw, double2 *spec);
+                          long long ntot, const double *geom, const double2 *tw, double2 *spec,
+                          int par);
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                            const int2 *map, int base, const DiskCtl *ctl, const double *khat,
                            const double2 *tw, double2 *spec);
@@ -25,12 +26,12 @@ hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int 
                              const int2 *map, int base, const DiskCtl *ctl, double *st,
                              long long ntot, const double *phi, const double *gR,
                              const double *gp, int has_gR, int has_gp, const double *geom,
-                             const double *trig, double *pcfl, double *pcm);
+                             const double *trig, double *pcfl, double *pcm, int par);
 hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, const DiskDev *disks,
                                     const int2 *map, int base, const DiskCtl *ctl, double *st,
                                     long long ntot, const double *phi, const double *gR,
                                     const double *gp, int has_gR, int has_gp, const double *geom,
-                                    const double *trig, double *pcfl, double *pcm);
+                                    const double *trig, double *pcfl, double *pcm, int par);
 hipError_t dfk_launch_khat(hipStream_t s, const DiskDev *D, int nrow_blk, int ncol_blk,
                            size_t lds_rows, size_t lds_cols, const double *ktab,
                            const double2 *tw, double2 *tmp, double *khat);

@@ -52,9 +52,10 @@ void set_err(const char *fmt, ...) {
     if (r_ != DFK_OK) return r_; \
   } while (0)
 
-// Radix factorisation used by both transforms (4s first, then 2, then 3, 5, 7, 11, 13).
+// Radix factorisation used by both transforms (8s first, then 4, 2, 3, 5, 7, 11, 13).
 bool factor(int n, int *rad, int *nst) {
   int m = n, k = 0;
+  while (m % 8 == 0) { rad[k++] = 8; m /= 8; }
   while (m % 4 == 0) { rad[k++] = 4; m /= 4; }
   while (m % 2 == 0) { rad[k++] = 2; m /= 2; }
   for (int q : {3, 5, 7, 11, 13})
@@ -121,6 +122,9 @@ struct dfk_handle {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   int chunk = 128;               // loop iterations per captured graph
+  long long iter = 0;            // loop iterations launched since dfk_run_init
+  hipStream_t side = nullptr;    // second stream: the control kernel beside the gravity kernels
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
   int thr_rows = 256, thr_cols = 256, thr_upd = 256;   // threads per workgroup
   int upd_kind = 0;              // 0: LDS-tile update kernel, 1: register-streaming
 };
@@ -182,8 +186,8 @@ int build_layout(dfk_handle *h) {
     if (D.selfgrav) {
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
       for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) mc.push_back(make_int2(d, g));
-      lr = std::max(lr, (size_t)2 * D.pr * D.Nphi * sizeof(double2));
-      lc = std::max(lc, (size_t)2 * D.cb * D.M * sizeof(double2));
+      lr = std::max(lr, (size_t)(2 * D.pr + 1) * D.Nphi * sizeof(double2));   // + twiddles
+      lc = std::max(lc, (size_t)(2 * D.cb + 1) * D.M * sizeof(double2));
     }
   }
   h->upd0[h->nd] = (int)mu.size();
@@ -242,13 +246,11 @@ int write_ctl(dfk_handle *h, int d, const DiskCtl *c) {
   return DFK_OK;
 }
 
-// Control block for a unit call: not done, src/dst as given, dt as given.
-int unit_ctl(dfk_handle *h, int d, int src, int dst, double dt) {
+// Control block for a unit call: not done, dt as given.
+int unit_ctl(dfk_handle *h, int d, double dt) {
   DiskCtl c;
   memset(&c, 0, sizeof c);
   c.cur = h->hcur[d];
-  c.src = src;
-  c.dst = dst;
   c.dt = dt;
   c.done = 0;
   h->run_ready = false;
@@ -259,11 +261,12 @@ double *field(dfk_handle *h, int buf, int f, int d) {
   return h->st + (i64)(buf * 3 + f) * h->ntot + h->hD[d].so;
 }
 
-// Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk.
-int launch_gravity_disk(dfk_handle *h, int d) {
+// Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk,
+// Sigma read from state buffer `par`.
+int launch_gravity_disk(dfk_handle *h, int d, int par) {
   const int nr = h->r2c0[d + 1] - h->r2c0[d], nc = h->conv0[d + 1] - h->conv0[d];
   HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->st,
-                        h->ntot, h->geom, h->tw, h->spec));
+                        h->ntot, h->geom, h->tw, h->spec, par));
   HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[d], h->ctl,
                          h->khat, h->tw, h->spec));
   HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
@@ -271,39 +274,49 @@ int launch_gravity_disk(dfk_handle *h, int d) {
   return DFK_OK;
 }
 
-hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR, int has_gp) {
+hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR, int has_gp, int par) {
   if (h->upd_kind == 1)
     return dfk_launch_update_stream(h->stream, nblk, garr, h->dD, h->map_upd, base, h->ctl, h->st,
                                     h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
-                                    h->trig, h->pcfl, h->pcm);
+                                    h->trig, h->pcfl, h->pcm, par);
   return dfk_launch_update(h->stream, nblk, h->thr_upd, h->lds_upd, garr, h->dD, h->map_upd, base,
                            h->ctl, h->st, h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
-                           h->trig, h->pcfl, h->pcm);
+                           h->trig, h->pcfl, h->pcm, par);
 }
 
-// One whole loop iteration for every disk (captured into the graph).
-int launch_step_all(dfk_handle *h) {
+// One whole loop iteration for every disk (captured into the graph).  Every
+// running disk's state is in buffer `par`; the update writes buffer 1 - par.
+// The control kernel runs on the side stream, beside the gravity kernels.
+int launch_step_all(dfk_handle *h, int par) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
-  HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap));
   if (nr > 0) {
+    HIPCHK(hipEventRecord(h->evFork, h->stream));
+    HIPCHK(hipStreamWaitEvent(h->side, h->evFork, 0));
+    HIPCHK(dfk_launch_control(h->side, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
+    HIPCHK(hipEventRecord(h->evJoin, h->side));
     HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
-                          h->geom, h->tw, h->spec));
+                          h->geom, h->tw, h->spec, par));
     HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
                            h->tw, h->spec));
     HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
                           h->geom, h->tw, h->phi));
+    HIPCHK(hipStreamWaitEvent(h->stream, h->evJoin, 0));
+  } else {
+    HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
   }
-  HIPCHK(launch_update(h, nu, 0, 0, 0, 0));
+  HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par));
   return DFK_OK;
 }
 
-int get_graph(dfk_handle *h, int nsteps, hipGraphExec_t *out) {
-  auto it = h->graphs.find(nsteps);
+// Graph of `nsteps` iterations starting with state buffer par0 (cached).
+int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
+  const int key = 2 * nsteps + par0;
+  auto it = h->graphs.find(key);
   if (it != h->graphs.end()) { *out = it->second; return DFK_OK; }
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = DFK_OK;
-  for (int s = 0; s < nsteps && rc == DFK_OK; ++s) rc = launch_step_all(h);
+  for (int s = 0; s < nsteps && rc == DFK_OK; ++s) rc = launch_step_all(h, (par0 + s) & 1);
   hipError_t e = hipStreamEndCapture(h->stream, &g);
   if (rc != DFK_OK) return rc;
   if (e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); return DFK_EHIP; }
@@ -311,7 +324,7 @@ int get_graph(dfk_handle *h, int nsteps, hipGraphExec_t *out) {
   e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
   if (e != hipSuccess) { set_err("graph instantiate failed: %s", hipGetErrorString(e)); return DFK_EHIP; }
-  h->graphs[nsteps] = ex;
+  h->graphs[key] = ex;
   *out = ex;
   return DFK_OK;
 }
@@ -352,7 +365,7 @@ int drain(dfk_handle *h) {
 // Per-tile partials of disk d's current state, reduced on the host in tile order.
 int diag_disk(dfk_handle *h, int d, double *mx, double sums[4]) {
   TRY(build_layout(h));
-  TRY(unit_ctl(h, d, h->hcur[d], 1 - h->hcur[d], 0.0));
+  TRY(unit_ctl(h, d, 0.0));
   const int nb = h->upd0[d + 1] - h->upd0[d];
   HIPCHK(dfk_launch_diag(h->stream, nb, h->dD, h->map_upd, h->upd0[d], h->ctl, h->st, h->ntot,
                          h->geom, h->trig, h->pcfl, h->pcm));
@@ -456,8 +469,8 @@ int build_khat(dfk_handle *h, int d, std::vector<double2> *full) {
   if (e == hipSuccess) e = hipMemcpy(dd, &D, sizeof D, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = dfk_launch_khat(h->stream, dd, ceil_div(D.M, 2 * D.pr), ceil_div(D.Nc, D.cb),
-                        (size_t)2 * D.pr * D.Nphi * sizeof(double2),
-                        (size_t)2 * D.cb * D.M * sizeof(double2), dk, h->tw, tmp, h->khat);
+                        (size_t)(2 * D.pr + 1) * D.Nphi * sizeof(double2),
+                        (size_t)(2 * D.cb + 1) * D.M * sizeof(double2), dk, h->tw, tmp, h->khat);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess && full) {
     full->resize((size_t)D.Nc * D.M);
@@ -493,9 +506,9 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
     const dfk_disk_params &p = params[d];
     if (p.Nu < 3 || p.Nphi < 4) { set_err("disk %d: need Nu >= 3 and Nphi >= 4 (got %d x %d)", d, p.Nu, p.Nphi); return DFK_EINVAL; }
     if (!(p.Rin > 0) || !(p.Rout > p.Rin)) { set_err("disk %d: need 0 < Rin < Rout", d); return DFK_EINVAL; }
-    if (p.Nu > 1024 || p.Nphi > 2048) {
-      // one row (Nphi) or one padded column (2 Nu) must fit a 64 KB LDS transform
-      set_err("disk %d: need Nu <= 1024 and Nphi <= 2048 (got %d x %d)", d, p.Nu, p.Nphi);
+    if (p.Nu > 512 || p.Nphi > 1024) {
+      // one row (Nphi) or one padded column (2 Nu), ping-pong + twiddles, must fit 64 KB of LDS
+      set_err("disk %d: need Nu <= 512 and Nphi <= 1024 (got %d x %d)", d, p.Nu, p.Nphi);
       return DFK_EINVAL;
     }
   }
@@ -517,7 +530,13 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
   h->hsamples.resize(ndisks);
   auto fail = [&](int rc) { dfk_destroy(h); return rc; };
   if (hipSetDevice(device) != hipSuccess) { set_err("hipSetDevice failed"); return fail(DFK_EHIP); }
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { set_err("stream create failed"); return fail(DFK_EHIP); }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->evFork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->evJoin, hipEventDisableTiming) != hipSuccess) {
+    set_err("stream/event create failed");
+    return fail(DFK_EHIP);
+  }
   hipEventCreate(&h->ev0);
   hipEventCreate(&h->ev1);
 
@@ -621,6 +640,9 @@ void dfk_destroy(dfk_handle *h) {
   hipFree(h->ctl); hipFree(h->samples);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->evFork) hipEventDestroy(h->evFork);
+  if (h->evJoin) hipEventDestroy(h->evJoin);
+  if (h->side) hipStreamDestroy(h->side);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -717,8 +739,8 @@ int dfk_potential(dfk_handle *h, int d, const double *Sigma, double *Phi) {
   const int scratch = 1 - h->hcur[d];
   const size_t n = (size_t)h->hD[d].Nu * h->hD[d].Nphi * sizeof(double);
   HIPCHK(hipMemcpy(field(h, scratch, 0, d), Sigma, n, hipMemcpyHostToDevice));
-  TRY(unit_ctl(h, d, scratch, h->hcur[d], 0.0));
-  TRY(launch_gravity_disk(h, d));
+  TRY(unit_ctl(h, d, 0.0));
+  TRY(launch_gravity_disk(h, d, scratch));
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(Phi, h->phi + h->hD[d].so, n, hipMemcpyDeviceToHost));
   return DFK_OK;
@@ -746,11 +768,12 @@ int dfk_step(dfk_handle *h, int d, double dt, const double *gR, const double *gp
   if (gR) HIPCHK(hipMemcpy(h->gR + D.so, gR, n, hipMemcpyHostToDevice));
   if (gp) HIPCHK(hipMemcpy(h->gp + D.so, gp, n, hipMemcpyHostToDevice));
   const int c = h->hcur[d];
-  TRY(unit_ctl(h, d, c, 1 - c, dt));
+  TRY(unit_ctl(h, d, dt));
   const int nb = h->upd0[d + 1] - h->upd0[d];
-  HIPCHK(launch_update(h, nb, h->upd0[d], 1, gR != nullptr, gp != nullptr));
+  HIPCHK(launch_update(h, nb, h->upd0[d], 1, gR != nullptr, gp != nullptr, c));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->hcur[d] = 1 - c;
+  TRY(unit_ctl(h, d, dt));   // records the new current buffer
   return DFK_OK;
 }
 
@@ -809,17 +832,25 @@ int dfk_run_init(dfk_handle *h, const dfk_run_params *rp, int sample_cap) {
     c[d].cfl = rp[d].cfl;
     c[d].max_steps = rp[d].max_steps;
     c[d].stop_on_nonfinite = rp[d].stop_on_nonfinite;
-    c[d].cur = h->hcur[d];
-    c[d].src = c[d].cur;
-    c[d].dst = 1 - c[d].cur;
+    // every disk starts the run with its state in buffer 0, so that iteration
+    // g of the run reads buffer g & 1 for all running disks
+    if (h->hcur[d] == 1) {
+      const size_t nb = (size_t)h->hD[d].Nu * h->hD[d].Nphi * sizeof(double);
+      for (int f = 0; f < 3; ++f)
+        HIPCHK(hipMemcpyAsync(field(h, 0, f, d), field(h, 1, f, d), nb, hipMemcpyDeviceToDevice, h->stream));
+      h->hcur[d] = 0;
+    }
+    c[d].cur = 0;
     h->mD[d] = rp[d].mD;
     h->drained[d] = 0;
     h->hsamples[d].clear();
   }
+  HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(h->ctl, c.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
   HIPCHK(dfk_launch_diag(h->stream, h->upd0[h->nd], h->dD, h->map_upd, 0, h->ctl, h->st, h->ntot,
                          h->geom, h->trig, h->pcfl, h->pcm));
   HIPCHK(hipStreamSynchronize(h->stream));
+  h->iter = 0;
   h->run_ready = true;
   return DFK_OK;
 }
@@ -835,7 +866,7 @@ int dfk_advance(dfk_handle *h, int nsteps, int *all_done) {
   while (left > 0) {
     const int chunk = std::min(left, chunk_max);
     hipGraphExec_t ex;
-    TRY(get_graph(h, chunk, &ex));
+    TRY(get_graph(h, chunk, (int)(h->iter & 1), &ex));
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     HIPCHK(hipGraphLaunch(ex, h->stream));
     HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -843,6 +874,7 @@ int dfk_advance(dfk_handle *h, int nsteps, int *all_done) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     total_ms += ms;
+    h->iter += chunk;
     TRY(drain(h));
     left -= chunk;
   }
@@ -946,8 +978,9 @@ int dfk_snapshot_load(dfk_handle *h, int d, const char *path) {
   HIPCHK(hipSetDevice(h->device));
   DiskCtl c;
   TRY(read_ctl(h, d, &c));
-  h->hcur[d] = c.cur;
-  const int cur = c.cur;
+  const int cur = (int)(h->iter & 1);   // where running disks hold their state now
+  h->hcur[d] = cur;
+  c.cur = cur;
   const size_t nb = N * sizeof(double);
   HIPCHK(hipMemcpy(field(h, cur, 0, d), buf.data(), nb, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(field(h, cur, 1, d), buf.data() + N, nb, hipMemcpyHostToDevice));
@@ -1048,12 +1081,11 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
   HIPCHK(hipSetDevice(h->device));
   std::vector<DiskCtl> saved(h->nd);
   HIPCHK(hipMemcpy(saved.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
-  // a control block that lets every kernel run: not done, src = cur
+  // running disks hold their state in buffer par and keep running; finished
+  // disks stay skipped (their state may sit in the buffer the update writes)
+  const int par = (int)(h->iter & 1);
   std::vector<DiskCtl> c = saved;
   for (int d = 0; d < h->nd; ++d) {
-    c[d].done = 0;
-    c[d].src = c[d].cur;
-    c[d].dst = 1 - c[d].cur;
     if (!(c[d].dt > 0)) c[d].dt = 1e-6;
     c[d].tlim = 1e300;
     c[d].max_steps = 0;
@@ -1066,11 +1098,11 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
     for (int r = 0; r < nrep; ++r) {
       switch (k) {
         case 0:
-          HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap));
+          HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
           break;
         case 1:
           HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
-                                h->geom, h->tw, h->spec));
+                                h->geom, h->tw, h->spec, par));
           break;
         case 2:
           HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
@@ -1081,7 +1113,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
                                 h->geom, h->tw, h->phi));
           break;
         case 4:
-          HIPCHK(launch_update(h, nu, 0, 0, 0, 0));
+          HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par));
           break;
       }
     }

@@ -131,6 +131,15 @@ class Surrogate:
         self.net = MLP(members, seed, device)
         self.mu = np.zeros(3)
         self.sd = np.ones(3)
+        self.sc = np.ones(3)
+
+    # targets are heavy tailed (fast inner-edge modes): the network learns
+    # asinh(y / s) with s the median |y| of the training set, standardised
+    def _fwd(self, Y):
+        return np.arcsinh(np.asarray(Y, dtype=np.float64) / self.sc)
+
+    def _inv(self, Z):
+        return self.sc * np.sinh(Z)
 
     @staticmethod
     def norm_x(X):
@@ -138,13 +147,15 @@ class Surrogate:
 
     def fit(self, X, Y, epochs=400, batch=256, lr=2e-3, seed=7):
         Y = np.asarray(Y, dtype=np.float64)
-        self.mu = Y.mean(0)
-        self.sd = Y.std(0) + 1e-12
-        return self.net.train(self.norm_x(X), (Y - self.mu) / self.sd, epochs, batch, lr, seed)
+        self.sc = np.median(np.abs(Y), axis=0) + 1e-12
+        Z = self._fwd(Y)
+        self.mu = Z.mean(0)
+        self.sd = Z.std(0) + 1e-12
+        return self.net.train(self.norm_x(X), (Z - self.mu) / self.sd, epochs, batch, lr, seed)
 
     def predict_members(self, X):
-        y = self.net.forward(self.norm_x(X)).astype(np.float64)
-        return y * self.sd + self.mu
+        z = self.net.forward(self.norm_x(X)).astype(np.float64)
+        return self._inv(z * self.sd + self.mu)
 
     def predict(self, X, spread_tol=0.05):
         """Mean prediction, relative member spread, and the mask of points

@@ -31,7 +31,7 @@ extern "C" {
 /* Constructor arguments of DiskHydro (hydro.hpp:28-29) and KalnajsSolver
  * (kalnajs.hpp:25-26) for one disk. */
 typedef struct {
-  int Nu, Nphi;          /* radial (log-spaced) and azimuthal zones; Nu >= 3, Nphi >= 4 */
+  int Nu, Nphi;          /* radial (log-spaced) and azimuthal zones; 3 <= Nu <= 512, 4 <= Nphi <= 1024 */
   double Rin, Rout;      /* grid radii */
   double K, gamma;       /* barotropic EOS P = K Sigma^gamma */
   double gmstar;         /* G M_star */
