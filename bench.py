@@ -58,7 +58,9 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk):
     m = dfk.model_defaults(dfk.MODEL_POWERLAW)
     p, S, U, W, mD, _ = dfk.model_ic(m, Nu, Nphi, eps)
     e = dfk.Ensemble(p)
-    tuned = autotune.apply(e, regime="latency")
+    # the latency-regime shapes where tuned for this grid, else the
+    # throughput ones (a large single disk fills the GPU by itself)
+    tuned = autotune.apply(e, regime="latency") or autotune.apply(e, regime="throughput")
     e.set_state(S, U, W)
     e.set_graph_steps(chunk)
     e.run_init(1e30, dtout=0.25, cfl=cfl, mD=mD, sample_cap=chunk + 8)
@@ -110,9 +112,18 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
     ens.advance(steps_batch)
     wall_b = time.perf_counter() - t0
     batched = zones * steps_batch / wall_b
+    names = ["control", "rows_r2c", "cols_conv", "rows_c2r", "update"]
+    kms = dict(zip(names, [float(x) for x in ens.profile_step(50)]))
+    kb = {k: sum(kernel_bytes(d["Nu"], d["Nphi"])[k] for d in disks) for k in names}
+    kern = {k: dict(ms=kms[k], alg_bytes=kb[k],
+                    achieved_GBs=kb[k] / (kms[k] * 1e-3) / 1e9 if kms[k] > 0 else 0.0)
+            for k in names}
     ens.close()
     # the same disks stepped one after another, each a plain single-disk run
     seq_time, seq_zones = 0.0, 0
+    if steps_seq <= 0:
+        return {"disks": 64, "zones": zones, "steps": steps_batch,
+                "batched_zone_updates_per_s": batched, "kernels": kern}
     for d, ic in zip(disks, ics):
         e = dfk.Ensemble(ic[4])
         autotune.apply(e, regime="latency")
@@ -131,6 +142,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
             "batched_zone_updates_per_s": batched,
             "sequential_zone_updates_per_s": seq,
             "speedup_vs_sequential": batched / seq, "tuned_disks": ntuned,
+            "kernels": kern,
             "hbm_frac_alg": batched * balg / HBM_PEAK}
 
 
@@ -148,6 +160,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the batched 64-disk leg (no single disk, no sequential leg)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -157,6 +171,9 @@ def main():
     if dfk.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_legs(dfk, args.ensemble_steps, 0)}), flush=True)
+        return
     Nu, Nphi = args.nu, args.nphi
     r = single_disk(dfk, Nu, Nphi, args.eps, args.cfl, args.steps, args.warmup, args.chunk)
     zps = Nu * Nphi * args.steps / r["wall"]

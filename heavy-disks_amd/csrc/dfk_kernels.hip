@@ -1,20 +1,25 @@
 // HIP kernels (gfx950, FP64) for the diskfft time-step loop.
 //
 // Per step, for every disk of the handle (a single disk is an ensemble of one):
-//   k_control   dt = cfl / max(...) from the previous step's per-tile maxima,
+//   k_rows_r2c  S = R^{3/2} Sigma, row transforms of length Nphi (two real rows
+//               packed per complex transform), half spectrum -> spec[Nc][Nu];
+//               its trailing workgroups (one per disk) run the loop control:
+//               dt = cfl / max(...) from the previous step's per-tile maxima,
 //               output-time check and C_m sample, last-step clip, t += dt
 //               (diskmain.cpp:92-110, hydro.hpp:48-61)
-//   k_rows_r2c  S = R^{3/2} Sigma, row transforms of length Nphi (two real rows
-//               packed per complex transform), half spectrum -> spec[Nc][Nu]
 //   k_cols_conv per column: zero-padded length-2Nu transform, x real K-hat,
 //               inverse, keep rows < Nu                    (kalnajs.hpp:48-57)
 //   k_rows_c2r  row c2r + scale pref/sqrt(R) -> Phi       (kalnajs.hpp:57-62)
 //   k_update    forces (kalnajs.hpp:65-80) + source (hydro.hpp:73-96) + radial
 //               MUSCL sweep (:99-140) + azimuthal sweep (:143-179) + floor
-//               (:68-69) fused on an LDS tile; epilogue writes the next step's
-//               CFL maximum and the C_0/C_2 partial sums per tile.
-// All FFTs are this file's own LDS Stockham transforms (radix 4, 2, 3 unrolled;
-// 5, 7, 11, 13 by a direct small DFT).
+//               (:68-69) fused on an LDS tile (k_update_stream: the same
+//               fused on a wavefront streaming down a 60-column strip);
+//               epilogue writes the new state's CFL maximum and C_0/C_2
+//               partial sums per tile.
+// k_control is the loop control as a launch of its own (handles without
+// self-gravity, profiling).
+// All FFTs are this file's own LDS Stockham transforms (radix 8, 4, 2, 3
+// unrolled; 5, 7, 11, 13 by a direct small DFT).
 #include <hip/hip_runtime.h>
 
 #include "dfk_internal.h"
@@ -40,10 +45,16 @@ __device__ __forceinline__ double vanleer(double a, double b) {
   return (a * b > 0.0) ? (2.0 * a * b) * frcp(a + b) : 0.0;
 }
 
+// Barotropic EOS (hydro.hpp:44-45).  G2: every disk of the launch has
+// gamma = 2 (no pow; the general path costs the update kernels ~45 VGPRs).
+template <bool G2 = false>
 __device__ __forceinline__ double pres(const DiskDev &D, double s) {
+  if constexpr (G2) return D.K * (s * s);
   return D.gamma2 ? D.K * (s * s) : D.K * pow(s, D.gamma);
 }
+template <bool G2 = false>
 __device__ __forceinline__ double cs2f(const DiskDev &D, double s) {
+  if constexpr (G2) return (2.0 * D.K) * s;
   return D.gamma2 ? (2.0 * D.K) * s : D.gamma * D.K * pow(s, D.gamma - 1.0);
 }
 
@@ -262,9 +273,10 @@ __device__ void wg_reduce5(double &mx, double &s0, double &s1, double &s2, doubl
 }
 
 // CFL rate of one zone (hydro.hpp:52-57); NaN rates never win the max.
+template <bool G2 = false>
 __device__ __forceinline__ double cfl_rate(const DiskDev &D, double s, double u, double w,
                                            double invdR, double invdl) {
-  double cs = sqrt(cs2f(D, smax(s, 1e-12)));
+  double cs = sqrt(cs2f<G2>(D, smax(s, 1e-12)));
   return (fabs(u) + cs) * invdR + (fabs(w) + cs) * invdl;
 }
 
@@ -281,21 +293,16 @@ __device__ __forceinline__ double *stpw(double *st, i64 ntot, int buf, int f, co
 }
 
 // ------------------------------------------------------------------ control
-// One workgroup per disk.  Reduces the per-tile partials of the state that
-// the previous update (or the initial diag) produced, then performs the top
-// of one iteration of diskmain.cpp:92-101.  It touches only the control block
-// and the sample ring, so it runs concurrently with the gravity kernels of
-// the same iteration (they read `done` only, and a stale `done` costs at most
-// one wasted transform).  `par` is the iteration's state buffer.
-__global__ void __launch_bounds__(WG)
-k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
-          const double *__restrict__ pcfl, const double *__restrict__ pcm,
-          DfkSampleDev *__restrict__ samples, int cap, int par) {
-  __shared__ double red[5 * (WG / 64)];
-  const int d = blockIdx.x;
-  const DiskDev &D = disks[d];
-  DiskCtl &C = ctl[d];
-  if (C.done) return;
+// Top of one iteration of diskmain.cpp:92-101 for one disk, run by one whole
+// workgroup: reduce the per-tile partials (CFL maximum, C_m sums) that the
+// previous update (or the initial diag) wrote for the state in buffer `par`;
+// stop at t >= tlim or at the step cap; record an output sample once t has
+// reached tnext; dt = cfl / max, clipped to land on tlim; t += dt.  It
+// touches only the control block and the sample ring.
+__device__ void control_disk(const DiskDev &D, DiskCtl &C, int d, const double *__restrict__ pcfl,
+                             const double *__restrict__ pcm, DfkSampleDev *__restrict__ samples,
+                             int cap, int par) {
+  __shared__ double red[5 * 16];
   const int nt = D.ntile_i * D.ntile_j;
   double mx = 1e-30, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (int t = threadIdx.x; t < nt; t += blockDim.x) {
@@ -308,7 +315,6 @@ k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
   }
   wg_reduce5(mx, s0, s1, s2, s3, red);
   if (threadIdx.x != 0) return;
-  // the current state of every disk still running is in buffer `par`
   if (s3 > 0.0) {
     C.flags |= DFK_FLAG_NONFINITE;
     if (C.stop_on_nonfinite) {
@@ -340,6 +346,18 @@ k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
   C.t = t + dt;
   C.step += 1;
   C.cur = 1 - par;   // where this iteration's update writes
+}
+
+// Control of every disk as a launch of its own (one workgroup per disk).  In
+// the time loop the same work rides in the row-transform launch instead
+// (trailing workgroups of k_rows_r2c, which needs nothing of it).
+__global__ void __launch_bounds__(WG)
+k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
+          const double *__restrict__ pcfl, const double *__restrict__ pcm,
+          DfkSampleDev *__restrict__ samples, int cap, int par) {
+  const int d = blockIdx.x;
+  if (ctl[d].done) return;
+  control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
 }
 
 // ------------------------------------------------------------------ diag
@@ -389,10 +407,19 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
 // Row r2c: S = R^{3/2} Sigma for rows [2 pr g, 2 pr (g+1)) of the current state.
 __global__ void __launch_bounds__(WG)
 k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
-           const DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,
+           DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,
            const double *__restrict__ geom, const double2 *__restrict__ tw,
-           double2 *__restrict__ spec, int par) {
+           double2 *__restrict__ spec, int par, int nrow_blk, const double *__restrict__ pcfl,
+           const double *__restrict__ pcm, DfkSampleDev *__restrict__ samples, int cap) {
   extern __shared__ double2 lds2[];
+  if ((int)blockIdx.x >= nrow_blk) {
+    // trailing workgroups: the loop control of disk blockIdx.x - nrow_blk.
+    // Row blocks of a disk that it stops read `done` racily; they then only
+    // transform a spectrum that no later kernel of the disk reads.
+    const int d = blockIdx.x - nrow_blk;
+    if (!ctl[d].done) control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
+    return;
+  }
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
   const DiskCtl &C = ctl[wm.x];
@@ -580,7 +607,7 @@ k_khat_cols(const DiskDev *__restrict__ D_, const double2 *__restrict__ tw,
 // state on (ti+4) x (tj+4) zones, the radial face fluxes, then the post-radial
 // state and the azimuthal face fluxes.  GARR: gravity from given g_R/g_phi
 // arrays (unit entry point) instead of the potential.
-template <bool GARR>
+template <bool GARR, bool G2>
 __global__ void __launch_bounds__(WG)
 k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
          const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
@@ -622,8 +649,8 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
     const int im = (i > 0) ? i - 1 : 0, ip = (i < Nu - 1) ? i + 1 : Nu - 1;
     const i64 c = (i64)i * Nphi + j;
     const double s = S[c], u = U[c], w = W[c];
-    const double dPdR = (pres(D, S[(i64)ip * Nphi + j]) - pres(D, S[(i64)im * Nphi + j])) * tInvdRc[i];
-    const double dPdphi = (pres(D, S[(i64)i * Nphi + jp]) - pres(D, S[(i64)i * Nphi + jm])) * D.inv2dphi;
+    const double dPdR = (pres<G2>(D, S[(i64)ip * Nphi + j]) - pres<G2>(D, S[(i64)im * Nphi + j])) * tInvdRc[i];
+    const double dPdphi = (pres<G2>(D, S[(i64)i * Nphi + jp]) - pres<G2>(D, S[(i64)i * Nphi + jm])) * D.inv2dphi;
     double gR = tGstar[i];
     double gp = 0.0;
     if (GARR) {
@@ -635,7 +662,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
       gR += -(Ph[(i64)iu * Nphi + j] - Ph[(i64)id * Nphi + j]) * tGrc[i];
       gp += -(Ph[(i64)i * Nphi + jp] - Ph[(i64)i * Nphi + jm]) * tGpc[i];
     }
-    const double rs = 1.0 / s;
+    const double rs = frcp(s);
     const double invR = tInvR[i];
     const double aR = -dPdR * rs + gR + w * w * invR;
     const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
@@ -692,7 +719,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
     const double ns = s + (fM[e] - fM[e + SW]) * iv;
     const double nmR = mR + (fU[e] - fU[e + SW]) * iv;
     const double nmP = mP + (fW[e] - fW[e + SW]) * iv;
-    const double rn = 1.0 / smax(ns, 1e-10);
+    const double rn = frcp(smax(ns, 1e-10));
     sS[l] = ns; sU[l] = nmR * rn; sW[l] = nmP * rn;
   }
   __syncthreads();
@@ -746,12 +773,12 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
     const double ns = s + (fM[f] - fM[f + 1]) * iv;
     const double nmR = mR + (fU[f] - fU[f + 1]) * iv;
     const double nmP = mP + (fW[f] - fW[f + 1]) * iv;
-    const double rn = 1.0 / smax(ns, 1e-10);
+    const double rn = frcp(smax(ns, 1e-10));
     const double so = smax(ns, 1e-10);
     const double uo = nmR * rn, wo = nmP * rn;
     const i64 c = (i64)i * Nphi + j;
     So[c] = so; Uo[c] = uo; Wo[c] = wo;
-    const double r = cfl_rate(D, so, uo, wo, tInvdR[i], tInvdl[i]);
+    const double r = cfl_rate<G2>(D, so, uo, wo, tInvdR[i], tInvdl[i]);
     mx = (r > mx) ? r : mx;
     a0 += so * dA[i];
     a1 += so * c2[j] * dA[i];
@@ -777,10 +804,21 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
 // Same arithmetic as k_update.
 #define STREAM_OUT 60
 
-__device__ __forceinline__ double shup(double v, int d) { return __shfl_up(v, d, 64); }
-__device__ __forceinline__ double shdn(double v, int d) { return __shfl_down(v, d, 64); }
+// Whole-wavefront lane shifts by one through DPP (wave_shr:1 / wave_shl:1 on
+// each 32-bit half): lane L receives lane L-1 (up) or L+1 (down); the lane
+// shifted in at the edge receives 0.  All 64 lanes must be active.
+__device__ __forceinline__ double up1(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, true),
+                          __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double dn1(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, true),
+                          __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, true));
+}
 
-template <bool GARR>
+template <bool GARR, bool G2>
 __global__ void __launch_bounds__(64)
 k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
                 const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
@@ -832,26 +870,41 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   double pfm = 0, pfu = 0, pfw = 0;     // previous radial face (f-1)
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 
+  // loads of the next row are issued one iteration ahead (rows clamped into
+  // the disk; values of rows outside it are never used)
+  double nU, nW, nSe, nSw, nPe = 0.0, nPw = 0.0, nSp, nPp = 0.0;
+  auto fetch = [&](int r) {
+    const i64 ro = (i64)clampr(r) * Nphi, rp = (i64)clampr(r + 1) * Nphi;
+    nU = U[ro + c];
+    nW = W[ro + c];
+    nSe = S[ro + cp];
+    nSw = S[ro + cm];
+    nSp = S[rp + c];
+    if (sg) {
+      nPe = Ph[ro + cp];
+      nPw = Ph[ro + cm];
+      nPp = Ph[rp + c];
+    }
+  };
+  fetch(sbeg);
   for (int s = sbeg; s <= send; ++s) {
-    const double Sp = S[(i64)clampr(s + 1) * Nphi + c];
-    double Pp = 0.0;
-    if (sg) Pp = Ph[(i64)clampr(s + 1) * Nphi + c];
+    const double Sp = nSp, Pp = nPp, u = nU, w = nW, Se = nSe, Sw = nSw, Pe = nPe, Pw = nPw;
+    if (s < send) fetch(s + 1);
     // ---- source at row s (hydro.hpp:73-96, forces kalnajs.hpp:65-80)
     double ns_s = 0.0, ns_u = 0.0, ns_w = 0.0;
     if (s >= 0 && s < Nu) {
       const i64 ro = (i64)s * Nphi;
-      const double u = U[ro + c], w = W[ro + c];
-      const double dPdR = (pres(D, Sp) - pres(D, Sm)) * tInvdRc[s];
-      const double dPdphi = (pres(D, S[ro + cp]) - pres(D, S[ro + cm])) * D.inv2dphi;
+      const double dPdR = (pres<G2>(D, Sp) - pres<G2>(D, Sm)) * tInvdRc[s];
+      const double dPdphi = (pres<G2>(D, Se) - pres<G2>(D, Sw)) * D.inv2dphi;
       double gR = tGstar[s], gp = 0.0;
       if (GARR) {
         if (has_gR) gR += gRa[D.so + ro + c];
         if (has_gp) gp += gpa[D.so + ro + c];
       } else if (sg) {
         gR += -(Pp - Pm) * tGrc[s];
-        gp += -(Ph[ro + cp] - Ph[ro + cm]) * tGpc[s];
+        gp += -(Pe - Pw) * tGpc[s];
       }
-      const double rs = 1.0 / S0;
+      const double rs = frcp(S0);
       const double invR = tInvR[s];
       const double aR = -dPdR * rs + gR + w * w * invR;
       const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
@@ -895,17 +948,17 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
         const double ns = s_ + (pfm - fm) * iv;
         const double nmR = s_ * qu1 + (pfu - fu) * iv;
         const double nmP = s_ * qw1 + (pfw - fw) * iv;
-        const double rn = 1.0 / smax(ns, 1e-10);
+        const double rn = frcp(smax(ns, 1e-10));
         const double Xs = ns, Xu = nmR * rn, Xw = nmP * rn;
         // azimuthal face at this lane's left edge (hydro.hpp:153-168)
-        const double sm1 = shup(Xs, 1), um1 = shup(Xu, 1), wm1 = shup(Xw, 1);
-        const double sm2 = shup(Xs, 2), um2 = shup(Xu, 2), wm2 = shup(Xw, 2);
-        const double sp1 = shdn(Xs, 1), up1 = shdn(Xu, 1), wp1 = shdn(Xw, 1);
+        const double sm1 = up1(Xs), um1 = up1(Xu), wm1 = up1(Xw);
+        const double sm2 = up1(sm1), um2 = up1(um1), wm2 = up1(wm1);
+        const double sp1 = dn1(Xs), vp1 = dn1(Xu), wp1 = dn1(Xw);
         const double vf = 0.5 * (wm1 + Xw);
         const bool upw = vf > 0;
         const double bs = upw ? sm1 : Xs, bu = upw ? um1 : Xu, bw = upw ? wm1 : Xw;
         const double ls = upw ? sm2 : sm1, lu = upw ? um2 : um1, lw = upw ? wm2 : wm1;
-        const double hs = upw ? Xs : sp1, hu = upw ? Xu : up1, hw = upw ? Xw : wp1;
+        const double hs = upw ? Xs : sp1, hu = upw ? Xu : vp1, hw = upw ? Xw : wp1;
         const double fac = upw ? 0.5 * (1.0 - vf * dt * tInvdl[g])
                                : -0.5 * (1.0 + vf * dt * tInvdl[g]);
         const double ss = bs + vanleer(hs - bs, bs - ls) * fac;
@@ -913,19 +966,19 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
         const double sw = bw + vanleer(hw - bw, bw - lw) * fac;
         const double Fm = vf * ss * tAface[g];
         const double FU = Fm * su, FW = Fm * sw;
-        const double Fm1 = shdn(Fm, 1), FU1 = shdn(FU, 1), FW1 = shdn(FW, 1);
+        const double Fm1 = dn1(Fm), FU1 = dn1(FU), FW1 = dn1(FW);
         // azimuthal gather + floor (hydro.hpp:170-178, 68-69)
         const double ns2 = Xs + (Fm - Fm1) * iv;
         const double nmR2 = Xs * Xu + (FU - FU1) * iv;
         const double nmP2 = Xs * Xw + (FW - FW1) * iv;
-        const double rn2 = 1.0 / smax(ns2, 1e-10);
+        const double rn2 = frcp(smax(ns2, 1e-10));
         const double so = smax(ns2, 1e-10), uo = nmR2 * rn2, wo = nmP2 * rn2;
         if (out_ok) {
           const i64 o = (i64)g * Nphi + c;
           So[o] = so;
           Uo[o] = uo;
           Wo[o] = wo;
-          const double r = cfl_rate(D, so, uo, wo, tInvdR[g], tInvdl[g]);
+          const double r = cfl_rate<G2>(D, so, uo, wo, tInvdR[g], tInvdl[g]);
           mx = (r > mx) ? r : mx;
           a0 += so * tdA[g];
           a1 += so * c2j * tdA[g];
@@ -1106,11 +1159,13 @@ hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const 
 }
 
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                          const int2 *map, int base, const DiskCtl *ctl, const double *st, i64 ntot,
-                          const double *geom, const double2 *tw, double2 *spec, int par) {
-  if (nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rows_r2c, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot, geom,
-                     tw, spec, par);
+                          const int2 *map, int base, DiskCtl *ctl, const double *st, i64 ntot,
+                          const double *geom, const double2 *tw, double2 *spec, int par,
+                          int nctl, const double *pcfl, const double *pcm, DfkSampleDev *samples,
+                          int cap) {
+  if (nblk + nctl <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_r2c, dim3(nblk + nctl), dim3(nthr), lds, s, disks, map, base, ctl, st,
+                     ntot, geom, tw, spec, par, nblk, pcfl, pcm, samples, cap);
   return hipGetLastError();
 }
 
@@ -1131,34 +1186,30 @@ hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const D
   return hipGetLastError();
 }
 
-hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int garr,
+hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int garr, int g2,
                              const DiskDev *disks,
                              const int2 *map, int base, const DiskCtl *ctl, double *st, i64 ntot,
                              const double *phi, const double *gR, const double *gp, int has_gR,
                              int has_gp, const double *geom, const double *trig, double *pcfl,
                              double *pcm, int par) {
   if (nblk <= 0) return hipSuccess;
-  if (garr)
-    hipLaunchKernelGGL(k_update<true>, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot,
-                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
-  else
-    hipLaunchKernelGGL(k_update<false>, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot,
-                       phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
+  auto k = garr ? (g2 ? k_update<true, true> : k_update<true, false>)
+                : (g2 ? k_update<false, true> : k_update<false, false>);
+  hipLaunchKernelGGL(k, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot, phi, gR, gp,
+                     has_gR, has_gp, geom, trig, pcfl, pcm, par);
   return hipGetLastError();
 }
 
-hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, const DiskDev *disks,
+hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, int g2, const DiskDev *disks,
                                     const int2 *map, int base, const DiskCtl *ctl, double *st,
                                     i64 ntot, const double *phi, const double *gR, const double *gp,
                                     int has_gR, int has_gp, const double *geom, const double *trig,
                                     double *pcfl, double *pcm, int par) {
   if (nblk <= 0) return hipSuccess;
-  if (garr)
-    hipLaunchKernelGGL(k_update_stream<true>, dim3(nblk), dim3(64), 0, s, disks, map, base, ctl, st,
-                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
-  else
-    hipLaunchKernelGGL(k_update_stream<false>, dim3(nblk), dim3(64), 0, s, disks, map, base, ctl, st,
-                       ntot, phi, gR, gp, has_gR, has_gp, geom, trig, pcfl, pcm, par);
+  auto k = garr ? (g2 ? k_update_stream<true, true> : k_update_stream<true, false>)
+                : (g2 ? k_update_stream<false, true> : k_update_stream<false, false>);
+  hipLaunchKernelGGL(k, dim3(nblk), dim3(64), 0, s, disks, map, base, ctl, st, ntot, phi, gR, gp,
+                     has_gR, has_gp, geom, trig, pcfl, pcm, par);
   return hipGetLastError();
 }
 

@@ -12,22 +12,26 @@ hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, Di
 hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const int2 *map,
                            int base, const DiskCtl *ctl, const double *st, long long ntot,
                            const double *geom, const double *trig, double *pcfl, double *pcm);
+// nctl > 0 appends one control workgroup per disk 0 .. nctl-1 to the launch.
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                          const int2 *map, int base, const DiskCtl *ctl, const double *st,
+                          const int2 *map, int base, DiskCtl *ctl, const double *st,
                           long long ntot, const double *geom, const double2 *tw, double2 *spec,
-                          int par);
+                          int par, int nctl, const double *pcfl, const double *pcm,
+                          DfkSampleDev *samples, int cap);
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                            const int2 *map, int base, const DiskCtl *ctl, const double *khat,
                            const double2 *tw, double2 *spec);
 hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
                           const double *geom, const double2 *tw, double *phi);
-hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int garr, const DiskDev *disks,
+// g2: every disk of the launch has gamma = 2 (EOS without pow).
+hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int garr, int g2,
+                             const DiskDev *disks,
                              const int2 *map, int base, const DiskCtl *ctl, double *st,
                              long long ntot, const double *phi, const double *gR,
                              const double *gp, int has_gR, int has_gp, const double *geom,
                              const double *trig, double *pcfl, double *pcm, int par);
-hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, const DiskDev *disks,
+hipError_t dfk_launch_update_stream(hipStream_t s, int nblk, int garr, int g2, const DiskDev *disks,
                                     const int2 *map, int base, const DiskCtl *ctl, double *st,
                                     long long ntot, const double *phi, const double *gR,
                                     const double *gp, int has_gR, int has_gp, const double *geom,

@@ -3,6 +3,7 @@
 // for line; the power-law model is this project's own (the reference has no
 // counterpart) and is pinned by its own tests (mass, Q_min, noise statistics,
 // determinism).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -230,7 +231,14 @@ double dfk_model_mass(const dfk_model *m, int Nu, int Nphi, const double *Sig) {
 int dfk_extract_growth(const double *t, const double *c0, const double *c2, const double *ph2,
                        int n, double lo_factor, double hi_frac, double grow_factor,
                        dfk_growth *out) {
-  if (!t || !c0 || !c2 || !ph2 || !out || n < 4) return 1;
+  return dfk_extract_growth_ex(t, c0, c2, ph2, n, lo_factor, HUGE_VAL, hi_frac, grow_factor, 0.25,
+                               out);
+}
+
+int dfk_extract_growth_ex(const double *t, const double *c0, const double *c2, const double *ph2,
+                          int n, double lo_factor, double hi_factor, double hi_frac,
+                          double grow_factor, double sat_frac, dfk_growth *out) {
+  if (!t || !c0 || !c2 || !ph2 || !out || n < 4 || !(sat_frac > 0.0 && sat_frac <= 1.0)) return 1;
   std::vector<double> a(n), ph(n);
   double amax = 0.0;
   for (int k = 0; k < n; ++k) {
@@ -249,8 +257,9 @@ int dfk_extract_growth(const double *t, const double *c0, const double *c2, cons
     int l = -1, h = -1;
     for (int k = 0; k < n; ++k)
       if (l < 0 && a[k] >= lo_factor * a[0]) l = k;
+    const double hi_level = std::min(hi_frac * amax, hi_factor * a[0]);
     for (int k = 0; k < n; ++k)
-      if (h < 0 && a[k] >= hi_frac * amax) h = k;
+      if (h < 0 && a[k] >= hi_level) h = k;
     if (l >= 0 && h >= 0 && h - l + 1 >= 4) {
       lo = l;
       hi = h;
@@ -271,7 +280,7 @@ int dfk_extract_growth(const double *t, const double *c0, const double *c2, cons
   }
   out->gamma = (stt > 0) ? sta / stt : 0.0;
   out->omega_p = (stt > 0) ? -0.5 * stp / stt : 0.0;
-  const int q0 = n - std::max(1, n / 4);
+  const int q0 = n - std::max(1, (int)(sat_frac * n));
   double ss = 0;
   for (int k = q0; k < n; ++k) ss += a[k];
   out->sat = ss / (n - q0);

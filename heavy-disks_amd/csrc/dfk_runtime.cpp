@@ -6,7 +6,9 @@
 // uploaded once, and stays resident; the K-hat transform itself runs on the
 // device with the same LDS FFT as the time loop.  The time loop is device
 // resident: dt, t, the output schedule and the C_m samples live in DiskCtl,
-// and dfk_advance launches a captured HIP graph of whole steps.
+// and dfk_advance launches a captured HIP graph of whole steps (four kernels
+// per step: row r2c with the loop control in trailing workgroups, column
+// convolution, row c2r, update).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -123,8 +125,6 @@ struct dfk_handle {
   double last_ms = 0.0;
   int chunk = 128;               // loop iterations per captured graph
   long long iter = 0;            // loop iterations launched since dfk_run_init
-  hipStream_t side = nullptr;    // second stream: the control kernel beside the gravity kernels
-  hipEvent_t evFork = nullptr, evJoin = nullptr;
   int thr_rows = 256, thr_cols = 256, thr_upd = 256;   // threads per workgroup
   int upd_kind = 0;              // 0: LDS-tile update kernel, 1: register-streaming
 };
@@ -261,12 +261,19 @@ double *field(dfk_handle *h, int buf, int f, int d) {
   return h->st + (i64)(buf * 3 + f) * h->ntot + h->hD[d].so;
 }
 
+// Row r2c of every self-gravitating disk, with the loop control of all disks
+// in trailing workgroups of the same launch (nctl = h->nd) or without (0).
+hipError_t launch_r2c(dfk_handle *h, int nblk, int base, int par, int nctl) {
+  return dfk_launch_r2c(h->stream, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl,
+                        h->st, h->ntot, h->geom, h->tw, h->spec, par, nctl, h->pcfl, h->pcm,
+                        h->samples, h->cap);
+}
+
 // Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk,
 // Sigma read from state buffer `par`.
 int launch_gravity_disk(dfk_handle *h, int d, int par) {
   const int nr = h->r2c0[d + 1] - h->r2c0[d], nc = h->conv0[d + 1] - h->conv0[d];
-  HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->st,
-                        h->ntot, h->geom, h->tw, h->spec, par));
+  HIPCHK(launch_r2c(h, nr, h->r2c0[d], par, 0));
   HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[d], h->ctl,
                          h->khat, h->tw, h->spec));
   HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
@@ -274,33 +281,33 @@ int launch_gravity_disk(dfk_handle *h, int d, int par) {
   return DFK_OK;
 }
 
+// Update of map range [base, base + nblk) from state buffer par; the
+// gamma = 2 kernels when every disk of the handle has gamma = 2.
 hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR, int has_gp, int par) {
+  int g2 = 1;
+  for (const DiskDev &D : h->hD) g2 &= (D.gamma2 != 0);
   if (h->upd_kind == 1)
-    return dfk_launch_update_stream(h->stream, nblk, garr, h->dD, h->map_upd, base, h->ctl, h->st,
+    return dfk_launch_update_stream(h->stream, nblk, garr, g2, h->dD, h->map_upd, base, h->ctl, h->st,
                                     h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
                                     h->trig, h->pcfl, h->pcm, par);
-  return dfk_launch_update(h->stream, nblk, h->thr_upd, h->lds_upd, garr, h->dD, h->map_upd, base,
+  return dfk_launch_update(h->stream, nblk, h->thr_upd, h->lds_upd, garr, g2, h->dD, h->map_upd, base,
                            h->ctl, h->st, h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
                            h->trig, h->pcfl, h->pcm, par);
 }
 
-// One whole loop iteration for every disk (captured into the graph).  Every
-// running disk's state is in buffer `par`; the update writes buffer 1 - par.
-// The control kernel runs on the side stream, beside the gravity kernels.
+
+// One whole loop iteration for every disk (captured into the graph): four
+// launches.  Every running disk's state is in buffer `par`; the update writes
+// buffer 1 - par.  The loop control rides in the r2c launch (a launch of its
+// own when no disk has self-gravity).
 int launch_step_all(dfk_handle *h, int par) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
   if (nr > 0) {
-    HIPCHK(hipEventRecord(h->evFork, h->stream));
-    HIPCHK(hipStreamWaitEvent(h->side, h->evFork, 0));
-    HIPCHK(dfk_launch_control(h->side, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
-    HIPCHK(hipEventRecord(h->evJoin, h->side));
-    HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
-                          h->geom, h->tw, h->spec, par));
+    HIPCHK(launch_r2c(h, nr, 0, par, h->nd));
     HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
                            h->tw, h->spec));
     HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
                           h->geom, h->tw, h->phi));
-    HIPCHK(hipStreamWaitEvent(h->stream, h->evJoin, 0));
   } else {
     HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
   }
@@ -530,11 +537,8 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
   h->hsamples.resize(ndisks);
   auto fail = [&](int rc) { dfk_destroy(h); return rc; };
   if (hipSetDevice(device) != hipSuccess) { set_err("hipSetDevice failed"); return fail(DFK_EHIP); }
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&h->evFork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->evJoin, hipEventDisableTiming) != hipSuccess) {
-    set_err("stream/event create failed");
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_err("stream create failed");
     return fail(DFK_EHIP);
   }
   hipEventCreate(&h->ev0);
@@ -640,9 +644,6 @@ void dfk_destroy(dfk_handle *h) {
   hipFree(h->ctl); hipFree(h->samples);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
-  if (h->evFork) hipEventDestroy(h->evFork);
-  if (h->evJoin) hipEventDestroy(h->evJoin);
-  if (h->side) hipStreamDestroy(h->side);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -990,6 +991,11 @@ int dfk_snapshot_load(dfk_handle *h, int d, const char *path) {
   c.step = hd.step;
   c.flags = hd.flags;
   c.done = 0;
+  // the disk's sample history restarts at the restored state (samples of the
+  // state it replaces, e.g. run_init's t = 0 output, are dropped)
+  c.nsamples = 0;
+  h->drained[d] = 0;
+  h->hsamples[d].clear();
   TRY(write_ctl(h, d, &c));
   // partials (CFL maximum, C_m sums) of the restored state
   const int nb_ = h->upd0[d + 1] - h->upd0[d];
@@ -1088,6 +1094,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
   for (int d = 0; d < h->nd; ++d) {
     if (!(c[d].dt > 0)) c[d].dt = 1e-6;
     c[d].tlim = 1e300;
+    c[d].tnext = 1e300;   // no samples: the ring keeps the run's own
     c[d].max_steps = 0;
   }
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
@@ -1101,8 +1108,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
           HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
           break;
         case 1:
-          HIPCHK(dfk_launch_r2c(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->st, h->ntot,
-                                h->geom, h->tw, h->spec, par));
+          HIPCHK(launch_r2c(h, nr, 0, par, h->nd));   // as in the loop: with the control
           break;
         case 2:
           HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,

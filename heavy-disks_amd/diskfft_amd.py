@@ -338,7 +338,7 @@ class Ensemble:
 # disk models (include/diskfft_model.h; host code, no GPU needed)
 MODEL_LKA98, MODEL_POWERLAW = 0, 1
 MODEL_SYMBOLS = ["dfk_model_defaults", "dfk_model_sigma0", "dfk_model_state", "dfk_model_mass",
-                 "dfk_splitmix64", "dfk_normal", "dfk_extract_growth"]
+                 "dfk_splitmix64", "dfk_normal", "dfk_extract_growth", "dfk_extract_growth_ex"]
 
 
 class Growth(ctypes.Structure):
@@ -371,6 +371,9 @@ def _model_lib():
         L.dfk_extract_growth.argtypes = [_DP, _DP, _DP, _DP, _I, _D, _D, _D,
                                          ctypes.POINTER(Growth)]
         L.dfk_extract_growth.restype = _I
+        L.dfk_extract_growth_ex.argtypes = [_DP, _DP, _DP, _DP, _I, _D, _D, _D, _D, _D,
+                                            ctypes.POINTER(Growth)]
+        L.dfk_extract_growth_ex.restype = _I
         L._model_sig = True
     return L
 
@@ -431,15 +434,23 @@ def model_ic(m, Nu, Nphi, eps, selfgrav=1, potential_forces=None, device=0):
     return p, S, U, W, model_mass(m, S), Q
 
 
-def extract_growth(samples, lo_factor=3.0, hi_frac=0.3, grow_factor=10.0):
+def extract_growth(samples, lo_factor=3.0, hi_frac=0.3, grow_factor=10.0, hi_factor=None,
+                   sat_frac=0.25):
     """gamma, Omega_p and saturated |C2|/|C0| from an (n, 4) array of
-    (t, |C0|, |C2|, arg C2) samples (see dfk_extract_growth)."""
+    (t, |C0|, |C2|, arg C2) samples (see dfk_extract_growth; hi_factor and
+    sat_frac select dfk_extract_growth_ex)."""
     a = np.ascontiguousarray(np.asarray(samples, dtype=np.float64))
     cols = [np.ascontiguousarray(a[:, k]) for k in range(4)]
     g = Growth()
-    rc = _model_lib().dfk_extract_growth(*[_dp(c) for c in cols], len(a), lo_factor, hi_frac,
-                                         grow_factor, ctypes.byref(g))
+    if hi_factor is None and sat_frac == 0.25:
+        rc = _model_lib().dfk_extract_growth(*[_dp(c) for c in cols], len(a), lo_factor, hi_frac,
+                                             grow_factor, ctypes.byref(g))
+    else:
+        rc = _model_lib().dfk_extract_growth_ex(
+            *[_dp(c) for c in cols], len(a), lo_factor,
+            np.inf if hi_factor is None else hi_factor, hi_frac, grow_factor, sat_frac,
+            ctypes.byref(g))
     if rc:
-        raise ValueError("dfk_extract_growth needs at least 4 samples")
+        raise ValueError("the growth extractor needs at least 4 samples and 0 < sat_frac <= 1")
     return dict(gamma=g.gamma, omega_p=g.omega_p, sat=g.sat, t_lo=g.t_lo, t_hi=g.t_hi,
                 n_fit=g.n_fit, grew=bool(g.grew))

@@ -2,8 +2,10 @@
 
 An ensemble of BF16 MLPs trained on the GPU by libdiskfft's own fused kernels
 (include/diskfft_mlp.h).  Inputs are (M_d/M*, Q_min, softening s, Sigma
-slope), mapped linearly onto [-1, 1] from the ensemble's parameter box; the
-three targets are standardised with the training set's mean and deviation.
+slope), mapped linearly onto [-1, 1] from the ensemble's parameter box.  The
+three targets are heavy tailed, so each is compressed as asinh(y / s), s being
+the training set's median |y|, then standardised with the mean and deviation
+of the compressed training values.
 Where the members disagree by more than `spread_tol` (relative deviation of
 any output across members) the caller should fall back to the hydro, which
 stays the judge of the surrogate's error.
@@ -133,8 +135,6 @@ class Surrogate:
         self.sd = np.ones(3)
         self.sc = np.ones(3)
 
-    # targets are heavy tailed (fast inner-edge modes): the network learns
-    # asinh(y / s) with s the median |y| of the training set, standardised
     def _fwd(self, Y):
         return np.arcsinh(np.asarray(Y, dtype=np.float64) / self.sc)
 

@@ -47,7 +47,8 @@ def errors(pred, true, scale):
         r = rel[:, k][rel_mask[:, k]]
         out[n] = dict(mean_range_err=float(e[:, k].mean()), p90_range_err=float(np.quantile(e[:, k], 0.9)),
                       median_rel_err=float(np.median(r)) if len(r) else None,
-                      mean_rel_err=float(r.mean()) if len(r) else None)
+                      mean_rel_err=float(r.mean()) if len(r) else None,
+                      frac_rel_le_3pct=float(np.mean(r <= 0.03)) if len(r) else None)
     out["all_mean_range_err"] = float(e.mean())
     return out
 

@@ -131,7 +131,8 @@ int dfk_get_status(dfk_handle *h, int disk, dfk_status *st);
  * a binary file.  load, called after dfk_run_init, restores state and clock of
  * a disk with the same grid and parameters (DFK_EINVAL otherwise); the run
  * then continues exactly as the saved one would have (dt is a function of the
- * state only). */
+ * state only).  The disk's undrained samples are discarded by load: its
+ * sample series restarts with the first output after the restored time. */
 int dfk_snapshot_save(dfk_handle *h, int disk, const char *path);
 int dfk_snapshot_load(dfk_handle *h, int disk, const char *path);
 

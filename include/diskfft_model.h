@@ -85,6 +85,15 @@ int dfk_extract_growth(const double *t, const double *c0, const double *c2, cons
                        int n, double lo_factor, double hi_frac, double grow_factor,
                        dfk_growth *out);
 
+/* The same with two more knobs: the window also ends no later than the first
+ * sample with a >= hi_factor * a(0) (so a later, larger peak cannot stretch
+ * the window of the first linear growth across a plateau), and sat is the
+ * mean of a over the last sat_frac of the samples (0 < sat_frac <= 1).
+ * dfk_extract_growth is hi_factor = +inf, sat_frac = 0.25. */
+int dfk_extract_growth_ex(const double *t, const double *c0, const double *c2, const double *ph2,
+                          int n, double lo_factor, double hi_factor, double hi_frac,
+                          double grow_factor, double sat_frac, dfk_growth *out);
+
 /* SplitMix64 stream: next 64-bit value (state updated), and a standard normal
  * by Box-Muller (one value per call; the pair's sine half is discarded). */
 unsigned long long dfk_splitmix64(unsigned long long *state);

@@ -7,7 +7,9 @@ Cases (name: model, grid, eps, tlim or steps, CFL):
   lka128_e<eps>   LKA98 ring 128x128, tlim 9, CFL 0.3 (diskfft/README.md:27-30)
   lka256          LKA98 ring 256x256, eps 0.005, tlim 16 (diskfft/README.md:36-43)
   pl128           power-law model 128x128, eps 0.01, 2000 steps, CFL 0.4
-  pl256           power-law model 256x256, eps 0.01, 20 outer orbits (t = 89), CFL 0.4
+  pl256           power-law model 256x256, eps 0.01, 20 outer orbits (t = 89), CFL 0.4;
+                  samples every 0.01, gamma / Omega_p of the first (inner-edge) growth
+                  from 10x to 100x the initial |C2|/|C0|, sat over the second half
 The oracle runs with OpenMP on this host's cores (<= 16).
 """
 import argparse
@@ -32,7 +34,8 @@ def case(name):
         return dict(model="lka98", N=(128, 128), eps=float(name[8:]), tlim=9.0, steps=0, cfl=0.3)
     return {"lka256": dict(model="lka98", N=(256, 256), eps=0.005, tlim=16.0, steps=0, cfl=0.3),
             "pl128": dict(model="powerlaw", N=(128, 128), eps=0.01, tlim=1e9, steps=2000, cfl=0.4),
-            "pl256": dict(model="powerlaw", N=(256, 256), eps=0.01, tlim=89.0, steps=0, cfl=0.4),
+            "pl256": dict(model="powerlaw", N=(256, 256), eps=0.01, tlim=89.0, steps=0, cfl=0.4,
+                          dtout=0.01, ex=dict(lo_factor=10.0, hi_factor=100.0, sat_frac=0.5)),
             }[name]
 
 
@@ -48,7 +51,9 @@ def run_case(name):
     e = dfk.Ensemble(p)
     e.set_state(S, U, W)
     t0 = time.perf_counter()
-    ((sg, stg),) = e.run(c["tlim"], dtout=0.25, cfl=c["cfl"], mD=mD, max_steps=c["steps"],
+    dtout = c.get("dtout", 0.25)
+    e.set_graph_steps(500)
+    ((sg, stg),) = e.run(c["tlim"], dtout=dtout, cfl=c["cfl"], mD=mD, max_steps=c["steps"],
                          chunk=20000)
     print(f"  {name} gpu: {stg['steps']} steps to t = {stg['t']:.2f}", flush=True)
     tg = time.perf_counter() - t0
@@ -56,11 +61,12 @@ def run_case(name):
     h.set_state(S, U, W)
     t0 = time.perf_counter()
     so, tf, c2f, steps = oracle.run_chunked(
-        h, k, 1, c["tlim"], dtout=0.25, cfl=c["cfl"], max_steps=c["steps"], mD=mD, chunk=20000,
+        h, k, 1, c["tlim"], dtout=dtout, cfl=c["cfl"], max_steps=c["steps"], mD=mD, chunk=20000,
         progress=lambda t, n: print(f"  {name} oracle: t = {t:.2f} after {n} steps "
                                     f"({time.perf_counter() - t0:.0f} s)", flush=True))
     to = time.perf_counter() - t0
-    gg, go = dfk.extract_growth(sg), dfk.extract_growth(so)
+    ex = c.get("ex", {})
+    gg, go = dfk.extract_growth(sg, **ex), dfk.extract_growth(so, **ex)
     rel = {q: abs(gg[q] - go[q]) / abs(go[q]) if go[q] else None
            for q in ("gamma", "omega_p", "sat")}
     Sg, _, _ = e.get_state()

@@ -56,3 +56,31 @@ def test_no_growth_uses_whole_series(dfk):
 def test_too_short_raises(dfk):
     with pytest.raises(ValueError):
         dfk.extract_growth(np.zeros((3, 4)))
+
+
+def test_window_cap_keeps_first_growth(dfk):
+    """An edge mode grows e^{9t} from 2e-5 to a plateau; much later a second,
+    larger peak appears.  With hi_factor the fit window stops inside the first
+    growth (exact rate and pattern speed); without it the window stretches to
+    0.3 of the late peak, across the plateau."""
+    t = np.arange(0, 40, 0.01)
+    la = np.minimum(np.log(2e-5) + 9.0 * t, np.log(2e-2))          # growth, then plateau
+    la = np.where(t > 30, la + 2.5 * (t - 30) / 10.0, la)           # late rise to ~0.24
+    ph = np.angle(np.exp(1j * (0.4 - 2 * 13.5 * t)))
+    s = np.stack([t, np.ones_like(t), np.exp(la), ph], axis=1)
+    g = dfk.extract_growth(s, lo_factor=10.0, hi_factor=100.0, hi_frac=0.3, sat_frac=0.5)
+    assert g["grew"] and g["t_hi"] < 0.8
+    assert abs(g["gamma"] - 9.0) < 1e-9
+    assert abs(g["omega_p"] - 13.5) < 1e-9
+    assert abs(g["sat"] - np.mean(np.exp(la[len(t) - len(t) // 2:]))) < 1e-15
+    g_old = dfk.extract_growth(s, lo_factor=10.0, hi_frac=0.3)
+    assert g_old["t_hi"] > 30 and g_old["gamma"] < 1.0
+
+
+def test_ex_defaults_reproduce_extractor(dfk):
+    s = _series(0.72, 1.57)
+    a = dfk.extract_growth(s, lo_factor=3.0, hi_frac=0.05)
+    b = dfk.extract_growth(s, lo_factor=3.0, hi_frac=0.05, hi_factor=np.inf, sat_frac=0.25 + 1e-17)
+    assert a == b
+    with pytest.raises(ValueError):
+        dfk.extract_growth(s, sat_frac=0.0)

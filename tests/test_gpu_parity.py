@@ -179,6 +179,47 @@ def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps, kind):
     assert abs(sg[0, 2] - so[0, 2]) / so[0, 2] < 1e-12
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+def test_general_eos_loop_and_mixed_ensemble(orc, dfk, kind):
+    """gamma != 2 takes the EOS path with pow (hydro.hpp:44-45): 10-step loop
+    parity with the oracle; a handle mixing gamma = 2 and 1.5 disks runs that
+    path for both and still gives each disk the bits of its own run."""
+    print(f"case general EOS, update kernel {kind}", flush=True)
+    Nu, Nphi, eps = 64, 64, 0.02
+    runs = []
+    for gamma, K in [(1.5, 0.2), (2.0, 0.25)]:
+        h = orc.Hydro(Nu, Nphi, K=K, gamma=gamma)
+        k = orc.Kalnajs(Nu, Nphi, 0.25, 1.0, eps)
+        orc.ic_lka98(h, k, 1, 1e-3)
+        p = dfk.disk_params(Nu, Nphi, eps=eps, K=K, gamma=gamma)
+        runs.append((h, k, p, h.state(), h.disk_mass()))
+    h, k, p, ic, mD = runs[0]
+    e = dfk.Ensemble(p)
+    e.set_update_kernel(kind)
+    e.set_state(*ic)
+    orc.run(h, k, 1, 1e9, dtout=0.25, cfl=0.3, max_steps=10, mD=mD)
+    e.run_init(1e9, dtout=0.25, cfl=0.3, mD=mD, max_steps=10)
+    e.advance(12)
+    errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
+    print("  gamma 1.5 rel errors Sigma, vR, vphi:", errs, flush=True)
+    assert e.status()["steps"] == 10 and max(errs) <= LOOP_TOL
+    mixed = dfk.Ensemble([r[2] for r in runs])
+    mixed.set_update_kernel(kind)
+    for d, r in enumerate(runs):
+        mixed.set_state(*r[3], d=d)
+    mixed.run_init(1e9, mD=[r[4] for r in runs], max_steps=15)
+    mixed.advance(16)
+    for d, r in enumerate(runs):
+        one = dfk.Ensemble(r[2])
+        one.set_update_kernel(kind)
+        one.set_shapes(*mixed.get_shapes(d))
+        one.set_state(*r[3])
+        one.run_init(1e9, mD=r[4], max_steps=15)
+        one.advance(16)
+        for a, b in zip(mixed.get_state(d), one.get_state()):
+            assert np.array_equal(a, b)
+
+
 def test_loop_samples_and_clip(orc, dfk):
     """Output schedule and last-step clip of diskmain.cpp:92-110 over ~300 steps."""
     Nu, Nphi, eps, tlim = 64, 64, 0.02, 0.6
