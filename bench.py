@@ -74,6 +74,7 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk):
     prof = e.profile_step(200)
     shapes = dict(zip(("ti", "tj", "pr", "cb"), e.get_shapes(0)))
     shapes["threads"] = e.get_threads()
+    shapes["kind"] = e.get_update_kernel()
     shapes["tuned"] = bool(tuned)
     e.close()
     return dict(p=p, S=S, U=U, W=W, mD=mD, wall=wall, dev_ms=dev_ms, prof=prof, shapes=shapes)
@@ -188,8 +189,13 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_single_{Nu}x{Nphi}.json")
     if os.path.exists(tpath):
-        kname = {"update": "k_update<false>"}.get(dom, "k_" + dom)
-        traffic = json.load(open(tpath))["kernels"].get(kname, {}).get("traffic_bytes")
+        tk = json.load(open(tpath))["kernels"]
+        if dom == "update":   # the loop's build: no force arrays, kernel of the tuned kind
+            pre = "k_update_stream<false" if r["shapes"].get("kind") == 1 else "k_update<false"
+            names = [k for k in tk if k.startswith(pre)]
+        else:
+            names = [k for k in tk if k == "k_" + dom]
+        traffic = tk[names[0]]["traffic_bytes"] if names else None
     out = {
         "metric": METRIC,
         "value": zps,

@@ -242,6 +242,115 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int n, int batch, int nst,
   return a;
 }
 
+// ------------------------------------------------------------------ stages with global I/O
+// Stockham stages (radix 2, 3, 4 or 8) whose input or output goes straight
+// to HBM instead of through an LDS staging pass, for transforms of >= 2
+// stages with only those radices:
+//   IO_ZP    column convolution, first forward stage (Ns = 1, no twiddles):
+//            column col of the half spectrum, rows >= Nu being the zero
+//            padding (for an even radix exactly the inputs m >= R/2)
+//   IO_KHAT  column convolution, last forward stage: outputs x real K-hat
+//            (rows M - p and p share a value)
+//   IO_OUT   column convolution, last inverse stage: only rows < Nu stored
+//   IO_ROUT  row c2r, last stage: batch entry p holds rows row0 + 2p (real
+//            part) and row0 + 2p + 1 (imaginary part); x pref / sqrt(R) into Phi
+// Butterflies and products are those of the LDS passes they replace, so the
+// results are the same bits.
+enum { IO_LDS = 0, IO_ZP = 1, IO_KHAT = 2, IO_OUT = 3, IO_ROUT = 4 };
+
+struct StageIO {
+  double2 *spec;          // IO_ZP / IO_OUT: the disk's half spectrum [Nc][Nu]
+  const double *kh;       // IO_KHAT: K-hat rows 0..Nu of the batch's columns (LDS)
+  const double *rsc;      // IO_ROUT: pref / sqrt(R) per row
+  double *P;              // IO_ROUT: Phi of the disk
+  int Nu, Nc, k0, row0;   // k0: first column of the batch; row0: first row
+};
+
+template <int R, int SIGN, int IO>
+__device__ __forceinline__ void io_stage(const double2 *__restrict__ a, double2 *__restrict__ b,
+                                         int n, int batch, int Ns, const double2 *__restrict__ tw,
+                                         const StageIO &g) {
+  const int nb = n / R;
+  const int tstep = n / (Ns * R);
+  const int total = nb * batch;
+  const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int p = qdiv(e, rnb);
+    const int j = e - p * nb;
+    const int jq = qdiv(j, rNs);
+    const int k = j - jq * Ns;
+    const int col = g.k0 + p;
+    const bool live = col < g.Nc;
+    const int ra = g.row0 + 2 * p, rb = ra + 1;
+    double2 v[R];
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const int idx = j + m * nb;
+      double2 x;
+      if constexpr (IO == IO_ZP) {
+        const bool pad = (R % 2 == 0) ? (m >= R / 2) : (idx >= g.Nu);
+        x = (live && !pad) ? g.spec[(i64)col * g.Nu + idx] : make_double2(0.0, 0.0);
+      } else {
+        x = a[p * n + idx];
+        if (m > 0 && k > 0) x = cmul(x, twid(tw, k * m * tstep, SIGN));
+      }
+      v[m] = x;
+    }
+    if constexpr (R == 2) bfly2<SIGN>(v[0], v[1]);
+    if constexpr (R == 3) bfly3<SIGN>(v);
+    if constexpr (R == 4) bfly4<SIGN>(v);
+    if constexpr (R == 8) bfly8<SIGN>(v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int o = jq * Ns * R + k + q * Ns;
+      if constexpr (IO == IO_KHAT) {
+        const double kv = g.kh[p * (g.Nu + 1) + (o <= g.Nu ? o : n - o)];
+        double2 z = v[q];
+        z.x *= kv;
+        z.y *= kv;
+        b[p * n + o] = z;
+      } else if constexpr (IO == IO_OUT) {
+        if (live && o < g.Nu) g.spec[(i64)col * g.Nu + o] = v[q];
+      } else if constexpr (IO == IO_ROUT) {
+        if (ra < g.Nu) g.P[(i64)ra * n + o] = v[q].x * g.rsc[ra];
+        if (rb < g.Nu) g.P[(i64)rb * n + o] = v[q].y * g.rsc[rb];
+      } else {
+        b[p * n + o] = v[q];
+      }
+    }
+  }
+}
+
+template <int SIGN, int IO>
+__device__ __forceinline__ void io_stage_r(int r, const double2 *a, double2 *b, int n, int batch,
+                                           int Ns, const double2 *tw, const StageIO &g) {
+  if (r == 8) io_stage<8, SIGN, IO>(a, b, n, batch, Ns, tw, g);
+  else if (r == 4) io_stage<4, SIGN, IO>(a, b, n, batch, Ns, tw, g);
+  else if (r == 2) io_stage<2, SIGN, IO>(a, b, n, batch, Ns, tw, g);
+  else io_stage<3, SIGN, IO>(a, b, n, batch, Ns, tw, g);
+}
+
+// Whether a transform can use the global-I/O stages.
+__device__ __forceinline__ bool io_fast(int nst, const int *rad) {
+  bool f = nst >= 2;
+  for (int q = 0; q < nst; ++q) f = f && (rad[q] == 2 || rad[q] == 3 || rad[q] == 4 || rad[q] == 8);
+  return f;
+}
+
+// Middle stages 1 .. last-1 (LDS -> LDS) of a transform whose first stage
+// wrote x; returns the buffer holding the result of stage `last - 1`.
+template <int SIGN>
+__device__ double2 *mid_stages(double2 *x, double2 *y, int n, int batch, int nst,
+                               const int *rad, const double2 *tw, const StageIO &g, int *Ns) {
+  for (int q = 1; q < nst - 1; ++q) {
+    io_stage_r<SIGN, IO_LDS>(rad[q], x, y, n, batch, *Ns, tw, g);
+    __syncthreads();
+    double2 *t = x; x = y; y = t;
+    *Ns *= rad[q];
+  }
+  return x;
+}
+
 // ------------------------------------------------------------------ reductions
 // Workgroup reduction of (max, 4 sums) in a fixed order (deterministic).
 __device__ void wg_reduce5(double &mx, double &s0, double &s1, double &s2, double &s3,
@@ -430,6 +539,8 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   stage_twiddles(TW, tw + D.two_row, n);
   const double *S = stp(st, ntot, par, 0, D);
   const double *R32 = gtab(geom, D, GT_R32);
+  // all threads stage the rows (more loads in flight than a first stage
+  // reading HBM through its few butterflies: measured faster here)
   for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
@@ -472,6 +583,39 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   double2 *A = lds2, *B = lds2 + cb * M, *TW = lds2 + 2 * cb * M;
   stage_twiddles(TW, tw + D.two_col, M);
   double2 *io = spec + D.spo;
+  const double *kh = khat + D.kho;
+  if (io_fast(D.nst_col, D.rad_col)) {
+    const int ns = D.nst_col;
+    // the batch's K-hat columns go to LDS beside the twiddles (loads in
+    // flight during the first stage; zero for columns past Nc)
+    double *KH = reinterpret_cast<double *>(TW + M);
+    for (int e = threadIdx.x; e < cb * (Nu + 1); e += blockDim.x) {
+      const int c = e / (Nu + 1), p = e - c * (Nu + 1);
+      KH[e] = (k0 + c < Nc) ? kh[(i64)(k0 + c) * (Nu + 1) + p] : 0.0;
+    }
+    StageIO g{};
+    g.spec = io; g.kh = KH; g.Nu = Nu; g.Nc = Nc; g.k0 = k0;
+    // forward: spectrum column -> A (first stage), ..., last stage x K-hat
+    io_stage_r<-1, IO_ZP>(D.rad_col[0], nullptr, A, M, cb, 1, TW, g);
+    __syncthreads();
+    int Ns = D.rad_col[0];
+    double2 *x = mid_stages<-1>(A, B, M, cb, ns, D.rad_col, TW, g, &Ns);
+    double2 *y = (x == A) ? B : A;
+    io_stage_r<-1, IO_KHAT>(D.rad_col[ns - 1], x, y, M, cb, Ns, TW, g);
+    __syncthreads();
+    // inverse: stages 0 .. ns-2 in LDS, the last one -> rows < Nu of the column
+    x = y;
+    y = (x == A) ? B : A;
+    Ns = 1;
+    for (int q = 0; q < ns - 1; ++q) {
+      io_stage_r<+1, IO_LDS>(D.rad_col[q], x, y, M, cb, Ns, TW, g);
+      __syncthreads();
+      double2 *t = x; x = y; y = t;
+      Ns *= D.rad_col[q];
+    }
+    io_stage_r<+1, IO_OUT>(D.rad_col[ns - 1], x, y, M, cb, Ns, TW, g);
+    return;
+  }
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
@@ -482,7 +626,6 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   __syncthreads();
   double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, TW);
   double2 *Wb = (Z == A) ? B : A;
-  const double *kh = khat + D.kho;
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;
@@ -529,9 +672,23 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
     if (k > 0 && n - k >= Nc) A[p * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
   }
   __syncthreads();
-  const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
   const double *sc = gtab(geom, D, GT_PHISC);
   double *P = phi + D.so;
+  if (io_fast(D.nst_row, D.rad_row)) {
+    StageIO g{};
+    g.rsc = sc; g.P = P; g.Nu = Nu; g.row0 = row0;
+    double2 *x = A, *y = B;
+    int Ns = 1;
+    for (int q = 0; q < D.nst_row - 1; ++q) {
+      io_stage_r<+1, IO_LDS>(D.rad_row[q], x, y, n, pr, Ns, TW, g);
+      __syncthreads();
+      double2 *t = x; x = y; y = t;
+      Ns *= D.rad_row[q];
+    }
+    io_stage_r<+1, IO_ROUT>(D.rad_row[D.nst_row - 1], x, y, n, pr, Ns, TW, g);
+    return;
+  }
+  const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
   for (int e = threadIdx.x; e < 2 * pr * n; e += blockDim.x) {
     const int rr = e / n, j = e - rr * n;
     const int row = row0 + rr;

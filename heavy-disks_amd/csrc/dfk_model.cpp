@@ -232,12 +232,13 @@ int dfk_extract_growth(const double *t, const double *c0, const double *c2, cons
                        int n, double lo_factor, double hi_frac, double grow_factor,
                        dfk_growth *out) {
   return dfk_extract_growth_ex(t, c0, c2, ph2, n, lo_factor, HUGE_VAL, hi_frac, grow_factor, 0.25,
-                               out);
+                               HUGE_VAL, out);
 }
 
 int dfk_extract_growth_ex(const double *t, const double *c0, const double *c2, const double *ph2,
                           int n, double lo_factor, double hi_factor, double hi_frac,
-                          double grow_factor, double sat_frac, dfk_growth *out) {
+                          double grow_factor, double sat_frac, double peak_window,
+                          dfk_growth *out) {
   if (!t || !c0 || !c2 || !ph2 || !out || n < 4 || !(sat_frac > 0.0 && sat_frac <= 1.0)) return 1;
   std::vector<double> a(n), ph(n);
   double amax = 0.0;
@@ -288,6 +289,10 @@ int dfk_extract_growth_ex(const double *t, const double *c0, const double *c2, c
   out->t_hi = t[hi];
   out->n_fit = m;
   out->grew = grew;
+  double pk = 0.0;
+  for (int k = 0; k < n; ++k)
+    if (!grew || (t[k] >= t[lo] && t[k] <= t[lo] + peak_window)) pk = std::max(pk, a[k]);
+  out->sat_peak = pk;
   return 0;
 }
 

@@ -165,6 +165,7 @@ int build_layout(dfk_handle *h) {
   h->conv0.assign(h->nd + 1, 0);
   i64 part = 0;
   size_t lu = 0, lr = 0, lc = 0;
+  const size_t kMaxLds = 64 * 1024;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
     if (h->upd_kind == 1) D.tj = kStreamCols;   // one wave = 60 output columns
@@ -184,16 +185,23 @@ int build_layout(dfk_handle *h) {
     h->r2c0[d] = (int)mr.size();
     h->conv0[d] = (int)mc.size();
     if (D.selfgrav) {
+      // batches shrink until they fit the LDS budget (a cached shape from a
+      // build with smaller LDS needs stays usable)
+      auto lds_rows = [&](int pr) { return (size_t)(2 * pr + 1) * D.Nphi * sizeof(double2); };
+      auto lds_cols = [&](int cb) {   // A, B, twiddles + K-hat columns
+        return (size_t)(2 * cb + 1) * D.M * sizeof(double2) + (size_t)cb * (D.Nu + 1) * sizeof(double);
+      };
+      while (D.pr > 1 && lds_rows(D.pr) > kMaxLds) --D.pr;
+      while (D.cb > 1 && lds_cols(D.cb) > kMaxLds) --D.cb;
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
       for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) mc.push_back(make_int2(d, g));
-      lr = std::max(lr, (size_t)(2 * D.pr + 1) * D.Nphi * sizeof(double2));   // + twiddles
-      lc = std::max(lc, (size_t)(2 * D.cb + 1) * D.M * sizeof(double2));
+      lr = std::max(lr, lds_rows(D.pr));
+      lc = std::max(lc, lds_cols(D.cb));
     }
   }
   h->upd0[h->nd] = (int)mu.size();
   h->r2c0[h->nd] = (int)mr.size();
   h->conv0[h->nd] = (int)mc.size();
-  const size_t kMaxLds = 64 * 1024;
   if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds) {
     set_err("tile/batch shapes need %zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, kMaxLds);
     return DFK_EINVAL;

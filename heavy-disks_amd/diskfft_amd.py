@@ -343,7 +343,7 @@ MODEL_SYMBOLS = ["dfk_model_defaults", "dfk_model_sigma0", "dfk_model_state", "d
 
 class Growth(ctypes.Structure):
     _fields_ = [("gamma", _D), ("omega_p", _D), ("sat", _D), ("t_lo", _D), ("t_hi", _D),
-                ("n_fit", _I), ("grew", _I)]
+                ("n_fit", _I), ("grew", _I), ("sat_peak", _D)]
 
 
 class Model(ctypes.Structure):
@@ -371,7 +371,7 @@ def _model_lib():
         L.dfk_extract_growth.argtypes = [_DP, _DP, _DP, _DP, _I, _D, _D, _D,
                                          ctypes.POINTER(Growth)]
         L.dfk_extract_growth.restype = _I
-        L.dfk_extract_growth_ex.argtypes = [_DP, _DP, _DP, _DP, _I, _D, _D, _D, _D, _D,
+        L.dfk_extract_growth_ex.argtypes = [_DP, _DP, _DP, _DP, _I, _D, _D, _D, _D, _D, _D,
                                             ctypes.POINTER(Growth)]
         L.dfk_extract_growth_ex.restype = _I
         L._model_sig = True
@@ -435,22 +435,27 @@ def model_ic(m, Nu, Nphi, eps, selfgrav=1, potential_forces=None, device=0):
 
 
 def extract_growth(samples, lo_factor=3.0, hi_frac=0.3, grow_factor=10.0, hi_factor=None,
-                   sat_frac=0.25):
+                   sat_frac=0.25, peak_window=None):
     """gamma, Omega_p and saturated |C2|/|C0| from an (n, 4) array of
-    (t, |C0|, |C2|, arg C2) samples (see dfk_extract_growth; hi_factor and
-    sat_frac select dfk_extract_growth_ex)."""
+    (t, |C0|, |C2|, arg C2) samples (see dfk_extract_growth; hi_factor,
+    sat_frac and peak_window select dfk_extract_growth_ex, which also
+    returns sat_peak)."""
     a = np.ascontiguousarray(np.asarray(samples, dtype=np.float64))
     cols = [np.ascontiguousarray(a[:, k]) for k in range(4)]
     g = Growth()
-    if hi_factor is None and sat_frac == 0.25:
+    ex = not (hi_factor is None and sat_frac == 0.25 and peak_window is None)
+    if not ex:
         rc = _model_lib().dfk_extract_growth(*[_dp(c) for c in cols], len(a), lo_factor, hi_frac,
                                              grow_factor, ctypes.byref(g))
     else:
         rc = _model_lib().dfk_extract_growth_ex(
             *[_dp(c) for c in cols], len(a), lo_factor,
             np.inf if hi_factor is None else hi_factor, hi_frac, grow_factor, sat_frac,
-            ctypes.byref(g))
+            np.inf if peak_window is None else peak_window, ctypes.byref(g))
     if rc:
         raise ValueError("the growth extractor needs at least 4 samples and 0 < sat_frac <= 1")
-    return dict(gamma=g.gamma, omega_p=g.omega_p, sat=g.sat, t_lo=g.t_lo, t_hi=g.t_hi,
-                n_fit=g.n_fit, grew=bool(g.grew))
+    out = dict(gamma=g.gamma, omega_p=g.omega_p, sat=g.sat, t_lo=g.t_lo, t_hi=g.t_hi,
+               n_fit=g.n_fit, grew=bool(g.grew))
+    if ex:
+        out["sat_peak"] = g.sat_peak
+    return out

@@ -2,12 +2,18 @@
 
     python heavy-disks_amd/prof_summary.py stats DIR            # --kernel-trace --stats run
     python heavy-disks_amd/prof_summary.py pmc DIR [DIR ...]    # --pmc runs (FETCH_SIZE, WRITE_SIZE, ...)
+    python heavy-disks_amd/prof_summary.py traffic OUT.json WORKLOAD FETCH_DIR WRITE_DIR \
+        COPY_FETCH_DIR COPY_WRITE_DIR COPY_MIB
 
 `stats` prints calls and average/min/max duration per kernel from
 DIR/*kernel_stats.csv; `pmc` prints the per-dispatch average of every counter
 per kernel from DIR/*counter_collection.csv (FETCH_SIZE and WRITE_SIZE are in
-KiB per dispatch).
+KiB per dispatch).  `traffic` turns one FETCH_SIZE and one WRITE_SIZE run into
+HBM bytes per launch, each counter scaled by what it reads for hbm_copy
+(COPY_MIB MiB read and written per launch), and writes OUT.json (read by
+bench.py for the roofline's traffic field).
 """
+import json
 import collections
 import csv
 import glob
@@ -46,8 +52,40 @@ def pmc(dirs):
         print(f"| {k} | {n} | " + " | ".join(cells) + " |")
 
 
+def per_kernel(d, counter):
+    agg = collections.defaultdict(list)
+    f = glob.glob(os.path.join(d, "*counter_collection.csv"))[0]
+    for r in csv.DictReader(open(f)):
+        if r["Counter_Name"] == counter:
+            agg[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in agg.items()}
+
+
+def traffic(out, workload, fdir, wdir, cfdir, cwdir, copy_mib):
+    kib = float(copy_mib) * 1024.0
+    cf = per_kernel(cfdir, "FETCH_SIZE")["copy_kernel"]
+    cw = per_kernel(cwdir, "WRITE_SIZE")["copy_kernel"]
+    ff, wf = kib / cf, kib / cw
+    fk, wk = per_kernel(fdir, "FETCH_SIZE"), per_kernel(wdir, "WRITE_SIZE")
+    res = {"workload": workload, "fetch_factor": ff, "write_factor": wf, "kernels": {}}
+    for k in sorted(set(fk) & set(wk)):
+        if k.startswith("__amd"):
+            continue
+        res["kernels"][k] = {"fetch_kib": fk[k], "write_kib": wk[k],
+                             "traffic_bytes": 1024.0 * (ff * fk[k] + wf * wk[k])}
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+    print(f"copy kernel: FETCH_SIZE {cf:.1f} KiB, WRITE_SIZE {cw:.1f} KiB for {kib:.0f} KiB "
+          f"each way -> factors {ff:.3f}, {wf:.3f}")
+    for k, v in res["kernels"].items():
+        print(f"| {k} | {v['fetch_kib']:.1f} | {v['write_kib']:.1f} | "
+              f"{v['traffic_bytes'] / 1e6:.3f} |")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(sys.argv[2])
+    elif sys.argv[1] == "traffic":
+        traffic(*sys.argv[2:9])
     else:
         pmc(sys.argv[2:])

@@ -1,7 +1,11 @@
 """Training data for the surrogate: batched hydro runs of the power-law disk.
 
 Each run draws (M_d/M*, Q_min, softening s, slope) from the ensemble
-distribution (ensemble_cfg.draw), builds its equilibrium with this library's
+distribution (ensemble_cfg.draw).  The 1e-3 seed noise uses one fixed
+realization (noise_seed, default 42) for every run, so that the targets are a
+function of the four parameters: with per-run seeds the inner-edge growth
+rate scatters by about +-10% at fixed parameters, which no surrogate of the
+parameters can learn.  Each run builds its equilibrium with this library's
 GPU gravity, and advances ~20 outer orbits (t = 20 * 2 pi / sqrt(G (M* + M_d))
 at R = 1) in one batched handle with every other run of the shard.  The
 extractor (dfk_extract_growth) turns each run's (t, |C0|, |C2|, arg C2)
@@ -14,8 +18,11 @@ edge (gamma ~ 10, Omega_p ~ 14, within t < 2) and then wanders nonlinearly.
 The targets are therefore: gamma and Omega_p of that first linear growth,
 fitted while |C2|/|C0| rises from 10x to 100x its initial value
 (dfk_extract_growth_ex, hi_factor = 100), from samples every dtout = 0.01;
-sat = mean |C2|/|C0| over the second half of the run.  The stored series keep
-the first 400 samples and every 25th after.
+sat = the amplitude at which that growth saturates, the largest |C2|/|C0|
+within 2 time units of the window start (sat_peak).  The mean |C2|/|C0| over
+the second half of the run (sat_late) is stored too; it follows the
+turbulent late state and scatters by ~20% between neighbouring parameter
+points.  The stored series keep the first 600 samples and every 25th after.
 
     python heavy-disks_amd/surrogate_data.py --first 0 --count 1024 --out shard0.npz
 """
@@ -42,8 +49,11 @@ def tlim_for(d):
 
 def run_shard(first, count, Nu=96, Nphi=96, seed=2026, total=4352, chunk=2000,
               max_wall=None, verbose=True, dtout=0.01, lo_factor=10.0, hi_factor=100.0,
-              sat_frac=0.5, keep_first=400, keep_every=25):
+              sat_frac=0.5, peak_window=2.0, keep_first=600, keep_every=25, noise_seed=42):
     disks = ensemble_cfg.draw(total, seed=seed, grids=[(Nu, Nphi)])[first:first + count]
+    if noise_seed is not None:
+        for d in disks:
+            d["seed"] = noise_seed
     t0 = time.perf_counter()
     # one handle; gravity of the axisymmetric disks first (K does not enter it)
     params = [dfk.disk_params(Nu, Nphi, Rin=0.1, Rout=1.0, K=1.0, gmstar=1.0, eps=d["eps"])
@@ -86,7 +96,7 @@ def run_shard(first, count, Nu=96, Nphi=96, seed=2026, total=4352, chunk=2000,
     steps = np.array([ens.status(k)["steps"] for k in range(len(disks))])
     flags = np.array([ens.status(k)["flags"] for k in range(len(disks))])
     tend = np.array([ens.status(k)["t"] for k in range(len(disks))])
-    X, Y, ok = [], [], []
+    X, Y, late, ok = [], [], [], []
     nmax = max(sum(len(x) for x in s) for s in series)
     nkeep = keep_first + max(0, nmax - keep_first + keep_every - 1) // keep_every
     ser = np.full((len(disks), nkeep, 4), np.nan, dtype=np.float32)
@@ -96,13 +106,16 @@ def run_shard(first, count, Nu=96, Nphi=96, seed=2026, total=4352, chunk=2000,
         ser[k, :len(sk)] = sk
         X.append([d["mass_ratio"], d["Qmin"], np.sqrt(d["eps"]), d["slope"]])
         good = len(s) >= 8 and flags[k] == 0 and tend[k] >= tl[k] * (1 - 1e-9)
-        g = (dfk.extract_growth(s, lo_factor=lo_factor, hi_factor=hi_factor, sat_frac=sat_frac)
-             if len(s) >= 4 else dict(gamma=np.nan, omega_p=np.nan, sat=np.nan))
-        Y.append([g["gamma"], g["omega_p"], g["sat"]])
+        g = (dfk.extract_growth(s, lo_factor=lo_factor, hi_factor=hi_factor, sat_frac=sat_frac,
+                                peak_window=peak_window)
+             if len(s) >= 4 else dict(gamma=np.nan, omega_p=np.nan, sat=np.nan, sat_peak=np.nan))
+        Y.append([g["gamma"], g["omega_p"], g["sat_peak"]])
+        late.append(g["sat"])
         ok.append(good)
     zones = float(Nu * Nphi) * steps.sum()
     ens.close()
-    return dict(X=np.array(X), Y=np.array(Y), ok=np.array(ok), K=np.array(Ks),
+    return dict(X=np.array(X), Y=np.array(Y), sat_late=np.array(late), ok=np.array(ok),
+                K=np.array(Ks),
                 Qgot=np.array(Qgot), steps=steps, flags=flags, tlim=np.array(tl),
                 series=ser, wall=wall, setup=t_setup, zone_updates=zones,
                 first=first, grid=np.array([Nu, Nphi]))
@@ -116,9 +129,12 @@ def main():
     ap.add_argument("--nu", type=int, default=96)
     ap.add_argument("--nphi", type=int, default=96)
     ap.add_argument("--max-wall", type=float, default=None)
+    ap.add_argument("--noise-seed", type=int, default=42,
+                    help="noise realization of every run (negative: each run's own seed)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
-    r = run_shard(a.first, a.count, a.nu, a.nphi, total=a.total, max_wall=a.max_wall)
+    r = run_shard(a.first, a.count, a.nu, a.nphi, total=a.total, max_wall=a.max_wall,
+                  noise_seed=None if a.noise_seed < 0 else a.noise_seed)
     np.savez_compressed(a.out, **r)
     print(f"shard {a.first}+{a.count}: {r['ok'].sum()} good runs, {r['zone_updates']:.3e} "
           f"zone-updates in {r['wall']:.1f}s ({r['zone_updates'] / r['wall']:.3e}/s), setup "

@@ -1,7 +1,8 @@
 """Train the surrogate ensemble on hydro shards, judge it on held-out hydro runs,
 scan 10^5 parameter points, and compare with surrogate-free baselines.
 
-    python heavy-disks_amd/surrogate_fit.py shard*.npz --out surrogate_eval.json
+    python heavy-disks_amd/surrogate_fit.py shard*.npz --out surrogate_eval.json \
+        [--export tests/golden/surrogate_runs.npz]
 
 Runs 0 .. n_train-1 of the shards train the ensemble, runs n_train .. end are
 held out.  Error of a prediction p of a hydro value h for output k:
@@ -29,6 +30,7 @@ NAMES = ["gamma", "omega_p", "sat"]
 
 
 def load(paths):
+    """(X, Y, ok) of shard files in run order, or of one exported table."""
     Xs, Ys, oks = [], [], []
     for p in sorted(paths, key=lambda q: int(np.load(q)["first"])):
         d = np.load(p)
@@ -36,6 +38,13 @@ def load(paths):
         Ys.append(d["Y"])
         oks.append(d["ok"])
     return np.concatenate(Xs), np.concatenate(Ys), np.concatenate(oks)
+
+
+def export(paths, out):
+    """The runs' parameters and targets only (a few hundred KB): the table the
+    surrogate tests train on without regenerating the hydro runs."""
+    X, Y, ok = load(paths)
+    np.savez_compressed(out, X=X, Y=Y, ok=ok, first=0)
 
 
 def errors(pred, true, scale):
@@ -73,7 +82,10 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--scan", type=int, default=100000)
+    ap.add_argument("--export", default=None, help="also write the (X, Y, ok) table here")
     a = ap.parse_args()
+    if a.export:
+        export(a.shards, a.export)
     X, Y, ok = load(a.shards)
     tr = np.arange(len(X)) < a.n_train
     te = ~tr

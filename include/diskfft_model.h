@@ -80,19 +80,23 @@ typedef struct {
   double t_lo, t_hi;         /* fit window */
   int n_fit;                 /* samples in the window */
   int grew;
+  double sat_peak;           /* dfk_extract_growth_ex only: max a over [t_lo, t_lo + peak_window] */
 } dfk_growth;
 int dfk_extract_growth(const double *t, const double *c0, const double *c2, const double *ph2,
                        int n, double lo_factor, double hi_frac, double grow_factor,
                        dfk_growth *out);
 
-/* The same with two more knobs: the window also ends no later than the first
+/* The same with more knobs: the window also ends no later than the first
  * sample with a >= hi_factor * a(0) (so a later, larger peak cannot stretch
- * the window of the first linear growth across a plateau), and sat is the
- * mean of a over the last sat_frac of the samples (0 < sat_frac <= 1).
+ * the window of the first linear growth across a plateau); sat is the mean of
+ * a over the last sat_frac of the samples (0 < sat_frac <= 1); sat_peak is
+ * the largest a in [t_lo, t_lo + peak_window], the amplitude at which that
+ * first growth saturates (whole series if the mode never grew).
  * dfk_extract_growth is hi_factor = +inf, sat_frac = 0.25. */
 int dfk_extract_growth_ex(const double *t, const double *c0, const double *c2, const double *ph2,
                           int n, double lo_factor, double hi_factor, double hi_frac,
-                          double grow_factor, double sat_frac, dfk_growth *out);
+                          double grow_factor, double sat_frac, double peak_window,
+                          dfk_growth *out);
 
 /* SplitMix64 stream: next 64-bit value (state updated), and a standard normal
  * by Box-Muller (one value per call; the pair's sine half is discarded). */

@@ -68,8 +68,12 @@ def test_window_cap_keeps_first_growth(dfk):
     la = np.where(t > 30, la + 2.5 * (t - 30) / 10.0, la)           # late rise to ~0.24
     ph = np.angle(np.exp(1j * (0.4 - 2 * 13.5 * t)))
     s = np.stack([t, np.ones_like(t), np.exp(la), ph], axis=1)
-    g = dfk.extract_growth(s, lo_factor=10.0, hi_factor=100.0, hi_frac=0.3, sat_frac=0.5)
+    g = dfk.extract_growth(s, lo_factor=10.0, hi_factor=100.0, hi_frac=0.3, sat_frac=0.5,
+                           peak_window=2.0)
     assert g["grew"] and g["t_hi"] < 0.8
+    # the first saturation: largest a within 2 time units of the window start
+    w = (t >= g["t_lo"]) & (t <= g["t_lo"] + 2.0)
+    assert g["sat_peak"] == np.max(np.exp(la)[w]) and abs(g["sat_peak"] - 2e-2) < 1e-15
     assert abs(g["gamma"] - 9.0) < 1e-9
     assert abs(g["omega_p"] - 13.5) < 1e-9
     assert abs(g["sat"] - np.mean(np.exp(la[len(t) - len(t) // 2:]))) < 1e-15
@@ -81,6 +85,6 @@ def test_ex_defaults_reproduce_extractor(dfk):
     s = _series(0.72, 1.57)
     a = dfk.extract_growth(s, lo_factor=3.0, hi_frac=0.05)
     b = dfk.extract_growth(s, lo_factor=3.0, hi_frac=0.05, hi_factor=np.inf, sat_frac=0.25 + 1e-17)
-    assert a == b
+    assert b.pop("sat_peak") == np.max(s[:, 2]) and a == b
     with pytest.raises(ValueError):
         dfk.extract_growth(s, sat_frac=0.0)
