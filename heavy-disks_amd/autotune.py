@@ -31,8 +31,7 @@ def candidates(Nu, Nphi):
              if tj <= Nphi and ti <= Nu
              and 8 * (3 * (ti + 4) * (tj + 4) + 3 * (ti + 1) * (tj + 4)) <= LDS_MAX]
     prs = [p for p in (1, 2, 4, 8, 16) if (2 * p + 1) * Nphi * 16 <= LDS_MAX and 2 * p <= Nu]
-    cbs = [c for c in (1, 2, 4, 8, 16)
-           if (2 * c + 1) * M * 16 + c * (Nu + 1) * 8 <= LDS_MAX and c <= Nphi // 2 + 1]
+    cbs = [c for c in (1, 2, 4, 8, 16) if (2 * c + 1) * M * 16 <= LDS_MAX and c <= Nphi // 2 + 1]
     return tiles, prs, cbs
 
 

@@ -183,7 +183,7 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
 // Stage of any other prime radix R: direct R-point DFT reading its inputs
 // from LDS (few registers; used only for lengths with factors 5, 7, 11, 13).
 template <int SIGN>
-__device__ void fft_stage_generic(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
+__device__ __forceinline__ void fft_stage_generic(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
                                   int batch, int Ns, int R, const double2 *__restrict__ tw) {
   const int nb = n / R;
   const int tstep = n / (Ns * R);
@@ -221,9 +221,11 @@ __device__ __forceinline__ void stage_twiddles(double2 *__restrict__ dst,
 }
 
 // Full transform of `batch` rows of length n in LDS: returns the buffer that
-// holds the result (a or b).  One barrier per stage.
+// holds the result (a or b).  One barrier per stage.  Inlined (as are the
+// stage helpers): as a called function the call convention alone took the
+// FFT kernels from ~70 to 162 VGPRs (3 instead of 6-7 waves per SIMD).
 template <int SIGN>
-__device__ double2 *lds_fft(double2 *a, double2 *b, int n, int batch, int nst,
+__device__ __forceinline__ double2 *lds_fft(double2 *a, double2 *b, int n, int batch, int nst,
                             const int *__restrict__ rad, const double2 *__restrict__ tw) {
   int Ns = 1;
   for (int s = 0; s < nst; ++s) {
@@ -260,7 +262,7 @@ enum { IO_LDS = 0, IO_ZP = 1, IO_KHAT = 2, IO_OUT = 3, IO_ROUT = 4 };
 
 struct StageIO {
   double2 *spec;          // IO_ZP / IO_OUT: the disk's half spectrum [Nc][Nu]
-  const double *kh;       // IO_KHAT: K-hat rows 0..Nu of the batch's columns (LDS)
+  const double *kh;       // IO_KHAT: the disk's real K-hat [Nc][Nu + 1]
   const double *rsc;      // IO_ROUT: pref / sqrt(R) per row
   double *P;              // IO_ROUT: Phi of the disk
   int Nu, Nc, k0, row0;   // k0: first column of the batch; row0: first row
@@ -304,7 +306,7 @@ __device__ __forceinline__ void io_stage(const double2 *__restrict__ a, double2 
     for (int q = 0; q < R; ++q) {
       const int o = jq * Ns * R + k + q * Ns;
       if constexpr (IO == IO_KHAT) {
-        const double kv = g.kh[p * (g.Nu + 1) + (o <= g.Nu ? o : n - o)];
+        const double kv = live ? g.kh[(i64)col * (g.Nu + 1) + (o <= g.Nu ? o : n - o)] : 0.0;
         double2 z = v[q];
         z.x *= kv;
         z.y *= kv;
@@ -340,7 +342,7 @@ __device__ __forceinline__ bool io_fast(int nst, const int *rad) {
 // Middle stages 1 .. last-1 (LDS -> LDS) of a transform whose first stage
 // wrote x; returns the buffer holding the result of stage `last - 1`.
 template <int SIGN>
-__device__ double2 *mid_stages(double2 *x, double2 *y, int n, int batch, int nst,
+__device__ __forceinline__ double2 *mid_stages(double2 *x, double2 *y, int n, int batch, int nst,
                                const int *rad, const double2 *tw, const StageIO &g, int *Ns) {
   for (int q = 1; q < nst - 1; ++q) {
     io_stage_r<SIGN, IO_LDS>(rad[q], x, y, n, batch, *Ns, tw, g);
@@ -586,15 +588,8 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   const double *kh = khat + D.kho;
   if (io_fast(D.nst_col, D.rad_col)) {
     const int ns = D.nst_col;
-    // the batch's K-hat columns go to LDS beside the twiddles (loads in
-    // flight during the first stage; zero for columns past Nc)
-    double *KH = reinterpret_cast<double *>(TW + M);
-    for (int e = threadIdx.x; e < cb * (Nu + 1); e += blockDim.x) {
-      const int c = e / (Nu + 1), p = e - c * (Nu + 1);
-      KH[e] = (k0 + c < Nc) ? kh[(i64)(k0 + c) * (Nu + 1) + p] : 0.0;
-    }
     StageIO g{};
-    g.spec = io; g.kh = KH; g.Nu = Nu; g.Nc = Nc; g.k0 = k0;
+    g.spec = io; g.kh = kh; g.Nu = Nu; g.Nc = Nc; g.k0 = k0;
     // forward: spectrum column -> A (first stage), ..., last stage x K-hat
     io_stage_r<-1, IO_ZP>(D.rad_col[0], nullptr, A, M, cb, 1, TW, g);
     __syncthreads();

@@ -188,9 +188,7 @@ int build_layout(dfk_handle *h) {
       // batches shrink until they fit the LDS budget (a cached shape from a
       // build with smaller LDS needs stays usable)
       auto lds_rows = [&](int pr) { return (size_t)(2 * pr + 1) * D.Nphi * sizeof(double2); };
-      auto lds_cols = [&](int cb) {   // A, B, twiddles + K-hat columns
-        return (size_t)(2 * cb + 1) * D.M * sizeof(double2) + (size_t)cb * (D.Nu + 1) * sizeof(double);
-      };
+      auto lds_cols = [&](int cb) { return (size_t)(2 * cb + 1) * D.M * sizeof(double2); };
       while (D.pr > 1 && lds_rows(D.pr) > kMaxLds) --D.pr;
       while (D.cb > 1 && lds_cols(D.cb) > kMaxLds) --D.cb;
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));

@@ -63,8 +63,8 @@ def per_kernel(d, counter):
 
 def traffic(out, workload, fdir, wdir, cfdir, cwdir, copy_mib):
     kib = float(copy_mib) * 1024.0
-    cf = per_kernel(cfdir, "FETCH_SIZE")["copy_kernel"]
-    cw = per_kernel(cwdir, "WRITE_SIZE")["copy_kernel"]
+    cf = next(v for k, v in per_kernel(cfdir, "FETCH_SIZE").items() if k.startswith("copy_kernel"))
+    cw = next(v for k, v in per_kernel(cwdir, "WRITE_SIZE").items() if k.startswith("copy_kernel"))
     ff, wf = kib / cf, kib / cw
     fk, wk = per_kernel(fdir, "FETCH_SIZE"), per_kernel(wdir, "WRITE_SIZE")
     res = {"workload": workload, "fetch_factor": ff, "write_factor": wf, "kernels": {}}

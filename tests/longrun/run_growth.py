@@ -39,7 +39,7 @@ def case(name):
             }[name]
 
 
-def run_case(name):
+def run_case(name, cache_dir=None, skip_gpu=False):
     c = case(name)
     Nu, Nphi = c["N"]
     kind = dfk.MODEL_LKA98 if c["model"] == "lka98" else dfk.MODEL_POWERLAW
@@ -48,23 +48,41 @@ def run_case(name):
     # the oracle's gravity sets up the equilibrium; both sides start from the same arrays
     p, S, U, W, mD, _ = dfk.model_ic(m, Nu, Nphi, c["eps"],
                                      potential_forces=lambda x: k.forces(k.potential(x)))
-    e = dfk.Ensemble(p)
-    e.set_state(S, U, W)
+    if skip_gpu:
+        e = None
+    else:
+        e = dfk.Ensemble(p)
+        e.set_state(S, U, W)
     t0 = time.perf_counter()
     dtout = c.get("dtout", 0.25)
-    e.set_graph_steps(500)
-    ((sg, stg),) = e.run(c["tlim"], dtout=dtout, cfl=c["cfl"], mD=mD, max_steps=c["steps"],
-                         chunk=20000)
-    print(f"  {name} gpu: {stg['steps']} steps to t = {stg['t']:.2f}", flush=True)
+    if not skip_gpu:
+        e.set_graph_steps(500)
+        ((sg, stg),) = e.run(c["tlim"], dtout=dtout, cfl=c["cfl"], mD=mD, max_steps=c["steps"],
+                             chunk=20000)
+        print(f"  {name} gpu: {stg['steps']} steps to t = {stg['t']:.2f}", flush=True)
     tg = time.perf_counter() - t0
-    h = oracle.Hydro(Nu, Nphi, p["Rin"], p["Rout"], p["K"], p["gamma"], p["gmstar"], fast=True)
-    h.set_state(S, U, W)
-    t0 = time.perf_counter()
-    so, tf, c2f, steps = oracle.run_chunked(
-        h, k, 1, c["tlim"], dtout=dtout, cfl=c["cfl"], max_steps=c["steps"], mD=mD, chunk=20000,
-        progress=lambda t, n: print(f"  {name} oracle: t = {t:.2f} after {n} steps "
-                                    f"({time.perf_counter() - t0:.0f} s)", flush=True))
-    to = time.perf_counter() - t0
+    cache = os.path.join(cache_dir, f"oracle_{name}.npz") if cache_dir else None
+    if cache and os.path.exists(cache):
+        # the oracle's run of this case from an earlier invocation (same code,
+        # same initial state; any host)
+        z = np.load(cache)
+        so, tf, c2f, steps, Sig_o = z["samples"], float(z["t"]), float(z["c2f"]), int(z["steps"]), z["Sig"]
+        to, nthr = float(z["wall"]), int(z["threads"])
+    else:
+        h = oracle.Hydro(Nu, Nphi, p["Rin"], p["Rout"], p["K"], p["gamma"], p["gmstar"], fast=True)
+        h.set_state(S, U, W)
+        t0 = time.perf_counter()
+        so, tf, c2f, steps = oracle.run_chunked(
+            h, k, 1, c["tlim"], dtout=dtout, cfl=c["cfl"], max_steps=c["steps"], mD=mD,
+            chunk=20000,
+            progress=lambda t, n: print(f"  {name} oracle: t = {t:.2f} after {n} steps "
+                                        f"({time.perf_counter() - t0:.0f} s)", flush=True))
+        to, nthr, Sig_o = time.perf_counter() - t0, oracle.lib(True).orc_get_threads(), h.Sig.copy()
+        if cache:
+            np.savez_compressed(cache, samples=so, t=tf, c2f=c2f, steps=steps, Sig=Sig_o, wall=to,
+                                threads=nthr)
+    if skip_gpu:
+        return dict(case=name, oracle_only=True, steps_oracle=steps, t_oracle=tf)
     ex = c.get("ex", {})
     gg, go = dfk.extract_growth(sg, **ex), dfk.extract_growth(so, **ex)
     rel = {q: abs(gg[q] - go[q]) / abs(go[q]) if go[q] else None
@@ -74,8 +92,8 @@ def run_case(name):
                steps_gpu=stg["steps"], steps_oracle=steps, t_gpu=stg["t"], t_oracle=tf,
                c2_final_gpu=stg["c2_final"], c2_final_oracle=c2f,
                mass_drift_gpu=float(sg[-1, 1] - 1.0), mass_drift_oracle=float(so[-1, 1] - 1.0),
-               final_sigma_maxrel=float(np.max(np.abs(Sg - h.Sig)) / np.max(np.abs(h.Sig))),
-               wall_gpu_s=tg, wall_oracle_s=to, oracle_threads=oracle.lib(True).orc_get_threads(),
+               final_sigma_maxrel=float(np.max(np.abs(Sg - Sig_o)) / np.max(np.abs(Sig_o))),
+               wall_gpu_s=tg, wall_oracle_s=to, oracle_threads=nthr,
                samples_gpu=sg.tolist(), samples_oracle=so.tolist())
     print(f"{name}: gamma gpu {gg['gamma']:.5f} oracle {go['gamma']:.5f} | Omega_p gpu "
           f"{gg['omega_p']:.5f} oracle {go['omega_p']:.5f} | sat gpu {gg['sat']:.4f} oracle "
@@ -87,11 +105,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cases", nargs="+")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--oracle-cache", default=None,
+                    help="directory holding (or receiving) the oracle's runs of the cases")
+    ap.add_argument("--oracle-only", action="store_true",
+                    help="run (and cache) the oracle side only; no GPU needed")
     args = ap.parse_args()
     oracle.set_threads(max(1, min(16, os.cpu_count() or 1)))
     res = []
     for name in args.cases:
-        res.append(run_case(name))
+        res.append(run_case(name, args.oracle_cache, args.oracle_only))
         with open(args.out, "w") as f:
             json.dump(res, f)
 

@@ -156,7 +156,8 @@ def _lka98(orc, Nu, Nphi, eps, amp=1e-3, selfgrav=1):
 
 @pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("Nu,Nphi,eps", [(64, 64, 0.02), (128, 128, 0.02), (96, 80, 0.01),
-                                         (256, 256, 0.005), (48, 36, 0.02)])
+                                         (256, 256, 0.005), (48, 36, 0.02), (512, 1024, 0.01),
+                                         (3, 4, 0.02)])
 def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps, kind):
     """The north-star gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative."""
     print(f"case 10-step loop {Nu}x{Nphi} eps={eps} update kernel {kind}", flush=True)
