@@ -7,8 +7,11 @@ The knobs (see dfk_set_shapes in include/diskfft.h) act on separate kernels:
 so each is tuned on its own kernel's time, measured with HIP events on the
 library's stream (dfk_profile_step).  Two regimes:
   "latency"     one disk alone (few workgroups; launch/latency bound)
-  "throughput"  enough copies of the disk to hold >= 2M zones (bandwidth bound),
-                which is what a disk sees inside a large batched ensemble.
+  "throughput"  enough copies of the disk to hold >= 8M zones (bandwidth bound),
+                which is what a disk sees inside a large batched ensemble (the
+                64-disk ensemble has 7.7M; with only ~2M the tuner preferred
+                short update strips for their extra waves, which re-read more
+                halo rows once the GPU is full anyway).
 Results are cached in a JSON file keyed by (grid, regime, device).
 
     python heavy-disks_amd/autotune.py 256x256 512x512 --regime throughput
@@ -96,7 +99,7 @@ def _handle(dfk, Nu, Nphi, copies):
 
 
 def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
-    copies = 1 if regime == "latency" else max(1, (2 << 20) // (Nu * Nphi))
+    copies = 1 if regime == "latency" else max(1, (8 << 20) // (Nu * Nphi))
     e, mD = _handle(dfk, Nu, Nphi, copies)
     tiles, prs, cbs = candidates(Nu, Nphi)
     base = e.get_shapes(0)
