@@ -25,7 +25,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned_shapes.json")
-LDS_MAX = 64 * 1024
+LDS_MAX = 160 * 1024    # gfx950: LDS per workgroup (the library checks the device's value)
 
 
 def candidates(Nu, Nphi):

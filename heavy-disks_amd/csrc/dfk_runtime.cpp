@@ -113,6 +113,7 @@ struct dfk_handle {
   int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr;
   std::vector<int> upd0, r2c0, conv0;   // per-disk first block (size nd+1)
   size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0;
+  size_t lds_max = 64 * 1024;   // per workgroup, from the device at create (gfx950: 160 KiB)
   DiskCtl *ctl = nullptr;
   // run
   bool run_ready = false;
@@ -165,7 +166,7 @@ int build_layout(dfk_handle *h) {
   h->conv0.assign(h->nd + 1, 0);
   i64 part = 0;
   size_t lu = 0, lr = 0, lc = 0;
-  const size_t kMaxLds = 64 * 1024;
+  const size_t kMaxLds = h->lds_max;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
     if (h->upd_kind == 1) D.tj = kStreamCols;   // one wave = 60 output columns
@@ -543,6 +544,10 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
   h->hsamples.resize(ndisks);
   auto fail = [&](int rc) { dfk_destroy(h); return rc; };
   if (hipSetDevice(device) != hipSuccess) { set_err("hipSetDevice failed"); return fail(DFK_EHIP); }
+  int lmax = 0;
+  if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+      lmax > 0)
+    h->lds_max = (size_t)lmax;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     set_err("stream create failed");
     return fail(DFK_EHIP);

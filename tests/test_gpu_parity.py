@@ -74,6 +74,20 @@ def test_potential_matches_oracle(orc, dfk, Nu, Nphi, eps):
         assert relerr(pg, k.potential_direct(Sig)) < 1e-12
 
 
+@pytest.mark.parametrize("Nu,Nphi,pr,cb", [(256, 256, 8, 8), (512, 1024, 4, 2), (96, 80, 16, 16)])
+def test_potential_large_batches(orc, dfk, Nu, Nphi, pr, cb):
+    """Row / column batches whose LDS use exceeds 64 KiB (gfx950 allows 160 KiB
+    per workgroup): pr = 8 rows of 256 use 68 KiB, cb = 8 columns of 512 use
+    136 KiB; 512 x 1024 uses 144 KiB (rows) and 80 KiB (columns)."""
+    print(f"case potential {Nu}x{Nphi} pr={pr} cb={cb}", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    e.set_shapes(0, 0, pr, cb)
+    assert tuple(e.get_shapes(0))[2:] == (min(pr, (Nu + 1) // 2), min(cb, Nphi // 2 + 1))
+    rng = np.random.default_rng(7)
+    Sig = rng.random((Nu, Nphi))
+    assert relerr(e.potential(Sig), k.potential(Sig)) < 1e-13
+
+
 @pytest.mark.parametrize("Nu,Nphi", [(16, 16), (35, 40), (128, 128), (33, 26)])
 def test_forces_match_oracle(orc, dfk, Nu, Nphi):
     print(f"case forces {Nu}x{Nphi}", flush=True)
