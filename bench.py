@@ -224,6 +224,8 @@ def main():
         threads = max(1, min(16, os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(r["p"], r["S"], r["U"], r["W"], args.cfl,
                                            args.cpu_seconds, threads)
+        one = cpu_baseline(r["p"], r["S"], r["U"], r["W"], args.cfl, args.cpu_seconds / 2, 1)
+        out["cpu_baseline"]["single_thread"] = {"value": one["value"], "sample": one["sample"]}
     if not args.no_ensemble:
         try:
             out["ensemble"] = ensemble_legs(dfk, args.ensemble_steps, 100)

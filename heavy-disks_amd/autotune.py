@@ -76,6 +76,7 @@ def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
     for d, c in enumerate(found):
         if c:
             ens.set_shapes(c["ti"], c["tj"], c["pr"], c["cb"], d=d)
+            ens.set_twiddle_lds(*c.get("tw_lds", (1, 1)), d=d)
             if "threads" in c:
                 t = tuple(c["threads"])
                 votes[t] = votes.get(t, 0) + ens.shape(d)[0] * ens.shape(d)[1]
@@ -118,15 +119,22 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
         if ts <= Nu:
             res["stream"][str(ts)] = float(prof(ts, 0, base[2], base[3])[4])
     e.set_update_kernel(0)
+    # row batches with the twiddle table in LDS; column batches with it in LDS
+    # (1) or read through the caches (0).  A launch's LDS is the largest any
+    # of its disks needs, so the knob only pays where the big transforms use
+    # it; for the rows it lost in mixed ensembles and is not tuned.
     for pr in prs:
         ms = prof(base[0], base[1], pr, base[3])
         res["pr"][str(pr)] = float(ms[1] + ms[3])
-    for cb in cbs:
-        res["cb"][str(cb)] = float(prof(base[0], base[1], base[2], cb)[2])
+    for twl in (1, 0):
+        e.set_twiddle_lds(1, twl)
+        for cb in cbs:
+            res["cb"][f"{cb}:{twl}"] = float(prof(base[0], base[1], base[2], cb)[2])
     bt = min(res["tile"], key=res["tile"].get)
     bst = min(res["stream"], key=res["stream"].get) if res["stream"] else None
-    bpr = int(min(res["pr"], key=res["pr"].get))
-    bcb = int(min(res["cb"], key=res["cb"].get))
+    bpr, twr = int(min(res["pr"], key=res["pr"].get)), 1
+    bcb, twc = (int(x) for x in min(res["cb"], key=res["cb"].get).split(":"))
+    e.set_twiddle_lds(twr, twc)
     ti, tj = (int(x) for x in bt.split("x"))
     kind = 0
     if bst is not None and res["stream"][bst] < res["tile"][bt]:
@@ -143,13 +151,14 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     best = prof(ti, tj, bpr, bcb)
     e.close()
     out = {"kind": kind, "ti": ti, "tj": tj, "pr": bpr, "cb": bcb, "threads": list(thr),
+           "tw_lds": [twr, twc],
            "copies": copies,
            "ms_per_kernel": [float(x) for x in best],
            "default": {"shape": list(base), "tile_ms": res["tile"].get(f"{base[0]}x{base[1]}")},
            "sweep": res}
     if verbose:
         print(f"{Nu}x{Nphi} {regime} x{copies}: kernel {kind} tile {ti}x{tj} pr {bpr} cb {bcb} "
-              f"threads {thr} "
+              f"twiddles in LDS (rows, cols) {(twr, twc)} threads {thr} "
               f"ms {np.round(best, 4).tolist()}", flush=True)
     return out
 

@@ -62,7 +62,8 @@ struct DiskDev {
   int pr;              // complex rows (= 2 real rows) per workgroup in row transforms
   int cb;              // columns per workgroup in the column convolution
   int cos_off;         // offset of cos(2 phi_j)/sin(2 phi_j) tables (Nphi each) in trig pool
-  int pad_;
+  int tw_lds;          // bit 0: row transforms, bit 1: column convolution stage their
+                       // twiddles in LDS (else read them through the caches)
 };
 
 // Per-disk loop control, device resident (diskmain.cpp:86-113 state).

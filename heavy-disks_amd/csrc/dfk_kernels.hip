@@ -220,6 +220,17 @@ __device__ __forceinline__ void stage_twiddles(double2 *__restrict__ dst,
   for (int e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
 }
 
+// The twiddle table a transform reads: staged in LDS at `lds` (visible after
+// the next barrier; no transform's first stage uses twiddles) or, to keep
+// the workgroup's LDS and so its occupancy smaller, the HBM table itself
+// read through L1/L2 (a per-disk autotuner knob).
+__device__ __forceinline__ const double2 *tw_table(double2 *lds, const double2 *__restrict__ src,
+                                                   int n, int in_lds) {
+  if (!in_lds) return src;
+  stage_twiddles(lds, src, n);
+  return lds;
+}
+
 // Full transform of `batch` rows of length n in LDS: returns the buffer that
 // holds the result (a or b).  One barrier per stage.  Inlined (as are the
 // stage helpers): as a called function the call convention alone took the
@@ -537,8 +548,8 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   if (C.done) return;
   const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * wm.y;
-  double2 *A = lds2, *B = lds2 + pr * n, *TW = lds2 + 2 * pr * n;
-  stage_twiddles(TW, tw + D.two_row, n);
+  double2 *A = lds2, *B = lds2 + pr * n;
+  const double2 *TW = tw_table(lds2 + 2 * pr * n, tw + D.two_row, n, D.tw_lds & 1);
   const double *S = stp(st, ntot, par, 0, D);
   const double *R32 = gtab(geom, D, GT_R32);
   // all threads stage the rows (more loads in flight than a first stage
@@ -582,8 +593,8 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   if (ctl[wm.x].done) return;
   const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
   const int k0 = wm.y * cb;
-  double2 *A = lds2, *B = lds2 + cb * M, *TW = lds2 + 2 * cb * M;
-  stage_twiddles(TW, tw + D.two_col, M);
+  double2 *A = lds2, *B = lds2 + cb * M;
+  const double2 *TW = tw_table(lds2 + 2 * cb * M, tw + D.two_col, M, D.tw_lds & 2);
   double2 *io = spec + D.spo;
   const double *kh = khat + D.kho;
   if (io_fast(D.nst_col, D.rad_col)) {
@@ -653,8 +664,8 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   if (ctl[wm.x].done) return;
   const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * wm.y;
-  double2 *A = lds2, *B = lds2 + pr * n, *TW = lds2 + 2 * pr * n;
-  stage_twiddles(TW, tw + D.two_row, n);
+  double2 *A = lds2, *B = lds2 + pr * n;
+  const double2 *TW = tw_table(lds2 + 2 * pr * n, tw + D.two_row, n, D.tw_lds & 1);
   const double2 *in = spec + D.spo;
   for (int e = threadIdx.x; e < pr * Nc; e += blockDim.x) {
     const int k = e / pr, p = e - k * pr;
@@ -704,8 +715,8 @@ k_khat_rows(const DiskDev *__restrict__ D_, const double *__restrict__ ktab,
   const DiskDev &D = *D_;
   const int n = D.Nphi, M = D.M, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * blockIdx.x;
-  double2 *A = lds2, *B = lds2 + pr * n, *TW = lds2 + 2 * pr * n;
-  stage_twiddles(TW, tw + D.two_row, n);
+  double2 *A = lds2, *B = lds2 + pr * n;
+  const double2 *TW = tw_table(lds2 + 2 * pr * n, tw + D.two_row, n, D.tw_lds & 1);
   for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
     const int p = e / n, j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;

@@ -188,8 +188,13 @@ int build_layout(dfk_handle *h) {
     if (D.selfgrav) {
       // batches shrink until they fit the LDS budget (a cached shape from a
       // build with smaller LDS needs stays usable)
-      auto lds_rows = [&](int pr) { return (size_t)(2 * pr + 1) * D.Nphi * sizeof(double2); };
-      auto lds_cols = [&](int cb) { return (size_t)(2 * cb + 1) * D.M * sizeof(double2); };
+      // ping-pong buffers (+ the twiddle table where staged in LDS)
+      auto lds_rows = [&](int pr) {
+        return (size_t)(2 * pr + (D.tw_lds & 1)) * D.Nphi * sizeof(double2);
+      };
+      auto lds_cols = [&](int cb) {
+        return (size_t)(2 * cb + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
+      };
       while (D.pr > 1 && lds_rows(D.pr) > kMaxLds) --D.pr;
       while (D.cb > 1 && lds_cols(D.cb) > kMaxLds) --D.cb;
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
@@ -601,6 +606,7 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
   const bool small = h->ntot < (1LL << 19);
   for (int d = 0; d < ndisks; ++d) {
     DiskDev &D = h->hD[d];
+    D.tw_lds = 3;   // twiddles of row and column transforms staged in LDS
     if (small) {
       D.ti = 8; D.tj = 16; D.pr = 1; D.cb = 1;
     } else {
@@ -1025,6 +1031,24 @@ int dfk_set_shapes(dfk_handle *h, int d, int ti, int tj, int pr, int cb) {
   if (cb > 0) D.cb = cb;
   h->layout_ok = false;
   return build_layout(h);
+}
+
+int dfk_set_twiddle_lds(dfk_handle *h, int d, int rows, int cols) {
+  TRY(check_disk(h, d));
+  DiskDev &D = h->hD[d];
+  const int v = (rows ? 1 : 0) | (cols ? 2 : 0);
+  if (v != D.tw_lds) {
+    D.tw_lds = v;
+    h->layout_ok = false;
+  }
+  return build_layout(h);
+}
+
+int dfk_get_twiddle_lds(dfk_handle *h, int d, int *rows, int *cols) {
+  TRY(check_disk(h, d));
+  if (rows) *rows = h->hD[d].tw_lds & 1;
+  if (cols) *cols = (h->hD[d].tw_lds >> 1) & 1;
+  return DFK_OK;
 }
 
 int dfk_get_shapes(dfk_handle *h, int d, int *ti, int *tj, int *pr, int *cb) {

@@ -3,7 +3,12 @@
     python heavy-disks_amd/prof_summary.py stats DIR            # --kernel-trace --stats run
     python heavy-disks_amd/prof_summary.py pmc DIR [DIR ...]    # --pmc runs (FETCH_SIZE, WRITE_SIZE, ...)
     python heavy-disks_amd/prof_summary.py traffic OUT.json WORKLOAD FETCH_DIR WRITE_DIR \
-        COPY_FETCH_DIR COPY_WRITE_DIR COPY_MIB
+        COPY_FETCH_DIR COPY_WRITE_DIR COPY_MIB [--largest]
+    python heavy-disks_amd/prof_summary.py largest DIR          # --kernel-trace run
+
+`largest` (and `traffic ... --largest`) keeps, per kernel, only the
+dispatches with the largest grid: the loop's launches over all disks, not
+the single-disk calls made while building initial conditions.
 
 `stats` prints calls and average/min/max duration per kernel from
 DIR/*kernel_stats.csv; `pmc` prints the per-dispatch average of every counter
@@ -52,21 +57,43 @@ def pmc(dirs):
         print(f"| {k} | {n} | " + " | ".join(cells) + " |")
 
 
-def per_kernel(d, counter):
+def per_kernel(d, counter, largest=False):
     agg = collections.defaultdict(list)
     f = glob.glob(os.path.join(d, "*counter_collection.csv"))[0]
     for r in csv.DictReader(open(f)):
         if r["Counter_Name"] == counter:
-            agg[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in agg.items()}
+            agg[short(r["Kernel_Name"])].append((int(r["Grid_Size"]), float(r["Counter_Value"])))
+    out = {}
+    for k, v in agg.items():
+        g = max(x[0] for x in v) if largest else None
+        vals = [x[1] for x in v if g is None or x[0] == g]
+        out[k] = sum(vals) / len(vals)
+    return out
 
 
-def traffic(out, workload, fdir, wdir, cfdir, cwdir, copy_mib):
+def largest_stats(d):
+    f = glob.glob(os.path.join(d, "*kernel_trace.csv"))[0]
+    agg = collections.defaultdict(list)
+    for r in csv.DictReader(open(f)):
+        g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        agg[short(r["Kernel_Name"])].append((g, (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+    rows = []
+    for k, v in agg.items():
+        g = max(x[0] for x in v)
+        t = [x[1] for x in v if x[0] == g]
+        rows.append((sum(t) / len(t), k, len(t), min(t), max(t), g))
+    print("| kernel | calls (largest grid) | grid (threads) | avg µs | min µs | max µs |")
+    print("|---|---|---|---|---|---|")
+    for avg, k, n, lo, hi, g in sorted(rows, reverse=True):
+        print(f"| {k} | {n} | {g} | {avg:.2f} | {lo:.2f} | {hi:.2f} |")
+
+
+def traffic(out, workload, fdir, wdir, cfdir, cwdir, copy_mib, largest=False):
     kib = float(copy_mib) * 1024.0
     cf = next(v for k, v in per_kernel(cfdir, "FETCH_SIZE").items() if k.startswith("copy_kernel"))
     cw = next(v for k, v in per_kernel(cwdir, "WRITE_SIZE").items() if k.startswith("copy_kernel"))
     ff, wf = kib / cf, kib / cw
-    fk, wk = per_kernel(fdir, "FETCH_SIZE"), per_kernel(wdir, "WRITE_SIZE")
+    fk, wk = per_kernel(fdir, "FETCH_SIZE", largest), per_kernel(wdir, "WRITE_SIZE", largest)
     res = {"workload": workload, "fetch_factor": ff, "write_factor": wf, "kernels": {}}
     for k in sorted(set(fk) & set(wk)):
         if k.startswith("__amd"):
@@ -86,6 +113,8 @@ if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(sys.argv[2])
     elif sys.argv[1] == "traffic":
-        traffic(*sys.argv[2:9])
+        traffic(*sys.argv[2:9], largest="--largest" in sys.argv[9:])
+    elif sys.argv[1] == "largest":
+        largest_stats(sys.argv[2])
     else:
         pmc(sys.argv[2:])

@@ -143,6 +143,14 @@ int dfk_snapshot_load(dfk_handle *h, int disk, const char *path);
 int dfk_set_shapes(dfk_handle *h, int disk, int ti, int tj, int pr, int cb);
 int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
 
+/* Where the row transforms (rows) and the column convolution (cols) of a disk
+ * read their twiddle table: 1 = staged in LDS per workgroup (default; fastest
+ * when few workgroups run), 0 = straight from HBM through L1/L2 (smaller
+ * workgroups, more of them per CU; faster in large batches).  An autotuner
+ * knob like dfk_set_shapes; no counterpart in the reference (FFTW plans). */
+int dfk_set_twiddle_lds(dfk_handle *h, int disk, int rows, int cols);
+int dfk_get_twiddle_lds(dfk_handle *h, int disk, int *rows, int *cols);
+
 /* Threads per workgroup (64, 128, 192 or 256; 0 keeps the current value) of
  * the row transforms, the column convolution and the update kernel, for all
  * disks of the handle (a batched launch has one workgroup size). */

@@ -88,6 +88,23 @@ def test_potential_large_batches(orc, dfk, Nu, Nphi, pr, cb):
     assert relerr(e.potential(Sig), k.potential(Sig)) < 1e-13
 
 
+@pytest.mark.parametrize("Nu,Nphi", [(256, 256), (35, 40), (96, 80), (33, 26)])
+def test_potential_twiddles_through_caches(orc, dfk, Nu, Nphi):
+    """Both transforms reading their twiddle table from HBM instead of LDS
+    (dfk_set_twiddle_lds(0, 0)): the same potential, on the HBM-I/O stage
+    path (radices 2/3/4/8) and the general one (35 x 40: 70 = 2 x 5 x 7)."""
+    print(f"case potential {Nu}x{Nphi} twiddles from caches", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    rng = np.random.default_rng(11)
+    Sig = rng.random((Nu, Nphi))
+    p_lds = e.potential(Sig)
+    e.set_twiddle_lds(0, 0)
+    assert e.get_twiddle_lds(0) == (0, 0)
+    p_l2 = e.potential(Sig)
+    assert np.array_equal(p_lds, p_l2)
+    assert relerr(p_l2, k.potential(Sig)) < 1e-13
+
+
 @pytest.mark.parametrize("Nu,Nphi", [(16, 16), (35, 40), (128, 128), (33, 26)])
 def test_forces_match_oracle(orc, dfk, Nu, Nphi):
     print(f"case forces {Nu}x{Nphi}", flush=True)
