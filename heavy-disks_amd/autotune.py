@@ -24,7 +24,8 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-CACHE = os.path.join(HERE, "tuned_shapes.json")
+# the shipped cache; DFK_TUNED_SHAPES points apply()/tune() at another file
+CACHE = os.environ.get("DFK_TUNED_SHAPES", os.path.join(HERE, "tuned_shapes.json"))
 LDS_MAX = 160 * 1024    # gfx950: LDS per workgroup (the library checks the device's value)
 
 
