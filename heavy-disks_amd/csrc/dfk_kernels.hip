@@ -78,7 +78,10 @@ __device__ __forceinline__ const double *gtab(const double *geom, const DiskDev 
 // ------------------------------------------------------------------ FFT
 // Quotient of small non-negative ints (a < 2^22, b <= 2^12) through a float
 // reciprocal: (a + 0.5) / b is at least 0.5/b away from an integer, which the
-// float rounding error (< 2^-22 relative) cannot cross.
+// float rounding error (< 2^-22 relative) cannot cross.  Used for every
+// division by a run-time value in the loop kernels: the GPU has no integer
+// divide, and the emulated one (~40 instructions, scalar or per lane) was a
+// large part of the FFT kernels' instruction count.
 __device__ __forceinline__ int qdiv(int a, float rb) { return (int)(((float)a + 0.5f) * rb); }
 
 __device__ __forceinline__ double2 twid(const double2 *__restrict__ tw, int t, int sign) {
@@ -154,7 +157,7 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
                                           int n, int batch, int Ns,
                                           const double2 *__restrict__ tw) {
   const int nb = n / R;
-  const int tstep = n / (Ns * R);
+  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));   // exact quotient, no integer division
   const int total = nb * batch;
   const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -185,9 +188,9 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
 template <int SIGN>
 __device__ __forceinline__ void fft_stage_generic(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
                                   int batch, int Ns, int R, const double2 *__restrict__ tw) {
-  const int nb = n / R;
-  const int tstep = n / (Ns * R);
-  const int rstep = n / R;
+  const int nb = qdiv(n, 1.0f / (float)R);
+  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));
+  const int rstep = nb;
   const int total = nb * batch;
   const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -284,7 +287,7 @@ __device__ __forceinline__ void io_stage(const double2 *__restrict__ a, double2 
                                          int n, int batch, int Ns, const double2 *__restrict__ tw,
                                          const StageIO &g) {
   const int nb = n / R;
-  const int tstep = n / (Ns * R);
+  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));
   const int total = nb * batch;
   const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -504,7 +507,7 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int e = threadIdx.x; e < D.ti * D.tj; e += blockDim.x) {
-    const int li = e / D.tj, lj = e - li * D.tj;
+    const int li = qdiv(e, 1.0f / (float)D.tj), lj = e - li * D.tj;
     const int i = i0 + li, j = j0 + lj;
     if (i >= Nu || j >= Nphi) continue;
     const i64 c = (i64)i * Nphi + j;
@@ -555,7 +558,7 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   // all threads stage the rows (more loads in flight than a first stage
   // reading HBM through its few butterflies: measured faster here)
   for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
-    const int p = e / n, j = e - p * n;
+    const int p = qdiv(e, 1.0f / (float)n), j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     double xa = 0.0, xb = 0.0;
     if (ra < Nu) xa = R32[ra] * S[(i64)ra * n + j];
@@ -567,7 +570,7 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   double2 *out = spec + D.spo;
   const int w2 = 2 * pr;
   for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
-    const int k = e / w2, rr = e - k * w2;
+    const int k = qdiv(e, 1.0f / (float)w2), rr = e - k * w2;
     const int p = rr >> 1, isb = rr & 1;
     const int row = row0 + rr;
     if (row >= Nu) continue;
@@ -623,7 +626,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
     return;
   }
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
-    const int c = e / M, p = e - c * M;
+    const int c = qdiv(e, 1.0f / (float)M), p = e - c * M;
     const int k = k0 + c;
     double2 v = make_double2(0.0, 0.0);
     if (k < Nc && p < Nu) v = io[(i64)k * Nu + p];
@@ -633,7 +636,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, TW);
   double2 *Wb = (Z == A) ? B : A;
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
-    const int c = e / M, p = e - c * M;
+    const int c = qdiv(e, 1.0f / (float)M), p = e - c * M;
     const int k = k0 + c;
     const int pp = (p <= Nu) ? p : M - p;
     const double kv = (k < Nc) ? kh[(i64)k * (Nu + 1) + pp] : 0.0;
@@ -645,7 +648,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   __syncthreads();
   const double2 *Y = lds_fft<+1>(Z, Wb, M, cb, D.nst_col, D.rad_col, TW);
   for (int e = threadIdx.x; e < cb * Nu; e += blockDim.x) {
-    const int c = e / Nu, p = e - c * Nu;
+    const int c = qdiv(e, 1.0f / (float)Nu), p = e - c * Nu;
     const int k = k0 + c;
     if (k < Nc) io[(i64)k * Nu + p] = Y[c * M + p];
   }
@@ -668,7 +671,7 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   const double2 *TW = tw_table(lds2 + 2 * pr * n, tw + D.two_row, n, D.tw_lds & 1);
   const double2 *in = spec + D.spo;
   for (int e = threadIdx.x; e < pr * Nc; e += blockDim.x) {
-    const int k = e / pr, p = e - k * pr;
+    const int k = qdiv(e, 1.0f / (float)pr), p = e - k * pr;
     const int ra = row0 + 2 * p, rb = ra + 1;
     double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
     if (ra < Nu) xa = in[(i64)k * Nu + ra];
@@ -696,7 +699,7 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   }
   const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
   for (int e = threadIdx.x; e < 2 * pr * n; e += blockDim.x) {
-    const int rr = e / n, j = e - rr * n;
+    const int rr = qdiv(e, 1.0f / (float)n), j = e - rr * n;
     const int row = row0 + rr;
     if (row >= Nu) continue;
     const double2 z = Z[(rr >> 1) * n + j];
@@ -804,7 +807,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
 
   // ---- phase 1: source (+ forces) on rows i0-2..i0+TI+1, cols j0-2..j0+TJ+1
   for (int e = threadIdx.x; e < SH * SW; e += blockDim.x) {
-    const int lr = e / SW, lc = e - lr * SW;
+    const int lr = qdiv(e, 1.0f / (float)SW), lc = e - lr * SW;
     const int i = i0 - 2 + lr;
     if (i < 0 || i >= Nu) continue;
     const int j = wrapj(j0 - 2 + lc, Nphi);
@@ -840,7 +843,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
                *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL);
   (void)tRf;
   for (int e = threadIdx.x; e < (TI + 1) * SW; e += blockDim.x) {
-    const int lf = e / SW, lc = e - lf * SW;
+    const int lf = qdiv(e, 1.0f / (float)SW), lc = e - lf * SW;
     const int f = i0 + lf;
     double fm = 0.0, fu = 0.0, fw = 0.0;
     if (f >= 1 && f <= Nu - 1) {
@@ -872,7 +875,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
 
   // ---- phase 3: radial gather for rows i0..i0+TI-1 (in place into s*)
   for (int e = threadIdx.x; e < TI * SW; e += blockDim.x) {
-    const int lr = e / SW, lc = e - lr * SW;
+    const int lr = qdiv(e, 1.0f / (float)SW), lc = e - lr * SW;
     const int i = i0 + lr;
     if (i >= Nu) continue;
     const int l = (lr + 2) * SW + lc;
@@ -891,7 +894,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   const double *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE);
   const int FW = TJ + 1;
   for (int e = threadIdx.x; e < TI * FW; e += blockDim.x) {
-    const int lr = e / FW, lf = e - lr * FW;
+    const int lr = qdiv(e, 1.0f / (float)FW), lf = e - lr * FW;
     const int i = i0 + lr;
     double fm = 0.0, fu = 0.0, fw = 0.0;
     if (i < Nu) {
@@ -925,7 +928,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int e = threadIdx.x; e < TI * TJ; e += blockDim.x) {
-    const int lr = e / TJ, lc = e - lr * TJ;
+    const int lr = qdiv(e, 1.0f / (float)TJ), lc = e - lr * TJ;
     const int i = i0 + lr, j = j0 + lc;
     if (i >= Nu || j >= Nphi) continue;
     const int l = (lr + 2) * SW + lc + 2;
