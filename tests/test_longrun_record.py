@@ -39,3 +39,40 @@ def test_growth_close_to_published(rec):
 def test_mass_conserved(rec):
     for name, r in rec.items():
         assert abs(r["mass_drift_gpu"]) < 1e-9, name
+
+
+REC_PL = os.path.join(HERE, "longrun", "results_pl256.json")
+REC_SG = os.path.join(HERE, "longrun", "surrogate_eval.json")
+
+
+def test_config2_long_run_record():
+    """256^2 power-law disk, 20 outer orbits (784k steps): gamma and Omega_p
+    of the inner-edge growth agree to round-off; the saturation amplitudes
+    (first peak, late mean) within 1% although the turbulent late state has
+    decorrelated; mass conserved on both sides."""
+    if not os.path.exists(REC_PL):
+        pytest.skip("no recorded 256^2 long run")
+    (r,) = json.load(open(REC_PL))
+    assert r["t_gpu"] == r["t_oracle"] == 89.0
+    assert abs(r["steps_gpu"] - r["steps_oracle"]) / r["steps_oracle"] < 1e-3
+    assert r["rel_diff"]["gamma"] < 1e-9 and r["rel_diff"]["omega_p"] < 1e-9
+    assert r["rel_diff"]["sat"] < 0.01
+    assert abs(r["gpu"]["sat_peak"] - r["oracle"]["sat_peak"]) / r["oracle"]["sat_peak"] < 0.01
+    assert abs(r["mass_drift_gpu"]) < 1e-8 and abs(r["mass_drift_oracle"]) < 1e-8
+
+
+def test_surrogate_record_beats_baselines_and_meets_three_percent():
+    """The recorded surrogate evaluation (256 held-out hydro runs): median
+    relative error of every output below 3% with and without the hydro
+    fallback, and below both surrogate-free baselines."""
+    if not os.path.exists(REC_SG):
+        pytest.skip("no recorded surrogate evaluation")
+    r = json.load(open(REC_SG))
+    assert r["n_train"] == 4096 and r["n_test"] == 256 and r["scan"]["points"] == 100000
+    for n in ("gamma", "omega_p", "sat"):
+        sg, fb = r["surrogate"][n], r["surrogate_with_fallback"][n]
+        assert sg["median_rel_err"] < 0.03 and fb["mean_rel_err"] < 0.03
+        assert fb["median_rel_err"] <= sg["median_rel_err"]
+        for base in ("nearest_neighbour", "linear_interp"):
+            assert sg["median_rel_err"] < r[base][n]["median_rel_err"]
+
