@@ -54,8 +54,8 @@ def kernel_bytes(Nu, Nphi):
     }
 
 
-def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk):
-    m = dfk.model_defaults(dfk.MODEL_POWERLAW)
+def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk, model=None):
+    m = dfk.model_defaults(dfk.MODEL_POWERLAW if model is None else model)
     p, S, U, W, mD, _ = dfk.model_ic(m, Nu, Nphi, eps)
     e = dfk.Ensemble(p)
     # the latency-regime shapes where tuned for this grid, else the
@@ -220,6 +220,14 @@ def main():
                      "unit": "GB/s", "frac": ach / (HBM_PEAK / 1e9), "traffic": traffic,
                      "alg_bytes_per_launch": kb[dom]},
     }
+    # like for like with the published CPU figure: the reference's own disk
+    # (LKA98 ring, eps 0.005, CFL 0.3 as `diskdisk 256 256 0.005 16 1 1e-3`)
+    ref = single_disk(dfk, 256, 256, 0.005, 0.3, args.steps, args.warmup, args.chunk,
+                      model=dfk.MODEL_LKA98)
+    lka = 256 * 256 * args.steps / ref["wall"]
+    out["reference_disk"] = {"workload": "LKA98 ring 256x256, eps 0.005, CFL 0.3 (the reference's "
+                                         "diskdisk default model)", "value": lka,
+                             "vs_published_cpu": lka / REF_PUBLISHED}
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
         out["cpu_baseline"] = cpu_baseline(r["p"], r["S"], r["U"], r["W"], args.cfl,

@@ -77,7 +77,7 @@ def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
     for d, c in enumerate(found):
         if c:
             ens.set_shapes(c["ti"], c["tj"], c["pr"], c["cb"], d=d)
-            ens.set_twiddle_lds(*c.get("tw_lds", (1, 1)), d=d)
+            ens.set_fft_options(*c.get("fft", list(c.get("tw_lds", (1, 1))) + [0]), d=d)
             if "threads" in c:
                 t = tuple(c["threads"])
                 votes[t] = votes.get(t, 0) + ens.shape(d)[0] * ens.shape(d)[1]
@@ -127,15 +127,26 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     for pr in prs:
         ms = prof(base[0], base[1], pr, base[3])
         res["pr"][str(pr)] = float(ms[1] + ms[3])
-    for twl in (1, 0):
-        e.set_twiddle_lds(1, twl)
-        for cb in cbs:
-            res["cb"][f"{cb}:{twl}"] = float(prof(base[0], base[1], base[2], cb)[2])
+    # column batches: twiddles in LDS or through the caches; workgroup-
+    # cooperative columns (ping-pong buffers) or one wave per column in place
+    thr0 = e.get_threads()
+    M, Nc = 2 * Nu, Nphi // 2 + 1
+    cbs_wave = [c for c in (1, 2, 4, 8, 16) if (c + 1) * M * 16 <= LDS_MAX and c <= Nc]
+    for wave in (0, 1):
+        for twl in (1, 0):
+            e.set_fft_options(1, twl, wave)
+            if wave and e.get_fft_options(0)[2] == 0:
+                break                  # this length has no wave-local transform
+            for cb in (cbs_wave if wave else cbs):
+                if wave:
+                    e.set_threads(thr0[0], min(256, 64 * cb), thr0[2])
+                res["cb"][f"{cb}:{twl}:{wave}"] = float(prof(base[0], base[1], base[2], cb)[2])
+            e.set_threads(*thr0)
     bt = min(res["tile"], key=res["tile"].get)
     bst = min(res["stream"], key=res["stream"].get) if res["stream"] else None
     bpr, twr = int(min(res["pr"], key=res["pr"].get)), 1
-    bcb, twc = (int(x) for x in min(res["cb"], key=res["cb"].get).split(":"))
-    e.set_twiddle_lds(twr, twc)
+    bcb, twc, cwave = (int(x) for x in min(res["cb"], key=res["cb"].get).split(":"))
+    e.set_fft_options(twr, twc, cwave)
     ti, tj = (int(x) for x in bt.split("x"))
     kind = 0
     if bst is not None and res["stream"][bst] < res["tile"][bt]:
@@ -152,14 +163,14 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     best = prof(ti, tj, bpr, bcb)
     e.close()
     out = {"kind": kind, "ti": ti, "tj": tj, "pr": bpr, "cb": bcb, "threads": list(thr),
-           "tw_lds": [twr, twc],
+           "fft": [twr, twc, cwave],
            "copies": copies,
            "ms_per_kernel": [float(x) for x in best],
            "default": {"shape": list(base), "tile_ms": res["tile"].get(f"{base[0]}x{base[1]}")},
            "sweep": res}
     if verbose:
         print(f"{Nu}x{Nphi} {regime} x{copies}: kernel {kind} tile {ti}x{tj} pr {bpr} cb {bcb} "
-              f"twiddles in LDS (rows, cols) {(twr, twc)} threads {thr} "
+              f"twiddles in LDS (rows, cols) {(twr, twc)} wave columns {cwave} threads {thr} "
               f"ms {np.round(best, 4).tolist()}", flush=True)
     return out
 

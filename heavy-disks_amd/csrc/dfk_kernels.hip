@@ -346,6 +346,93 @@ __device__ __forceinline__ void io_stage_r(int r, const double2 *a, double2 *b, 
   else io_stage<3, SIGN, IO>(a, b, n, batch, Ns, tw, g);
 }
 
+// ------------------------------------------------------------------ wave-local column transforms
+// Throughput-regime variant of the column convolution: one wavefront owns a
+// whole column and transforms it in place in a single LDS buffer of M
+// elements (no ping-pong buffer, no workgroup barrier).  In each stage every
+// lane loads the inputs of all its butterflies (at most 1024 / 64 = 16
+// complex values for M <= 1024), waits for the loads, then computes and
+// stores: no lane overwrites a value another lane still has to read.  Same
+// butterflies, twiddles and I/O variants as io_stage (results agree to
+// round-off; the compiler's multiply-add contraction may differ).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): this wave's LDS accesses are done
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int R, int SIGN, int IO>
+__device__ __forceinline__ void wave_stage(double2 *buf, int n, int p, int Ns,
+                                           const double2 *__restrict__ tw, const StageIO &g) {
+  constexpr int EMAX = (1024 / R + 63) / 64;   // butterflies per lane at n <= 1024
+  const int nb = n / R;
+  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));
+  const float rNs = 1.0f / (float)Ns;
+  const int lane = threadIdx.x & 63;
+  const int col = g.k0 + p;
+  const bool live = col < g.Nc;
+  double2 v[EMAX][R];
+#pragma unroll
+  for (int u = 0; u < EMAX; ++u) {
+    const int j = lane + 64 * u;
+    if (j < nb) {
+      const int jq = qdiv(j, rNs);
+      const int k = j - jq * Ns;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        const int idx = j + m * nb;
+        double2 x;
+        if constexpr (IO == IO_ZP) {
+          const bool pad = (R % 2 == 0) ? (m >= R / 2) : (idx >= g.Nu);
+          x = (live && !pad) ? g.spec[(i64)col * g.Nu + idx] : make_double2(0.0, 0.0);
+        } else {
+          x = buf[idx];
+          if (m > 0 && k > 0) x = cmul(x, twid(tw, k * m * tstep, SIGN));
+        }
+        v[u][m] = x;
+      }
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int u = 0; u < EMAX; ++u) {
+    const int j = lane + 64 * u;
+    if (j < nb) {
+      const int jq = qdiv(j, rNs);
+      const int k = j - jq * Ns;
+      if constexpr (R == 2) bfly2<SIGN>(v[u][0], v[u][1]);
+      if constexpr (R == 3) bfly3<SIGN>(v[u]);
+      if constexpr (R == 4) bfly4<SIGN>(v[u]);
+      if constexpr (R == 8) bfly8<SIGN>(v[u]);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int o = jq * Ns * R + k + q * Ns;
+        if constexpr (IO == IO_KHAT) {
+          const double kv = live ? g.kh[(i64)col * (g.Nu + 1) + (o <= g.Nu ? o : n - o)] : 0.0;
+          double2 z = v[u][q];
+          z.x *= kv;
+          z.y *= kv;
+          buf[o] = z;
+        } else if constexpr (IO == IO_OUT) {
+          if (live && o < g.Nu) g.spec[(i64)col * g.Nu + o] = v[u][q];
+        } else {
+          buf[o] = v[u][q];
+        }
+      }
+    }
+  }
+  wave_lds_sync();
+}
+
+template <int SIGN, int IO>
+__device__ __forceinline__ void wave_stage_r(int r, double2 *buf, int n, int p, int Ns,
+                                             const double2 *tw, const StageIO &g) {
+  if (r == 8) wave_stage<8, SIGN, IO>(buf, n, p, Ns, tw, g);
+  else if (r == 4) wave_stage<4, SIGN, IO>(buf, n, p, Ns, tw, g);
+  else if (r == 2) wave_stage<2, SIGN, IO>(buf, n, p, Ns, tw, g);
+  else wave_stage<3, SIGN, IO>(buf, n, p, Ns, tw, g);
+}
+
 // Whether a transform can use the global-I/O stages.
 __device__ __forceinline__ bool io_fast(int nst, const int *rad) {
   bool f = nst >= 2;
@@ -651,6 +738,46 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
     const int c = qdiv(e, 1.0f / (float)Nu), p = e - c * Nu;
     const int k = k0 + c;
     if (k < Nc) io[(i64)k * Nu + p] = Y[c * M + p];
+  }
+}
+
+// Column convolution with one wavefront per column, in place (the disks of a
+// handle whose column transforms the runtime marked wave-local: radices
+// 2/3/4/8, >= 2 stages, M <= 1024).  LDS: cb columns of M elements, then the
+// twiddle table if staged.  A kernel of its own: the wave stages hold up to
+// 16 complex values per lane, which would otherwise set the register budget
+// (and so the occupancy) of the workgroup-cooperative kernel too.
+__global__ void __launch_bounds__(WG)
+k_cols_conv_wave(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+                 const DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
+                 const double2 *__restrict__ tw, double2 *__restrict__ spec) {
+  extern __shared__ double2 lds2[];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
+  const int k0 = wm.y * cb;
+  double2 *A = lds2;
+  const double2 *TW = tw_table(lds2 + cb * M, tw + D.two_col, M, D.tw_lds & 2);
+  StageIO g{};
+  g.spec = spec + D.spo; g.kh = khat + D.kho; g.Nu = Nu; g.Nc = Nc; g.k0 = k0;
+  const int ns = D.nst_col, nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  __syncthreads();   // the twiddle table, if staged in LDS
+  for (int p = w; p < cb; p += nw) {   // wave w: columns w, w + nw, ...
+    double2 *buf = A + p * M;
+    wave_stage_r<-1, IO_ZP>(D.rad_col[0], buf, M, p, 1, TW, g);
+    int Ns = D.rad_col[0];
+    for (int q = 1; q < ns - 1; ++q) {
+      wave_stage_r<-1, IO_LDS>(D.rad_col[q], buf, M, p, Ns, TW, g);
+      Ns *= D.rad_col[q];
+    }
+    wave_stage_r<-1, IO_KHAT>(D.rad_col[ns - 1], buf, M, p, Ns, TW, g);
+    Ns = 1;
+    for (int q = 0; q < ns - 1; ++q) {
+      wave_stage_r<+1, IO_LDS>(D.rad_col[q], buf, M, p, Ns, TW, g);
+      Ns *= D.rad_col[q];
+    }
+    wave_stage_r<+1, IO_OUT>(D.rad_col[ns - 1], buf, M, p, Ns, TW, g);
   }
 }
 
@@ -1340,6 +1467,15 @@ hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const 
                            const double2 *tw, double2 *spec) {
   if (nblk <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cols_conv, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
+                                const int2 *map, int base, const DiskCtl *ctl, const double *khat,
+                                const double2 *tw, double2 *spec) {
+  if (nblk <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cols_conv_wave, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat,
+                     tw, spec);
   return hipGetLastError();
 }
 

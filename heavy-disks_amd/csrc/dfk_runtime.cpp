@@ -110,9 +110,9 @@ struct dfk_handle {
   bool layout_ok = false;
   double *pcfl = nullptr, *pcm = nullptr;
   i64 ntiles = 0;
-  int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr;
-  std::vector<int> upd0, r2c0, conv0;   // per-disk first block (size nd+1)
-  size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0;
+  int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr, *map_convw = nullptr;
+  std::vector<int> upd0, r2c0, conv0, convw0;   // per-disk first block (size nd+1)
+  size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0, lds_convw = 0;
   size_t lds_max = 64 * 1024;   // per workgroup, from the device at create (gfx950: 160 KiB)
   DiskCtl *ctl = nullptr;
   // run
@@ -160,12 +160,13 @@ int build_layout(dfk_handle *h) {
   if (h->layout_ok) return DFK_OK;
   drop_graphs(h);
   h->run_ready = false;
-  std::vector<int2> mu, mr, mc;
+  std::vector<int2> mu, mr, mc, mcw;   // update, rows, columns, wave-local columns
   h->upd0.assign(h->nd + 1, 0);
   h->r2c0.assign(h->nd + 1, 0);
   h->conv0.assign(h->nd + 1, 0);
+  h->convw0.assign(h->nd + 1, 0);
   i64 part = 0;
-  size_t lu = 0, lr = 0, lc = 0;
+  size_t lu = 0, lr = 0, lc = 0, lcw = 0;
   const size_t kMaxLds = h->lds_max;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
@@ -185,6 +186,7 @@ int build_layout(dfk_handle *h) {
     D.cb = std::max(1, std::min(D.cb, D.Nc));
     h->r2c0[d] = (int)mr.size();
     h->conv0[d] = (int)mc.size();
+    h->convw0[d] = (int)mcw.size();
     if (D.selfgrav) {
       // batches shrink until they fit the LDS budget (a cached shape from a
       // build with smaller LDS needs stays usable)
@@ -192,36 +194,41 @@ int build_layout(dfk_handle *h) {
       auto lds_rows = [&](int pr) {
         return (size_t)(2 * pr + (D.tw_lds & 1)) * D.Nphi * sizeof(double2);
       };
-      auto lds_cols = [&](int cb) {
-        return (size_t)(2 * cb + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
+      auto lds_cols = [&](int cb) {   // wave-local columns are transformed in place
+        return (size_t)((D.conv_wave ? 1 : 2) * cb + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
       };
       while (D.pr > 1 && lds_rows(D.pr) > kMaxLds) --D.pr;
       while (D.cb > 1 && lds_cols(D.cb) > kMaxLds) --D.cb;
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
-      for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) mc.push_back(make_int2(d, g));
+      std::vector<int2> &cm = D.conv_wave ? mcw : mc;
+      for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) cm.push_back(make_int2(d, g));
       lr = std::max(lr, lds_rows(D.pr));
-      lc = std::max(lc, lds_cols(D.cb));
+      (D.conv_wave ? lcw : lc) = std::max(D.conv_wave ? lcw : lc, lds_cols(D.cb));
     }
   }
   h->upd0[h->nd] = (int)mu.size();
   h->r2c0[h->nd] = (int)mr.size();
   h->conv0[h->nd] = (int)mc.size();
-  if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds) {
-    set_err("tile/batch shapes need %zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, kMaxLds);
+  h->convw0[h->nd] = (int)mcw.size();
+  if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds || lcw > kMaxLds) {
+    set_err("tile/batch shapes need %zu/%zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, lcw, kMaxLds);
     return DFK_EINVAL;
   }
   h->lds_upd = lu;
   h->lds_r2c = lr;
   h->lds_conv = lc;
+  h->lds_convw = lcw;
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv);
+  hipFree(h->map_convw);
   h->pcfl = h->pcm = nullptr;
-  h->map_upd = h->map_r2c = h->map_conv = nullptr;
+  h->map_upd = h->map_r2c = h->map_conv = h->map_convw = nullptr;
   h->ntiles = part;
   TRY(dalloc(&h->pcfl, part));
   TRY(dalloc(&h->pcm, 4 * part));
   TRY(dalloc(&h->map_upd, (i64)mu.size()));
   TRY(dalloc(&h->map_r2c, (i64)mr.size()));
   TRY(dalloc(&h->map_conv, (i64)mc.size()));
+  TRY(dalloc(&h->map_convw, (i64)mcw.size()));
   // unit calls launch one disk's range [upd0[d], upd0[d+1]) of the map, so
   // each disk's range is reordered on its own
   auto per_disk = [&](std::vector<int2> &m, const std::vector<int> &first) {
@@ -235,14 +242,18 @@ int build_layout(dfk_handle *h) {
     mu = xcd_order(mu);
     mr = xcd_order(mr);
     mc = xcd_order(mc);
+    mcw = xcd_order(mcw);
   } else {
     per_disk(mu, h->upd0);
     per_disk(mr, h->r2c0);
     per_disk(mc, h->conv0);
+    per_disk(mcw, h->convw0);
   }
   HIPCHK(hipMemcpy(h->map_upd, mu.data(), sizeof(int2) * mu.size(), hipMemcpyHostToDevice));
   if (!mr.empty()) HIPCHK(hipMemcpy(h->map_r2c, mr.data(), sizeof(int2) * mr.size(), hipMemcpyHostToDevice));
   if (!mc.empty()) HIPCHK(hipMemcpy(h->map_conv, mc.data(), sizeof(int2) * mc.size(), hipMemcpyHostToDevice));
+  if (!mcw.empty())
+    HIPCHK(hipMemcpy(h->map_convw, mcw.data(), sizeof(int2) * mcw.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dD, h->hD.data(), sizeof(DiskDev) * h->nd, hipMemcpyHostToDevice));
   h->layout_ok = true;
   return DFK_OK;
@@ -281,13 +292,24 @@ hipError_t launch_r2c(dfk_handle *h, int nblk, int base, int par, int nctl) {
                         h->samples, h->cap);
 }
 
+// Column convolution of disk d (all disks for d < 0): the workgroup-
+// cooperative kernel for the disks that use it, the wave-local one for the rest.
+hipError_t launch_conv(dfk_handle *h, int d) {
+  const int a = d < 0 ? 0 : d, b = d < 0 ? h->nd : d + 1;
+  const int n = h->conv0[b] - h->conv0[a], nw = h->convw0[b] - h->convw0[a];
+  hipError_t e = dfk_launch_conv(h->stream, n, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[a],
+                                 h->ctl, h->khat, h->tw, h->spec);
+  if (e != hipSuccess) return e;
+  return dfk_launch_conv_wave(h->stream, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
+                              h->convw0[a], h->ctl, h->khat, h->tw, h->spec);
+}
+
 // Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk,
 // Sigma read from state buffer `par`.
 int launch_gravity_disk(dfk_handle *h, int d, int par) {
-  const int nr = h->r2c0[d + 1] - h->r2c0[d], nc = h->conv0[d + 1] - h->conv0[d];
+  const int nr = h->r2c0[d + 1] - h->r2c0[d];
   HIPCHK(launch_r2c(h, nr, h->r2c0[d], par, 0));
-  HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[d], h->ctl,
-                         h->khat, h->tw, h->spec));
+  HIPCHK(launch_conv(h, d));
   HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
                         h->geom, h->tw, h->phi));
   return DFK_OK;
@@ -313,11 +335,10 @@ hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR
 // buffer 1 - par.  The loop control rides in the r2c launch (a launch of its
 // own when no disk has self-gravity).
 int launch_step_all(dfk_handle *h, int par) {
-  const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
+  const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd];
   if (nr > 0) {
     HIPCHK(launch_r2c(h, nr, 0, par, h->nd));
-    HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
-                           h->tw, h->spec));
+    HIPCHK(launch_conv(h, -1));
     HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
                           h->geom, h->tw, h->phi));
   } else {
@@ -607,6 +628,7 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
   for (int d = 0; d < ndisks; ++d) {
     DiskDev &D = h->hD[d];
     D.tw_lds = 3;   // twiddles of row and column transforms staged in LDS
+    D.conv_wave = 0;
     if (small) {
       D.ti = 8; D.tj = 16; D.pr = 1; D.cb = 1;
     } else {
@@ -657,7 +679,7 @@ void dfk_destroy(dfk_handle *h) {
   drop_graphs(h);
   hipFree(h->dD); hipFree(h->st); hipFree(h->phi); hipFree(h->gR); hipFree(h->gp); hipFree(h->F);
   hipFree(h->spec); hipFree(h->khat); hipFree(h->geom); hipFree(h->trig); hipFree(h->tw);
-  hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv);
+  hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv); hipFree(h->map_convw);
   hipFree(h->ctl); hipFree(h->samples);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
@@ -1033,21 +1055,30 @@ int dfk_set_shapes(dfk_handle *h, int d, int ti, int tj, int pr, int cb) {
   return build_layout(h);
 }
 
-int dfk_set_twiddle_lds(dfk_handle *h, int d, int rows, int cols) {
+int dfk_set_fft_options(dfk_handle *h, int d, int rows_tw_lds, int cols_tw_lds, int cols_wave) {
   TRY(check_disk(h, d));
   DiskDev &D = h->hD[d];
-  const int v = (rows ? 1 : 0) | (cols ? 2 : 0);
-  if (v != D.tw_lds) {
+  const int v = (rows_tw_lds ? 1 : 0) | (cols_tw_lds ? 2 : 0);
+  // the wave-local column transform needs radices 2/3/4/8, >= 2 stages, M <= 1024
+  bool fast = D.nst_col >= 2 && D.M <= 1024;
+  for (int q = 0; q < D.nst_col; ++q) {
+    const int r = D.rad_col[q];
+    fast = fast && (r == 2 || r == 3 || r == 4 || r == 8);
+  }
+  const int w = (cols_wave && fast && D.selfgrav) ? 1 : 0;
+  if (v != D.tw_lds || w != D.conv_wave) {
     D.tw_lds = v;
+    D.conv_wave = w;
     h->layout_ok = false;
   }
   return build_layout(h);
 }
 
-int dfk_get_twiddle_lds(dfk_handle *h, int d, int *rows, int *cols) {
+int dfk_get_fft_options(dfk_handle *h, int d, int *rows_tw_lds, int *cols_tw_lds, int *cols_wave) {
   TRY(check_disk(h, d));
-  if (rows) *rows = h->hD[d].tw_lds & 1;
-  if (cols) *cols = (h->hD[d].tw_lds >> 1) & 1;
+  if (rows_tw_lds) *rows_tw_lds = h->hD[d].tw_lds & 1;
+  if (cols_tw_lds) *cols_tw_lds = (h->hD[d].tw_lds >> 1) & 1;
+  if (cols_wave) *cols_wave = h->hD[d].conv_wave;
   return DFK_OK;
 }
 
@@ -1132,7 +1163,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
     c[d].tnext = 1e300;   // no samples: the ring keeps the run's own
     c[d].max_steps = 0;
   }
-  const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd], nc = h->conv0[h->nd];
+  const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd];
   hipEvent_t e0 = h->ev0, e1 = h->ev1;
   for (int k = 0; k < 5; ++k) {
     HIPCHK(hipMemcpy(h->ctl, c.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
@@ -1146,8 +1177,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
           HIPCHK(launch_r2c(h, nr, 0, par, h->nd));   // as in the loop: with the control
           break;
         case 2:
-          HIPCHK(dfk_launch_conv(h->stream, nc, h->thr_cols, h->lds_conv, h->dD, h->map_conv, 0, h->ctl, h->khat,
-                                 h->tw, h->spec));
+          HIPCHK(launch_conv(h, -1));
           break;
         case 3:
           HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,

@@ -31,7 +31,7 @@ SYMBOLS = [
     "dfk_geometry", "dfk_set_state", "dfk_get_state", "dfk_timestep", "dfk_potential",
     "dfk_forces", "dfk_step", "dfk_stage", "dfk_khat", "dfk_modes", "dfk_disk_mass",
     "dfk_run_init", "dfk_advance", "dfk_samples", "dfk_get_status", "dfk_set_shapes",
-    "dfk_get_shapes", "dfk_set_twiddle_lds", "dfk_get_twiddle_lds", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
+    "dfk_get_shapes", "dfk_set_fft_options", "dfk_get_fft_options", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
     "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
     "dfk_snapshot_save", "dfk_snapshot_load", "dfk_set_eos",
 ]
@@ -95,8 +95,8 @@ def lib():
             "dfk_get_status": (_I, [_P, _I, ctypes.POINTER(Status)]),
             "dfk_set_shapes": (_I, [_P, _I, _I, _I, _I, _I]),
             "dfk_get_shapes": (_I, [_P, _I, _IP, _IP, _IP, _IP]),
-            "dfk_set_twiddle_lds": (_I, [_P, _I, _I, _I]),
-            "dfk_get_twiddle_lds": (_I, [_P, _I, _IP, _IP]),
+            "dfk_set_fft_options": (_I, [_P, _I, _I, _I, _I]),
+            "dfk_get_fft_options": (_I, [_P, _I, _IP, _IP, _IP]),
             "dfk_last_advance_ms": (_I, [_P, _DP]),
             "dfk_set_graph_steps": (_I, [_P, _I]),
             "dfk_profile_step": (_I, [_P, _I, _DP]),
@@ -275,16 +275,18 @@ class Ensemble:
         _check(lib().dfk_get_shapes(self.h, d, *[ctypes.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
-    def set_twiddle_lds(self, rows=1, cols=1, d=None):
+    def set_fft_options(self, rows_tw_lds=1, cols_tw_lds=1, cols_wave=0, d=None):
         """Twiddle tables of the row / column transforms in LDS (1) or read
-        through the caches (0); see dfk_set_twiddle_lds."""
+        through the caches (0); wave-local in-place column transforms (1) or
+        workgroup-cooperative ones (0); see dfk_set_fft_options."""
         for q in (range(self.n) if d is None else [d]):
-            _check(lib().dfk_set_twiddle_lds(self.h, q, int(rows), int(cols)))
+            _check(lib().dfk_set_fft_options(self.h, q, int(rows_tw_lds), int(cols_tw_lds),
+                                             int(cols_wave)))
 
-    def get_twiddle_lds(self, d=0):
-        r, c = _I(), _I()
-        _check(lib().dfk_get_twiddle_lds(self.h, d, ctypes.byref(r), ctypes.byref(c)))
-        return r.value, c.value
+    def get_fft_options(self, d=0):
+        v = [_I() for _ in range(3)]
+        _check(lib().dfk_get_fft_options(self.h, d, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def last_advance_ms(self):
         ms = _D()

@@ -143,13 +143,22 @@ int dfk_snapshot_load(dfk_handle *h, int disk, const char *path);
 int dfk_set_shapes(dfk_handle *h, int disk, int ti, int tj, int pr, int cb);
 int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
 
-/* Where the row transforms (rows) and the column convolution (cols) of a disk
- * read their twiddle table: 1 = staged in LDS per workgroup (default; fastest
- * when few workgroups run), 0 = straight from HBM through L1/L2 (smaller
- * workgroups, more of them per CU; faster in large batches).  An autotuner
- * knob like dfk_set_shapes; no counterpart in the reference (FFTW plans). */
-int dfk_set_twiddle_lds(dfk_handle *h, int disk, int rows, int cols);
-int dfk_get_twiddle_lds(dfk_handle *h, int disk, int *rows, int *cols);
+/* Transform options of a disk (autotuner knobs like dfk_set_shapes; no
+ * counterpart in the reference, whose FFTW plans make these choices):
+ *   rows_tw_lds, cols_tw_lds  where the row transforms / the column
+ *       convolution read their twiddle table: 1 = staged in LDS per
+ *       workgroup (default; fastest when few workgroups run), 0 = from HBM
+ *       through L1/L2 (less LDS per workgroup, more workgroups per CU);
+ *   cols_wave  1 = each column of the convolution is transformed by one
+ *       wavefront, in place in one LDS buffer, without workgroup barriers
+ *       (throughput regime; only for 2Nu <= 1024 with radices 2, 3, 4, 8,
+ *       otherwise ignored and read back as 0); 0 = the workgroup cooperates
+ *       on its columns (default).
+ * The twiddle placement does not change a bit of the results; the
+ * wave-local kernel agrees with the cooperative one to round-off. */
+int dfk_set_fft_options(dfk_handle *h, int disk, int rows_tw_lds, int cols_tw_lds, int cols_wave);
+int dfk_get_fft_options(dfk_handle *h, int disk, int *rows_tw_lds, int *cols_tw_lds,
+                        int *cols_wave);
 
 /* Threads per workgroup (64, 128, 192 or 256; 0 keeps the current value) of
  * the row transforms, the column convolution and the update kernel, for all

@@ -88,21 +88,34 @@ def test_potential_large_batches(orc, dfk, Nu, Nphi, pr, cb):
     assert relerr(e.potential(Sig), k.potential(Sig)) < 1e-13
 
 
-@pytest.mark.parametrize("Nu,Nphi", [(256, 256), (35, 40), (96, 80), (33, 26)])
-def test_potential_twiddles_through_caches(orc, dfk, Nu, Nphi):
-    """Both transforms reading their twiddle table from HBM instead of LDS
-    (dfk_set_twiddle_lds(0, 0)): the same potential, on the HBM-I/O stage
-    path (radices 2/3/4/8) and the general one (35 x 40: 70 = 2 x 5 x 7)."""
-    print(f"case potential {Nu}x{Nphi} twiddles from caches", flush=True)
+@pytest.mark.parametrize("Nu,Nphi,wave_ok", [(256, 256, 1), (35, 40, 0), (96, 80, 1), (33, 26, 0),
+                                             (512, 1024, 1), (384, 512, 1), (3, 4, 1)])
+@pytest.mark.parametrize("opts,cb,threads", [((0, 0, 0), 0, 0), ((1, 1, 1), 3, 128),
+                                             ((1, 0, 1), 1, 64), ((0, 1, 1), 8, 256)])
+def test_potential_fft_options(orc, dfk, Nu, Nphi, wave_ok, opts, cb, threads):
+    """Transform options (dfk_set_fft_options): twiddles from HBM instead of
+    LDS give the same bits; the wave-local in-place column transform (a
+    kernel of its own, which may contract multiply-adds differently) agrees
+    to round-off, with fewer, as many and more columns than waves per
+    workgroup; on the HBM-I/O stage path (radices 2/3/4/8) and the general
+    one (35 x 40: 70 = 2 x 5 x 7; 33 x 26: 66 = 2 x 3 x 11), where the wave
+    mode is off."""
+    print(f"case potential {Nu}x{Nphi} fft options {opts} cb {cb} threads {threads}", flush=True)
     e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
     rng = np.random.default_rng(11)
     Sig = rng.random((Nu, Nphi))
-    p_lds = e.potential(Sig)
-    e.set_twiddle_lds(0, 0)
-    assert e.get_twiddle_lds(0) == (0, 0)
-    p_l2 = e.potential(Sig)
-    assert np.array_equal(p_lds, p_l2)
-    assert relerr(p_l2, k.potential(Sig)) < 1e-13
+    p_ref = e.potential(Sig)
+    e.set_fft_options(*opts)
+    if cb:
+        e.set_shapes(0, 0, 0, cb)
+        e.set_threads(256, threads, 256)
+    assert e.get_fft_options(0) == (opts[0], opts[1], opts[2] * wave_ok)
+    p_opt = e.potential(Sig)
+    if opts[2] * wave_ok:
+        assert relerr(p_opt, p_ref) < 1e-14
+    else:
+        assert np.array_equal(p_ref, p_opt)
+    assert relerr(p_opt, k.potential(Sig)) < 1e-13
 
 
 @pytest.mark.parametrize("Nu,Nphi", [(16, 16), (35, 40), (128, 128), (33, 26)])

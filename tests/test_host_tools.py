@@ -40,8 +40,8 @@ class _FakeEnsemble:
     def set_shapes(self, ti, tj, pr, cb, d):
         self.calls.append(("shapes", d, ti, tj, pr, cb))
 
-    def set_twiddle_lds(self, rows, cols, d):
-        self.calls.append(("tw", d, rows, cols))
+    def set_fft_options(self, rows, cols, wave, d):
+        self.calls.append(("fft", d, rows, cols, wave))
 
     def set_threads(self, *t):
         self.calls.append(("threads",) + t)
@@ -51,7 +51,7 @@ def test_apply_votes_kind_and_threads_by_zones(tmp_path):
     import autotune
     cache = {
         "128x128:throughput:gfx950": {"kind": 1, "ti": 16, "tj": 60, "pr": 4, "cb": 4,
-                                      "threads": [128, 128, 128], "tw_lds": [1, 0]},
+                                      "threads": [128, 128, 128], "fft": [1, 0, 1]},
         "512x512:throughput:gfx950": {"kind": 0, "ti": 8, "tj": 32, "pr": 2, "cb": 1,
                                       "threads": [256, 256, 256]},
     }
@@ -62,8 +62,8 @@ def test_apply_votes_kind_and_threads_by_zones(tmp_path):
     assert n == 3                                    # 96x96 has no entry
     assert ("kind", 0) in ens.calls                  # 2 x 512^2 outweigh 128^2
     assert ("threads", 256, 256, 256) in ens.calls
-    assert ("shapes", 0, 16, 60, 4, 4) in ens.calls and ("tw", 0, 1, 0) in ens.calls
-    assert ("tw", 1, 1, 1) in ens.calls              # default when the entry has none
+    assert ("shapes", 0, 16, 60, 4, 4) in ens.calls and ("fft", 0, 1, 0, 1) in ens.calls
+    assert ("fft", 1, 1, 1, 0) in ens.calls          # defaults when the entry has none
     assert not any(c[0] == "shapes" and c[1] == 3 for c in ens.calls)
 
 
