@@ -75,6 +75,7 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk, model=None):
     shapes = dict(zip(("ti", "tj", "pr", "cb"), e.get_shapes(0)))
     shapes["threads"] = e.get_threads()
     shapes["kind"] = e.get_update_kernel()
+    shapes["fft"] = list(e.get_fft_options(0))
     shapes["tuned"] = bool(tuned)
     e.close()
     return dict(p=p, S=S, U=U, W=W, mD=mD, wall=wall, dev_ms=dev_ms, prof=prof, shapes=shapes)
@@ -193,6 +194,9 @@ def main():
         if dom == "update":   # the loop's build: no force arrays, kernel of the tuned kind
             pre = "k_update_stream<false" if r["shapes"].get("kind") == 1 else "k_update<false"
             names = [k for k in tk if k.startswith(pre)]
+        elif dom == "cols_conv":  # cooperative or wave-local column kernel, as tuned
+            wave = r["shapes"].get("fft", [0, 0, 0])[2]
+            names = [k for k in tk if k == ("k_cols_conv_wave" if wave else "k_cols_conv")]
         else:
             names = [k for k in tk if k == "k_" + dom]
         traffic = tk[names[0]]["traffic_bytes"] if names else None

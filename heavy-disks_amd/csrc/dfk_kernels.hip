@@ -77,17 +77,34 @@ __device__ __forceinline__ const double *gtab(const double *geom, const DiskDev 
 
 // ------------------------------------------------------------------ FFT
 // Quotient of small non-negative ints (a < 2^22, b <= 2^12) through a float
-// reciprocal: (a + 0.5) / b is at least 0.5/b away from an integer, which the
-// float rounding error (< 2^-22 relative) cannot cross.  Used for every
+// reciprocal: (a + 0.5) / b is at least 0.5/b away from an integer.  rb comes
+// from the hardware reciprocal (rcpf, within 1 ulp), so the product is within
+// 1.5 * 2^-23 relative of (a + 0.5) / b, which cannot cross that distance for
+// a < 2^21 (every use divides a loop index below 2^16).  Used for every
 // division by a run-time value in the loop kernels: the GPU has no integer
 // divide, and the emulated one (~40 instructions, scalar or per lane) was a
-// large part of the FFT kernels' instruction count.
+// large part of the FFT kernels' instruction count; so was the IEEE float
+// division (~10 instructions) that 1.0f / x compiles to.
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ int qdiv(int a, float rb) { return (int)(((float)a + 0.5f) * rb); }
 
 __device__ __forceinline__ double2 twid(const double2 *__restrict__ tw, int t, int sign) {
   double2 w = tw[t];
   if (sign > 0) w.y = -w.y;
   return w;
+}
+
+// Inputs 1 .. R-1 of a butterfly times their twiddles tw[m kt]: every twiddle
+// load is issued before the first product (one memory latency per butterfly,
+// not R - 1 dependent ones), and lanes with kt = 0 multiply by tw[0] = 1,
+// which is exact, instead of branching around the product.
+template <int R, int SIGN>
+__device__ __forceinline__ void twiddle_mul(double2 (&v)[R], const double2 *__restrict__ tw, int kt) {
+  double2 w[R];
+#pragma unroll
+  for (int m = 1; m < R; ++m) w[m] = twid(tw, m * kt, SIGN);
+#pragma unroll
+  for (int m = 1; m < R; ++m) v[m] = cmul(v[m], w[m]);
 }
 
 // In-register butterflies (forward for SIGN < 0).
@@ -157,9 +174,9 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
                                           int n, int batch, int Ns,
                                           const double2 *__restrict__ tw) {
   const int nb = n / R;
-  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));   // exact quotient, no integer division
+  const int tstep = qdiv(n, rcpf((float)(Ns * R)));   // exact quotient, no integer division
   const int total = nb * batch;
-  const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
+  const float rnb = rcpf((float)nb), rNs = rcpf((float)Ns);
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int p = qdiv(e, rnb);
     const int j = e - p * nb;
@@ -169,11 +186,8 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
     double2 *dst = b + p * n + jq * Ns * R + k;
     double2 v[R];
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      double2 x = src[m * nb];
-      if (m > 0 && k > 0) x = cmul(x, twid(tw, k * m * tstep, SIGN));
-      v[m] = x;
-    }
+    for (int m = 0; m < R; ++m) v[m] = src[m * nb];
+    if (Ns > 1) twiddle_mul<R, SIGN>(v, tw, k * tstep);
     if constexpr (R == 2) bfly2<SIGN>(v[0], v[1]);
     if constexpr (R == 3) bfly3<SIGN>(v);
     if constexpr (R == 4) bfly4<SIGN>(v);
@@ -188,11 +202,11 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
 template <int SIGN>
 __device__ __forceinline__ void fft_stage_generic(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
                                   int batch, int Ns, int R, const double2 *__restrict__ tw) {
-  const int nb = qdiv(n, 1.0f / (float)R);
-  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));
+  const int nb = qdiv(n, rcpf((float)R));
+  const int tstep = qdiv(n, rcpf((float)(Ns * R)));
   const int rstep = nb;
   const int total = nb * batch;
-  const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
+  const float rnb = rcpf((float)nb), rNs = rcpf((float)Ns);
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int p = qdiv(e, rnb);
     const int j = e - p * nb;
@@ -287,9 +301,9 @@ __device__ __forceinline__ void io_stage(const double2 *__restrict__ a, double2 
                                          int n, int batch, int Ns, const double2 *__restrict__ tw,
                                          const StageIO &g) {
   const int nb = n / R;
-  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));
+  const int tstep = qdiv(n, rcpf((float)(Ns * R)));
   const int total = nb * batch;
-  const float rnb = 1.0f / (float)nb, rNs = 1.0f / (float)Ns;
+  const float rnb = rcpf((float)nb), rNs = rcpf((float)Ns);
   for (int e = threadIdx.x; e < total; e += blockDim.x) {
     const int p = qdiv(e, rnb);
     const int j = e - p * nb;
@@ -302,15 +316,15 @@ __device__ __forceinline__ void io_stage(const double2 *__restrict__ a, double2 
 #pragma unroll
     for (int m = 0; m < R; ++m) {
       const int idx = j + m * nb;
-      double2 x;
       if constexpr (IO == IO_ZP) {
         const bool pad = (R % 2 == 0) ? (m >= R / 2) : (idx >= g.Nu);
-        x = (live && !pad) ? g.spec[(i64)col * g.Nu + idx] : make_double2(0.0, 0.0);
+        v[m] = (live && !pad) ? g.spec[(i64)col * g.Nu + idx] : make_double2(0.0, 0.0);
       } else {
-        x = a[p * n + idx];
-        if (m > 0 && k > 0) x = cmul(x, twid(tw, k * m * tstep, SIGN));
+        v[m] = a[p * n + idx];
       }
-      v[m] = x;
+    }
+    if constexpr (IO != IO_ZP) {
+      if (Ns > 1) twiddle_mul<R, SIGN>(v, tw, k * tstep);
     }
     if constexpr (R == 2) bfly2<SIGN>(v[0], v[1]);
     if constexpr (R == 3) bfly3<SIGN>(v);
@@ -366,8 +380,8 @@ __device__ __forceinline__ void wave_stage(double2 *buf, int n, int p, int Ns,
                                            const double2 *__restrict__ tw, const StageIO &g) {
   constexpr int EMAX = (1024 / R + 63) / 64;   // butterflies per lane at n <= 1024
   const int nb = n / R;
-  const int tstep = qdiv(n, 1.0f / (float)(Ns * R));
-  const float rNs = 1.0f / (float)Ns;
+  const int tstep = qdiv(n, rcpf((float)(Ns * R)));
+  const float rNs = rcpf((float)Ns);
   const int lane = threadIdx.x & 63;
   const int col = g.k0 + p;
   const bool live = col < g.Nc;
@@ -381,15 +395,15 @@ __device__ __forceinline__ void wave_stage(double2 *buf, int n, int p, int Ns,
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         const int idx = j + m * nb;
-        double2 x;
         if constexpr (IO == IO_ZP) {
           const bool pad = (R % 2 == 0) ? (m >= R / 2) : (idx >= g.Nu);
-          x = (live && !pad) ? g.spec[(i64)col * g.Nu + idx] : make_double2(0.0, 0.0);
+          v[u][m] = (live && !pad) ? g.spec[(i64)col * g.Nu + idx] : make_double2(0.0, 0.0);
         } else {
-          x = buf[idx];
-          if (m > 0 && k > 0) x = cmul(x, twid(tw, k * m * tstep, SIGN));
+          v[u][m] = buf[idx];
         }
-        v[u][m] = x;
+      }
+      if constexpr (IO != IO_ZP) {
+        if (Ns > 1) twiddle_mul<R, SIGN>(v[u], tw, k * tstep);
       }
     }
   }
@@ -594,7 +608,7 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int e = threadIdx.x; e < D.ti * D.tj; e += blockDim.x) {
-    const int li = qdiv(e, 1.0f / (float)D.tj), lj = e - li * D.tj;
+    const int li = qdiv(e, rcpf((float)D.tj)), lj = e - li * D.tj;
     const int i = i0 + li, j = j0 + lj;
     if (i >= Nu || j >= Nphi) continue;
     const i64 c = (i64)i * Nphi + j;
@@ -645,7 +659,7 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   // all threads stage the rows (more loads in flight than a first stage
   // reading HBM through its few butterflies: measured faster here)
   for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
-    const int p = qdiv(e, 1.0f / (float)n), j = e - p * n;
+    const int p = qdiv(e, rcpf((float)n)), j = e - p * n;
     const int ra = row0 + 2 * p, rb = ra + 1;
     double xa = 0.0, xb = 0.0;
     if (ra < Nu) xa = R32[ra] * S[(i64)ra * n + j];
@@ -657,7 +671,7 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   double2 *out = spec + D.spo;
   const int w2 = 2 * pr;
   for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
-    const int k = qdiv(e, 1.0f / (float)w2), rr = e - k * w2;
+    const int k = qdiv(e, rcpf((float)w2)), rr = e - k * w2;
     const int p = rr >> 1, isb = rr & 1;
     const int row = row0 + rr;
     if (row >= Nu) continue;
@@ -713,7 +727,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
     return;
   }
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
-    const int c = qdiv(e, 1.0f / (float)M), p = e - c * M;
+    const int c = qdiv(e, rcpf((float)M)), p = e - c * M;
     const int k = k0 + c;
     double2 v = make_double2(0.0, 0.0);
     if (k < Nc && p < Nu) v = io[(i64)k * Nu + p];
@@ -723,7 +737,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   double2 *Z = lds_fft<-1>(A, B, M, cb, D.nst_col, D.rad_col, TW);
   double2 *Wb = (Z == A) ? B : A;
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
-    const int c = qdiv(e, 1.0f / (float)M), p = e - c * M;
+    const int c = qdiv(e, rcpf((float)M)), p = e - c * M;
     const int k = k0 + c;
     const int pp = (p <= Nu) ? p : M - p;
     const double kv = (k < Nc) ? kh[(i64)k * (Nu + 1) + pp] : 0.0;
@@ -735,7 +749,7 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
   __syncthreads();
   const double2 *Y = lds_fft<+1>(Z, Wb, M, cb, D.nst_col, D.rad_col, TW);
   for (int e = threadIdx.x; e < cb * Nu; e += blockDim.x) {
-    const int c = qdiv(e, 1.0f / (float)Nu), p = e - c * Nu;
+    const int c = qdiv(e, rcpf((float)Nu)), p = e - c * Nu;
     const int k = k0 + c;
     if (k < Nc) io[(i64)k * Nu + p] = Y[c * M + p];
   }
@@ -798,7 +812,7 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   const double2 *TW = tw_table(lds2 + 2 * pr * n, tw + D.two_row, n, D.tw_lds & 1);
   const double2 *in = spec + D.spo;
   for (int e = threadIdx.x; e < pr * Nc; e += blockDim.x) {
-    const int k = qdiv(e, 1.0f / (float)pr), p = e - k * pr;
+    const int k = qdiv(e, rcpf((float)pr)), p = e - k * pr;
     const int ra = row0 + 2 * p, rb = ra + 1;
     double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
     if (ra < Nu) xa = in[(i64)k * Nu + ra];
@@ -826,7 +840,7 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   }
   const double2 *Z = lds_fft<+1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
   for (int e = threadIdx.x; e < 2 * pr * n; e += blockDim.x) {
-    const int rr = qdiv(e, 1.0f / (float)n), j = e - rr * n;
+    const int rr = qdiv(e, rcpf((float)n)), j = e - rr * n;
     const int row = row0 + rr;
     if (row >= Nu) continue;
     const double2 z = Z[(rr >> 1) * n + j];
@@ -934,7 +948,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
 
   // ---- phase 1: source (+ forces) on rows i0-2..i0+TI+1, cols j0-2..j0+TJ+1
   for (int e = threadIdx.x; e < SH * SW; e += blockDim.x) {
-    const int lr = qdiv(e, 1.0f / (float)SW), lc = e - lr * SW;
+    const int lr = qdiv(e, rcpf((float)SW)), lc = e - lr * SW;
     const int i = i0 - 2 + lr;
     if (i < 0 || i >= Nu) continue;
     const int j = wrapj(j0 - 2 + lc, Nphi);
@@ -970,7 +984,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
                *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL);
   (void)tRf;
   for (int e = threadIdx.x; e < (TI + 1) * SW; e += blockDim.x) {
-    const int lf = qdiv(e, 1.0f / (float)SW), lc = e - lf * SW;
+    const int lf = qdiv(e, rcpf((float)SW)), lc = e - lf * SW;
     const int f = i0 + lf;
     double fm = 0.0, fu = 0.0, fw = 0.0;
     if (f >= 1 && f <= Nu - 1) {
@@ -1002,7 +1016,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
 
   // ---- phase 3: radial gather for rows i0..i0+TI-1 (in place into s*)
   for (int e = threadIdx.x; e < TI * SW; e += blockDim.x) {
-    const int lr = qdiv(e, 1.0f / (float)SW), lc = e - lr * SW;
+    const int lr = qdiv(e, rcpf((float)SW)), lc = e - lr * SW;
     const int i = i0 + lr;
     if (i >= Nu) continue;
     const int l = (lr + 2) * SW + lc;
@@ -1021,7 +1035,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   const double *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE);
   const int FW = TJ + 1;
   for (int e = threadIdx.x; e < TI * FW; e += blockDim.x) {
-    const int lr = qdiv(e, 1.0f / (float)FW), lf = e - lr * FW;
+    const int lr = qdiv(e, rcpf((float)FW)), lf = e - lr * FW;
     const int i = i0 + lr;
     double fm = 0.0, fu = 0.0, fw = 0.0;
     if (i < Nu) {
@@ -1055,7 +1069,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int e = threadIdx.x; e < TI * TJ; e += blockDim.x) {
-    const int lr = qdiv(e, 1.0f / (float)TJ), lc = e - lr * TJ;
+    const int lr = qdiv(e, rcpf((float)TJ)), lc = e - lr * TJ;
     const int i = i0 + lr, j = j0 + lc;
     if (i >= Nu || j >= Nphi) continue;
     const int l = (lr + 2) * SW + lc + 2;
@@ -1090,11 +1104,11 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
 // Register-streaming variant of k_update for bandwidth-bound (large or
 // batched) runs.  One wavefront per work unit = (disk, column group g, radial
 // strip k).  Lane L owns azimuthal column c = 60 g - 2 + L (periodic); lanes
-// 2..61 produce output.  The wave walks source rows s = i0-2 .. i1+1 of its
-// strip [i0, i1): the radial stencil (Sigma/Phi windows, the last four
-// post-source rows, the previous radial face) stays in registers, azimuthal
-// neighbours come from cross-lane shuffles, and row tables are wave-uniform.
-// Same arithmetic as k_update.
+// 2..61 produce output.  The wave walks source rows i0-2 .. i1+1 of its
+// strip [i0, i1) (down on even strips, up on odd ones): the radial stencil
+// (Sigma/Phi windows, the last four post-source rows, the previous radial
+// face) stays in registers, azimuthal neighbours come from cross-lane
+// shuffles, and row tables are wave-uniform.  Same arithmetic as k_update.
 #define STREAM_OUT 60
 
 // Whole-wavefront lane shifts by one through DPP (wave_shr:1 / wave_shl:1 on
@@ -1150,138 +1164,175 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   const bool sg = !GARR && D.selfgrav != 0;
 
   auto clampr = [&](int r) { return r < 0 ? 0 : (r >= Nu ? Nu - 1 : r); };
-  const int sbeg = i0 - 2, send = i1 + 1;
-  double Sm = S[(i64)clampr(sbeg - 1) * Nphi + c], S0 = S[(i64)clampr(sbeg) * Nphi + c];
-  double Pm = 0.0, P0 = 0.0;
-  if (sg) {
-    Pm = Ph[(i64)clampr(sbeg - 1) * Nphi + c];
-    P0 = Ph[(i64)clampr(sbeg) * Nphi + c];
-  }
-  // post-source ring: slot k holds row s-3+k after row s is pushed
-  double qs0 = 0, qs1 = 0, qs2 = 0, qs3 = 0, qu0 = 0, qu1 = 0, qu2 = 0, qu3 = 0;
-  double qw0 = 0, qw1 = 0, qw2 = 0, qw3 = 0;
-  double pfm = 0, pfu = 0, pfw = 0;     // previous radial face (f-1)
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 
-  // loads of the next row are issued one iteration ahead (rows clamped into
-  // the disk; values of rows outside it are never used)
-  double nU, nW, nSe, nSw, nPe = 0.0, nPw = 0.0, nSp, nPp = 0.0;
-  auto fetch = [&](int r) {
-    const i64 ro = (i64)clampr(r) * Nphi, rp = (i64)clampr(r + 1) * Nphi;
-    nU = U[ro + c];
-    nW = W[ro + c];
-    nSe = S[ro + cp];
-    nSw = S[ro + cm];
-    nSp = S[rp + c];
+  // Even strips walk down (rows i0-2 .. i1+1), odd strips walk up (i1+1 ..
+  // i0-2), so two neighbouring strips reach the rows they share at the same
+  // time -- both at their start or both at their end -- and the second read
+  // of those rows hits L2.  The arithmetic of every face and cell is the
+  // same either way; only the order of the partial sums differs.
+  auto strip = [&](auto down_tag) {
+    constexpr bool DOWN = decltype(down_tag)::value;
+    constexpr int STEP = DOWN ? 1 : -1;
+    const int sfirst = DOWN ? i0 - 2 : i1 + 1;   // the last row is i1 + 1 (down) or i0 - 2 (up)
+    // the row beyond the first one on the far side, and the first row
+    double Sb = S[(i64)clampr(sfirst - STEP) * Nphi + c], S0 = S[(i64)clampr(sfirst) * Nphi + c];
+    double Pb = 0.0, P0 = 0.0;
     if (sg) {
-      nPe = Ph[ro + cp];
-      nPw = Ph[ro + cm];
-      nPp = Ph[rp + c];
+      Pb = Ph[(i64)clampr(sfirst - STEP) * Nphi + c];
+      P0 = Ph[(i64)clampr(sfirst) * Nphi + c];
+    }
+    // post-source ring over rows f-2 .. f+1 of the face f in hand: slot k
+    // holds row f-2+k (rows enter at slot 3 walking down, at slot 0 walking up)
+    double qs0 = 0, qs1 = 0, qs2 = 0, qs3 = 0, qu0 = 0, qu1 = 0, qu2 = 0, qu3 = 0;
+    double qw0 = 0, qw1 = 0, qw2 = 0, qw3 = 0;
+    double pfm = 0, pfu = 0, pfw = 0;     // the face met before this one
+
+    // loads of the next row are issued one iteration ahead (rows clamped
+    // into the disk; values of rows outside it are never used)
+    double nU, nW, nSe, nSw, nPe = 0.0, nPw = 0.0, nSa, nPa = 0.0;
+    auto fetch = [&](int r) {
+      const i64 ro = (i64)clampr(r) * Nphi, ra = (i64)clampr(r + STEP) * Nphi;
+      nU = U[ro + c];
+      nW = W[ro + c];
+      nSe = S[ro + cp];
+      nSw = S[ro + cm];
+      nSa = S[ra + c];
+      if (sg) {
+        nPe = Ph[ro + cp];
+        nPw = Ph[ro + cm];
+        nPa = Ph[ra + c];
+      }
+    };
+    // rows in groups of four, so that the unrolled body renames the ring
+    // instead of moving it; the up to three rows past the last one compute
+    // nothing that is kept (their faces fall outside [i0, i1])
+    const int ngrp = (i1 - i0 + 4 + 3) >> 2;
+    fetch(sfirst);
+    int s = sfirst;
+    for (int grp = 0; grp < ngrp; ++grp)
+#pragma unroll
+    for (int k = 0; k < 4; ++k, s += STEP) {
+      // row s + STEP (ahead) and row s - STEP (behind)
+      const double Sa = nSa, Pa = nPa, u = nU, w = nW, Se = nSe, Sw = nSw, Pe = nPe, Pw = nPw;
+      fetch(s + STEP);
+      const double Sp = DOWN ? Sa : Sb, Sm = DOWN ? Sb : Sa;
+      const double Pp = DOWN ? Pa : Pb, Pm = DOWN ? Pb : Pa;
+      // ---- source at row s (hydro.hpp:73-96, forces kalnajs.hpp:65-80)
+      double ns_s = 0.0, ns_u = 0.0, ns_w = 0.0;
+      if (s >= 0 && s < Nu) {
+        const i64 ro = (i64)s * Nphi;
+        const double dPdR = (pres<G2>(D, Sp) - pres<G2>(D, Sm)) * tInvdRc[s];
+        const double dPdphi = (pres<G2>(D, Se) - pres<G2>(D, Sw)) * D.inv2dphi;
+        double gR = tGstar[s], gp = 0.0;
+        if (GARR) {
+          if (has_gR) gR += gRa[D.so + ro + c];
+          if (has_gp) gp += gpa[D.so + ro + c];
+        } else if (sg) {
+          gR += -(Pp - Pm) * tGrc[s];
+          gp += -(Pe - Pw) * tGpc[s];
+        }
+        const double rs = frcp(S0);
+        const double invR = tInvR[s];
+        const double aR = -dPdR * rs + gR + w * w * invR;
+        const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
+        ns_s = S0;
+        ns_u = u + dt * aR;
+        ns_w = w + dt * ap;
+      }
+      if (DOWN) {
+        qs0 = qs1; qs1 = qs2; qs2 = qs3; qs3 = ns_s;
+        qu0 = qu1; qu1 = qu2; qu2 = qu3; qu3 = ns_u;
+        qw0 = qw1; qw1 = qw2; qw2 = qw3; qw3 = ns_w;
+      } else {
+        qs3 = qs2; qs2 = qs1; qs1 = qs0; qs0 = ns_s;
+        qu3 = qu2; qu2 = qu1; qu1 = qu0; qu0 = ns_u;
+        qw3 = qw2; qw2 = qw1; qw1 = qw0; qw0 = ns_w;
+      }
+      Sb = S0; S0 = Sa;
+      Pb = P0; P0 = Pa;
+      // ---- radial face f between cells f-1 (slot 1) and f (slot 2) (hydro.hpp:111-130)
+      const int f = DOWN ? s - 1 : s + 2;
+      double fm = 0.0, fu = 0.0, fw = 0.0;
+      if (f >= i0 && f <= i1) {
+        if (f >= 1 && f <= Nu - 1) {
+          const double vf = 0.5 * (qu1 + qu2);
+          const bool up = vf > 0;
+          const int cell = up ? f - 1 : f;
+          const double bs = up ? qs1 : qs2, bu = up ? qu1 : qu2, bw = up ? qw1 : qw2;
+          const double ls = up ? qs0 : qs1, lu = up ? qu0 : qu1, lw = up ? qw0 : qw1;
+          const double hs = up ? qs2 : qs3, hu = up ? qu2 : qu3, hw = up ? qw2 : qw3;
+          const double fac = up ? 0.5 * (1.0 - vf * dt * tInvdR[f - 1])
+                                : -0.5 * (1.0 + vf * dt * tInvdR[f]);
+          double ss = bs, su = bu, sw = bw;
+          if (cell > 0 && cell < Nu - 1) {
+            ss += vanleer(hs - bs, bs - ls) * fac;
+            su += vanleer(hu - bu, bu - lu) * fac;
+            sw += vanleer(hw - bw, bw - lw) * fac;
+          }
+          fm = vf * ss * tAf[f];
+          fu = fm * su;
+          fw = fm * sw;
+        }
+        // ---- radial gather of the row g whose two faces are now known
+        // (faces g and g+1; walking down g = f-1, walking up g = f), then the
+        // azimuthal sweep
+        const int g = DOWN ? f - 1 : f;
+        if (g >= i0 && g < i1) {
+          const double s_ = DOWN ? qs1 : qs2, u_ = DOWN ? qu1 : qu2, w_ = DOWN ? qw1 : qw2;
+          const double flo = DOWN ? pfm : fm, fhi = DOWN ? fm : pfm;
+          const double ulo = DOWN ? pfu : fu, uhi = DOWN ? fu : pfu;
+          const double wlo = DOWN ? pfw : fw, whi = DOWN ? fw : pfw;
+          const double iv = dt * tInvVol[g];
+          const double ns = s_ + (flo - fhi) * iv;
+          const double nmR = s_ * u_ + (ulo - uhi) * iv;
+          const double nmP = s_ * w_ + (wlo - whi) * iv;
+          const double rn = frcp(smax(ns, 1e-10));
+          const double Xs = ns, Xu = nmR * rn, Xw = nmP * rn;
+          // azimuthal face at this lane's left edge (hydro.hpp:153-168)
+          const double sm1 = up1(Xs), um1 = up1(Xu), wm1 = up1(Xw);
+          const double sm2 = up1(sm1), um2 = up1(um1), wm2 = up1(wm1);
+          const double sp1 = dn1(Xs), vp1 = dn1(Xu), wp1 = dn1(Xw);
+          const double vf = 0.5 * (wm1 + Xw);
+          const bool upw = vf > 0;
+          const double bs = upw ? sm1 : Xs, bu = upw ? um1 : Xu, bw = upw ? wm1 : Xw;
+          const double ls = upw ? sm2 : sm1, lu = upw ? um2 : um1, lw = upw ? wm2 : wm1;
+          const double hs = upw ? Xs : sp1, hu = upw ? Xu : vp1, hw = upw ? Xw : wp1;
+          const double fac = upw ? 0.5 * (1.0 - vf * dt * tInvdl[g])
+                                 : -0.5 * (1.0 + vf * dt * tInvdl[g]);
+          const double ss = bs + vanleer(hs - bs, bs - ls) * fac;
+          const double su = bu + vanleer(hu - bu, bu - lu) * fac;
+          const double sw = bw + vanleer(hw - bw, bw - lw) * fac;
+          const double Fm = vf * ss * tAface[g];
+          const double FU = Fm * su, FW = Fm * sw;
+          const double Fm1 = dn1(Fm), FU1 = dn1(FU), FW1 = dn1(FW);
+          // azimuthal gather + floor (hydro.hpp:170-178, 68-69)
+          const double ns2 = Xs + (Fm - Fm1) * iv;
+          const double nmR2 = Xs * Xu + (FU - FU1) * iv;
+          const double nmP2 = Xs * Xw + (FW - FW1) * iv;
+          const double rn2 = frcp(smax(ns2, 1e-10));
+          const double so = smax(ns2, 1e-10), uo = nmR2 * rn2, wo = nmP2 * rn2;
+          if (out_ok) {
+            const i64 o = (i64)g * Nphi + c;
+            So[o] = so;
+            Uo[o] = uo;
+            Wo[o] = wo;
+            const double r = cfl_rate<G2>(D, so, uo, wo, tInvdR[g], tInvdl[g]);
+            mx = (r > mx) ? r : mx;
+            a0 += so * tdA[g];
+            a1 += so * c2j * tdA[g];
+            a2 -= so * s2j * tdA[g];
+            a3 += nonfinite(so, uo, wo);
+          }
+        }
+        pfm = fm; pfu = fu; pfw = fw;
+      }
     }
   };
-  fetch(sbeg);
-  for (int s = sbeg; s <= send; ++s) {
-    const double Sp = nSp, Pp = nPp, u = nU, w = nW, Se = nSe, Sw = nSw, Pe = nPe, Pw = nPw;
-    if (s < send) fetch(s + 1);
-    // ---- source at row s (hydro.hpp:73-96, forces kalnajs.hpp:65-80)
-    double ns_s = 0.0, ns_u = 0.0, ns_w = 0.0;
-    if (s >= 0 && s < Nu) {
-      const i64 ro = (i64)s * Nphi;
-      const double dPdR = (pres<G2>(D, Sp) - pres<G2>(D, Sm)) * tInvdRc[s];
-      const double dPdphi = (pres<G2>(D, Se) - pres<G2>(D, Sw)) * D.inv2dphi;
-      double gR = tGstar[s], gp = 0.0;
-      if (GARR) {
-        if (has_gR) gR += gRa[D.so + ro + c];
-        if (has_gp) gp += gpa[D.so + ro + c];
-      } else if (sg) {
-        gR += -(Pp - Pm) * tGrc[s];
-        gp += -(Pe - Pw) * tGpc[s];
-      }
-      const double rs = frcp(S0);
-      const double invR = tInvR[s];
-      const double aR = -dPdR * rs + gR + w * w * invR;
-      const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
-      ns_s = S0;
-      ns_u = u + dt * aR;
-      ns_w = w + dt * ap;
-    }
-    qs0 = qs1; qs1 = qs2; qs2 = qs3; qs3 = ns_s;
-    qu0 = qu1; qu1 = qu2; qu2 = qu3; qu3 = ns_u;
-    qw0 = qw1; qw1 = qw2; qw2 = qw3; qw3 = ns_w;
-    Sm = S0; S0 = Sp;
-    Pm = P0; P0 = Pp;
-    // ---- radial face f = s-1 between cells f-1 (slot 1) and f (slot 2) (hydro.hpp:111-130)
-    const int f = s - 1;
-    double fm = 0.0, fu = 0.0, fw = 0.0;
-    if (f >= i0 && f <= i1) {
-      if (f >= 1 && f <= Nu - 1) {
-        const double vf = 0.5 * (qu1 + qu2);
-        const bool up = vf > 0;
-        const int cell = up ? f - 1 : f;
-        const double bs = up ? qs1 : qs2, bu = up ? qu1 : qu2, bw = up ? qw1 : qw2;
-        const double ls = up ? qs0 : qs1, lu = up ? qu0 : qu1, lw = up ? qw0 : qw1;
-        const double hs = up ? qs2 : qs3, hu = up ? qu2 : qu3, hw = up ? qw2 : qw3;
-        const double fac = up ? 0.5 * (1.0 - vf * dt * tInvdR[f - 1])
-                              : -0.5 * (1.0 + vf * dt * tInvdR[f]);
-        double ss = bs, su = bu, sw = bw;
-        if (cell > 0 && cell < Nu - 1) {
-          ss += vanleer(hs - bs, bs - ls) * fac;
-          su += vanleer(hu - bu, bu - lu) * fac;
-          sw += vanleer(hw - bw, bw - lw) * fac;
-        }
-        fm = vf * ss * tAf[f];
-        fu = fm * su;
-        fw = fm * sw;
-      }
-      // ---- radial gather of row g = f-1 (slot 1), then the azimuthal sweep
-      const int g = f - 1;
-      if (g >= i0 && g < i1) {
-        const double s_ = qs1;
-        const double iv = dt * tInvVol[g];
-        const double ns = s_ + (pfm - fm) * iv;
-        const double nmR = s_ * qu1 + (pfu - fu) * iv;
-        const double nmP = s_ * qw1 + (pfw - fw) * iv;
-        const double rn = frcp(smax(ns, 1e-10));
-        const double Xs = ns, Xu = nmR * rn, Xw = nmP * rn;
-        // azimuthal face at this lane's left edge (hydro.hpp:153-168)
-        const double sm1 = up1(Xs), um1 = up1(Xu), wm1 = up1(Xw);
-        const double sm2 = up1(sm1), um2 = up1(um1), wm2 = up1(wm1);
-        const double sp1 = dn1(Xs), vp1 = dn1(Xu), wp1 = dn1(Xw);
-        const double vf = 0.5 * (wm1 + Xw);
-        const bool upw = vf > 0;
-        const double bs = upw ? sm1 : Xs, bu = upw ? um1 : Xu, bw = upw ? wm1 : Xw;
-        const double ls = upw ? sm2 : sm1, lu = upw ? um2 : um1, lw = upw ? wm2 : wm1;
-        const double hs = upw ? Xs : sp1, hu = upw ? Xu : vp1, hw = upw ? Xw : wp1;
-        const double fac = upw ? 0.5 * (1.0 - vf * dt * tInvdl[g])
-                               : -0.5 * (1.0 + vf * dt * tInvdl[g]);
-        const double ss = bs + vanleer(hs - bs, bs - ls) * fac;
-        const double su = bu + vanleer(hu - bu, bu - lu) * fac;
-        const double sw = bw + vanleer(hw - bw, bw - lw) * fac;
-        const double Fm = vf * ss * tAface[g];
-        const double FU = Fm * su, FW = Fm * sw;
-        const double Fm1 = dn1(Fm), FU1 = dn1(FU), FW1 = dn1(FW);
-        // azimuthal gather + floor (hydro.hpp:170-178, 68-69)
-        const double ns2 = Xs + (Fm - Fm1) * iv;
-        const double nmR2 = Xs * Xu + (FU - FU1) * iv;
-        const double nmP2 = Xs * Xw + (FW - FW1) * iv;
-        const double rn2 = frcp(smax(ns2, 1e-10));
-        const double so = smax(ns2, 1e-10), uo = nmR2 * rn2, wo = nmP2 * rn2;
-        if (out_ok) {
-          const i64 o = (i64)g * Nphi + c;
-          So[o] = so;
-          Uo[o] = uo;
-          Wo[o] = wo;
-          const double r = cfl_rate<G2>(D, so, uo, wo, tInvdR[g], tInvdl[g]);
-          mx = (r > mx) ? r : mx;
-          a0 += so * tdA[g];
-          a1 += so * c2j * tdA[g];
-          a2 -= so * s2j * tdA[g];
-          a3 += nonfinite(so, uo, wo);
-        }
-      }
-    }
-    if (f >= i0 && f <= i1) { pfm = fm; pfu = fu; pfw = fw; }
-  }
+  if (ks & 1)
+    strip(std::integral_constant<bool, false>{});
+  else
+    strip(std::integral_constant<bool, true>{});
   // wave reduction of the partials (fixed order)
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {

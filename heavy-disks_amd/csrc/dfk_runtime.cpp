@@ -179,7 +179,15 @@ int build_layout(dfk_handle *h) {
     const int nt = D.ntile_i * D.ntile_j;
     part += nt;
     h->upd0[d] = (int)mu.size();
-    for (int t = 0; t < nt; ++t) mu.push_back(make_int2(d, t));
+    if (h->upd_kind == 1) {
+      // streaming units t = column group * strips + strip, listed strip by
+      // strip: neighbouring column groups of a strip run side by side on
+      // one XCD and share the lines at their common edge
+      for (int k = 0; k < D.ntile_i; ++k)
+        for (int g = 0; g < D.ntile_j; ++g) mu.push_back(make_int2(d, g * D.ntile_i + k));
+    } else {
+      for (int t = 0; t < nt; ++t) mu.push_back(make_int2(d, t));
+    }
     size_t l = (size_t)(3 * (D.ti + 4) * (D.tj + 4) + 3 * (D.ti + 1) * (D.tj + 4)) * sizeof(double);
     if (h->upd_kind == 0) lu = std::max(lu, l);
     D.pr = std::max(1, std::min(D.pr, ceil_div(D.Nu, 2)));

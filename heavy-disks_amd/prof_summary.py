@@ -1,7 +1,7 @@
 """Summarise rocprofv3 CSV output for the loop kernels.
 
     python heavy-disks_amd/prof_summary.py stats DIR            # --kernel-trace --stats run
-    python heavy-disks_amd/prof_summary.py pmc DIR [DIR ...]    # --pmc runs (FETCH_SIZE, WRITE_SIZE, ...)
+    python heavy-disks_amd/prof_summary.py pmc DIR [DIR ...] [--largest]   # --pmc runs
     python heavy-disks_amd/prof_summary.py traffic OUT.json WORKLOAD FETCH_DIR WRITE_DIR \
         COPY_FETCH_DIR COPY_WRITE_DIR COPY_MIB [--largest]
     python heavy-disks_amd/prof_summary.py largest DIR          # --kernel-trace run
@@ -42,12 +42,18 @@ def stats(d):
               f"{float(r['Percentage']):.1f} |")
 
 
-def pmc(dirs):
+def pmc(dirs, largest=False):
     agg = collections.defaultdict(lambda: collections.defaultdict(list))
     for d in dirs:
         f = glob.glob(os.path.join(d, "*counter_collection.csv"))[0]
-        for r in csv.DictReader(open(f)):
-            agg[short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        rows = list(csv.DictReader(open(f)))
+        big = collections.defaultdict(int)
+        for r in rows:
+            big[short(r["Kernel_Name"])] = max(big[short(r["Kernel_Name"])], int(r["Grid_Size"]))
+        for r in rows:
+            k = short(r["Kernel_Name"])
+            if not largest or int(r["Grid_Size"]) == big[k]:
+                agg[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
     names = sorted({c for v in agg.values() for c in v})
     print("| kernel | dispatches | " + " | ".join(names) + " |")
     print("|---|---|" + "---|" * len(names))
@@ -117,4 +123,4 @@ if __name__ == "__main__":
     elif sys.argv[1] == "largest":
         largest_stats(sys.argv[2])
     else:
-        pmc(sys.argv[2:])
+        pmc([a for a in sys.argv[2:] if a != "--largest"], largest="--largest" in sys.argv[2:])
