@@ -194,9 +194,10 @@ def main():
         if dom == "update":   # the loop's build: no force arrays, kernel of the tuned kind
             pre = "k_update_stream<false" if r["shapes"].get("kind") == 1 else "k_update<false"
             names = [k for k in tk if k.startswith(pre)]
-        elif dom == "cols_conv":  # cooperative or wave-local column kernel, as tuned
-            wave = r["shapes"].get("fft", [0, 0, 0])[2]
-            names = [k for k in tk if k == ("k_cols_conv_wave" if wave else "k_cols_conv")]
+        elif dom == "cols_conv":  # the column kernel of the tuned mode
+            pre = {0: "k_cols_conv", 1: "k_cols_conv_wave", 2: "k_cols_conv_group<"}[
+                r["shapes"].get("fft", [0, 0, 0])[2]]
+            names = [k for k in tk if k == pre or (pre.endswith("<") and k.startswith(pre))]
         else:
             names = [k for k in tk if k == "k_" + dom]
         traffic = tk[names[0]]["traffic_bytes"] if names else None

@@ -128,18 +128,22 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
         ms = prof(base[0], base[1], pr, base[3])
         res["pr"][str(pr)] = float(ms[1] + ms[3])
     # column batches: twiddles in LDS or through the caches; workgroup-
-    # cooperative columns (ping-pong buffers) or one wave per column in place
+    # cooperative columns (ping-pong buffers), one wave per column in place,
+    # or the workgroup in place on one column after another
     thr0 = e.get_threads()
     M, Nc = 2 * Nu, Nphi // 2 + 1
     cbs_wave = [c for c in (1, 2, 4, 8, 16) if (c + 1) * M * 16 <= LDS_MAX and c <= Nc]
-    for wave in (0, 1):
+    cbs_group = [c for c in (1, 2, 4) if c <= Nc]
+    for wave in (0, 1, 2):
         for twl in (1, 0):
             e.set_fft_options(1, twl, wave)
             if wave and e.get_fft_options(0)[2] == 0:
-                break                  # this length has no wave-local transform
-            for cb in (cbs_wave if wave else cbs):
-                if wave:
+                break                  # this length has no in-place transform
+            for cb in {0: cbs, 1: cbs_wave, 2: cbs_group}[wave]:
+                if wave == 1:
                     e.set_threads(thr0[0], min(256, 64 * cb), thr0[2])
+                elif wave == 2:
+                    e.set_threads(thr0[0], 128, thr0[2])
                 res["cb"][f"{cb}:{twl}:{wave}"] = float(prof(base[0], base[1], base[2], cb)[2])
             e.set_threads(*thr0)
     bt = min(res["tile"], key=res["tile"].get)

@@ -64,8 +64,9 @@ struct DiskDev {
   int cos_off;         // offset of cos(2 phi_j)/sin(2 phi_j) tables (Nphi each) in trig pool
   int tw_lds;          // bit 0: row transforms, bit 1: column convolution stage their
                        // twiddles in LDS (else read them through the caches)
-  int conv_wave;       // column convolution: one wave per column, in place (else the
-                       // workgroup cooperates on each column, ping-pong buffers)
+  int conv_wave;       // column convolution: 0 the workgroup cooperates on its columns
+                       // (ping-pong buffers), 1 one wave per column in place, 2 the
+                       // workgroup in place on one column after another
   int pad2_;
 };
 

@@ -360,13 +360,15 @@ __device__ __forceinline__ void io_stage_r(int r, const double2 *a, double2 *b, 
   else io_stage<3, SIGN, IO>(a, b, n, batch, Ns, tw, g);
 }
 
-// ------------------------------------------------------------------ wave-local column transforms
-// Throughput-regime variant of the column convolution: one wavefront owns a
-// whole column and transforms it in place in a single LDS buffer of M
-// elements (no ping-pong buffer, no workgroup barrier).  In each stage every
-// lane loads the inputs of all its butterflies (at most 1024 / 64 = 16
-// complex values for M <= 1024), waits for the loads, then computes and
-// stores: no lane overwrites a value another lane still has to read.  Same
+// ------------------------------------------------------------------ in-place column transforms
+// Throughput-regime variants of the column convolution: a group of TG
+// threads (one wavefront, or the whole workgroup) owns a column and
+// transforms it in place in a single LDS buffer of M elements (no ping-pong
+// buffer: half the LDS of the cooperative kernel per column).  In each stage
+// every thread loads the inputs of all its butterflies (at most 1024 / TG
+// radix-R butterflies for M <= 1024), the group synchronises, then every
+// thread computes and stores: no thread overwrites a value another one still
+// has to read.  A wavefront synchronises without a workgroup barrier.  Same
 // butterflies, twiddles and I/O variants as io_stage (results agree to
 // round-off; the compiler's multiply-add contraction may differ).
 __device__ __forceinline__ void wave_lds_sync() {
@@ -375,20 +377,26 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-template <int R, int SIGN, int IO>
-__device__ __forceinline__ void wave_stage(double2 *buf, int n, int p, int Ns,
-                                           const double2 *__restrict__ tw, const StageIO &g) {
-  constexpr int EMAX = (1024 / R + 63) / 64;   // butterflies per lane at n <= 1024
+template <int TG>
+__device__ __forceinline__ void group_sync() {
+  if constexpr (TG == 64) wave_lds_sync();
+  else __syncthreads();
+}
+
+// One stage for thread `lane` (0 .. TG-1) of the group.
+template <int R, int SIGN, int IO, int TG>
+__device__ __forceinline__ void ip_stage(double2 *buf, int n, int p, int Ns, int lane,
+                                         const double2 *__restrict__ tw, const StageIO &g) {
+  constexpr int EMAX = (1024 / R + TG - 1) / TG;   // butterflies per thread at n <= 1024
   const int nb = n / R;
   const int tstep = qdiv(n, rcpf((float)(Ns * R)));
   const float rNs = rcpf((float)Ns);
-  const int lane = threadIdx.x & 63;
   const int col = g.k0 + p;
   const bool live = col < g.Nc;
   double2 v[EMAX][R];
 #pragma unroll
   for (int u = 0; u < EMAX; ++u) {
-    const int j = lane + 64 * u;
+    const int j = lane + TG * u;
     if (j < nb) {
       const int jq = qdiv(j, rNs);
       const int k = j - jq * Ns;
@@ -407,10 +415,10 @@ __device__ __forceinline__ void wave_stage(double2 *buf, int n, int p, int Ns,
       }
     }
   }
-  wave_lds_sync();
+  group_sync<TG>();
 #pragma unroll
   for (int u = 0; u < EMAX; ++u) {
-    const int j = lane + 64 * u;
+    const int j = lane + TG * u;
     if (j < nb) {
       const int jq = qdiv(j, rNs);
       const int k = j - jq * Ns;
@@ -435,16 +443,37 @@ __device__ __forceinline__ void wave_stage(double2 *buf, int n, int p, int Ns,
       }
     }
   }
-  wave_lds_sync();
+  group_sync<TG>();
 }
 
-template <int SIGN, int IO>
-__device__ __forceinline__ void wave_stage_r(int r, double2 *buf, int n, int p, int Ns,
-                                             const double2 *tw, const StageIO &g) {
-  if (r == 8) wave_stage<8, SIGN, IO>(buf, n, p, Ns, tw, g);
-  else if (r == 4) wave_stage<4, SIGN, IO>(buf, n, p, Ns, tw, g);
-  else if (r == 2) wave_stage<2, SIGN, IO>(buf, n, p, Ns, tw, g);
-  else wave_stage<3, SIGN, IO>(buf, n, p, Ns, tw, g);
+template <int SIGN, int IO, int TG>
+__device__ __forceinline__ void ip_stage_r(int r, double2 *buf, int n, int p, int Ns, int lane,
+                                           const double2 *tw, const StageIO &g) {
+  if (r == 8) ip_stage<8, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
+  else if (r == 4) ip_stage<4, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
+  else if (r == 2) ip_stage<2, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
+  else ip_stage<3, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
+}
+
+// The whole in-place convolution of column p (batch entry p of the
+// workgroup) in buf by a group of TG threads.
+template <int TG>
+__device__ __forceinline__ void ip_conv_column(const DiskDev &D, double2 *buf, int p, int lane,
+                                               const double2 *TW, const StageIO &g) {
+  const int M = D.M, ns = D.nst_col;
+  ip_stage_r<-1, IO_ZP, TG>(D.rad_col[0], buf, M, p, 1, lane, TW, g);
+  int Ns = D.rad_col[0];
+  for (int q = 1; q < ns - 1; ++q) {
+    ip_stage_r<-1, IO_LDS, TG>(D.rad_col[q], buf, M, p, Ns, lane, TW, g);
+    Ns *= D.rad_col[q];
+  }
+  ip_stage_r<-1, IO_KHAT, TG>(D.rad_col[ns - 1], buf, M, p, Ns, lane, TW, g);
+  Ns = 1;
+  for (int q = 0; q < ns - 1; ++q) {
+    ip_stage_r<+1, IO_LDS, TG>(D.rad_col[q], buf, M, p, Ns, lane, TW, g);
+    Ns *= D.rad_col[q];
+  }
+  ip_stage_r<+1, IO_OUT, TG>(D.rad_col[ns - 1], buf, M, p, Ns, lane, TW, g);
 }
 
 // Whether a transform can use the global-I/O stages.
@@ -775,24 +804,34 @@ k_cols_conv_wave(const DiskDev *__restrict__ disks, const int2 *__restrict__ map
   const double2 *TW = tw_table(lds2 + cb * M, tw + D.two_col, M, D.tw_lds & 2);
   StageIO g{};
   g.spec = spec + D.spo; g.kh = khat + D.kho; g.Nu = Nu; g.Nc = Nc; g.k0 = k0;
-  const int ns = D.nst_col, nw = blockDim.x >> 6, w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
   __syncthreads();   // the twiddle table, if staged in LDS
-  for (int p = w; p < cb; p += nw) {   // wave w: columns w, w + nw, ...
-    double2 *buf = A + p * M;
-    wave_stage_r<-1, IO_ZP>(D.rad_col[0], buf, M, p, 1, TW, g);
-    int Ns = D.rad_col[0];
-    for (int q = 1; q < ns - 1; ++q) {
-      wave_stage_r<-1, IO_LDS>(D.rad_col[q], buf, M, p, Ns, TW, g);
-      Ns *= D.rad_col[q];
-    }
-    wave_stage_r<-1, IO_KHAT>(D.rad_col[ns - 1], buf, M, p, Ns, TW, g);
-    Ns = 1;
-    for (int q = 0; q < ns - 1; ++q) {
-      wave_stage_r<+1, IO_LDS>(D.rad_col[q], buf, M, p, Ns, TW, g);
-      Ns *= D.rad_col[q];
-    }
-    wave_stage_r<+1, IO_OUT>(D.rad_col[ns - 1], buf, M, p, Ns, TW, g);
-  }
+  for (int p = w; p < cb; p += nw)   // wave w: columns w, w + nw, ...
+    ip_conv_column<64>(D, A + p * M, p, threadIdx.x & 63, TW, g);
+}
+
+// Column convolution in place by the whole workgroup (TG = blockDim.x
+// threads), its cb columns one after another through one LDS buffer of M
+// elements (then the twiddle table if staged): with half the LDS of the
+// cooperative kernel per column and at most 1024 / TG butterflies per thread,
+// more columns fit on a CU at once.  Same radix limits as the wave-local
+// kernel.
+template <int TG>
+__global__ void __launch_bounds__(TG)
+k_cols_conv_group(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+                  const DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
+                  const double2 *__restrict__ tw, double2 *__restrict__ spec) {
+  extern __shared__ double2 lds2[];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const int M = D.M, Nc = D.Nc, cb = D.cb;
+  const int k0 = wm.y * cb;
+  const double2 *TW = tw_table(lds2 + M, tw + D.two_col, M, D.tw_lds & 2);
+  StageIO g{};
+  g.spec = spec + D.spo; g.kh = khat + D.kho; g.Nu = D.Nu; g.Nc = Nc; g.k0 = k0;
+  __syncthreads();   // the twiddle table, if staged in LDS
+  for (int p = 0; p < cb && k0 + p < Nc; ++p) ip_conv_column<TG>(D, lds2, p, threadIdx.x, TW, g);
 }
 
 // Row c2r (FFTW convention: imaginary parts of the DC and Nyquist bins are
@@ -1527,6 +1566,18 @@ hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, c
   if (nblk <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cols_conv_wave, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat,
                      tw, spec);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
+                                 const int2 *map, int base, const DiskCtl *ctl, const double *khat,
+                                 const double2 *tw, double2 *spec) {
+  if (nblk <= 0) return hipSuccess;
+  auto k = nthr == 64 ? k_cols_conv_group<64>
+         : nthr == 128 ? k_cols_conv_group<128>
+         : nthr == 192 ? k_cols_conv_group<192> : k_cols_conv_group<256>;
+  if (nthr != 64 && nthr != 128 && nthr != 192 && nthr != 256) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec);
   return hipGetLastError();
 }
 

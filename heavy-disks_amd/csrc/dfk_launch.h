@@ -24,6 +24,9 @@ hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const 
 hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                                 const int2 *map, int base, const DiskCtl *ctl, const double *khat,
                                 const double2 *tw, double2 *spec);
+hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
+                                 const int2 *map, int base, const DiskCtl *ctl, const double *khat,
+                                 const double2 *tw, double2 *spec);
 hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
                           const double *geom, const double2 *tw, double *phi);

@@ -110,9 +110,10 @@ struct dfk_handle {
   bool layout_ok = false;
   double *pcfl = nullptr, *pcm = nullptr;
   i64 ntiles = 0;
-  int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr, *map_convw = nullptr;
-  std::vector<int> upd0, r2c0, conv0, convw0;   // per-disk first block (size nd+1)
-  size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0, lds_convw = 0;
+  int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr, *map_convw = nullptr,
+       *map_convg = nullptr;
+  std::vector<int> upd0, r2c0, conv0, convw0, convg0;   // per-disk first block (size nd+1)
+  size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0, lds_convw = 0, lds_convg = 0;
   size_t lds_max = 64 * 1024;   // per workgroup, from the device at create (gfx950: 160 KiB)
   DiskCtl *ctl = nullptr;
   // run
@@ -160,13 +161,15 @@ int build_layout(dfk_handle *h) {
   if (h->layout_ok) return DFK_OK;
   drop_graphs(h);
   h->run_ready = false;
-  std::vector<int2> mu, mr, mc, mcw;   // update, rows, columns, wave-local columns
+  // update, rows, columns: cooperative, wave-local and workgroup in-place
+  std::vector<int2> mu, mr, mc, mcw, mcg;
   h->upd0.assign(h->nd + 1, 0);
   h->r2c0.assign(h->nd + 1, 0);
   h->conv0.assign(h->nd + 1, 0);
   h->convw0.assign(h->nd + 1, 0);
+  h->convg0.assign(h->nd + 1, 0);
   i64 part = 0;
-  size_t lu = 0, lr = 0, lc = 0, lcw = 0;
+  size_t lu = 0, lr = 0, lc = 0, lcw = 0, lcg = 0;
   const size_t kMaxLds = h->lds_max;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
@@ -195,6 +198,7 @@ int build_layout(dfk_handle *h) {
     h->r2c0[d] = (int)mr.size();
     h->conv0[d] = (int)mc.size();
     h->convw0[d] = (int)mcw.size();
+    h->convg0[d] = (int)mcg.size();
     if (D.selfgrav) {
       // batches shrink until they fit the LDS budget (a cached shape from a
       // build with smaller LDS needs stays usable)
@@ -202,34 +206,41 @@ int build_layout(dfk_handle *h) {
       auto lds_rows = [&](int pr) {
         return (size_t)(2 * pr + (D.tw_lds & 1)) * D.Nphi * sizeof(double2);
       };
-      auto lds_cols = [&](int cb) {   // wave-local columns are transformed in place
-        return (size_t)((D.conv_wave ? 1 : 2) * cb + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
+      // in-place columns need one buffer per column (the workgroup kernel
+      // one for all its columns, done one after another), cooperative ones two
+      auto lds_cols = [&](int cb) {
+        const int nbuf = D.conv_wave == 2 ? 1 : (D.conv_wave == 1 ? cb : 2 * cb);
+        return (size_t)(nbuf + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
       };
       while (D.pr > 1 && lds_rows(D.pr) > kMaxLds) --D.pr;
       while (D.cb > 1 && lds_cols(D.cb) > kMaxLds) --D.cb;
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
-      std::vector<int2> &cm = D.conv_wave ? mcw : mc;
+      std::vector<int2> &cm = D.conv_wave == 2 ? mcg : (D.conv_wave == 1 ? mcw : mc);
       for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) cm.push_back(make_int2(d, g));
       lr = std::max(lr, lds_rows(D.pr));
-      (D.conv_wave ? lcw : lc) = std::max(D.conv_wave ? lcw : lc, lds_cols(D.cb));
+      size_t &lcol = D.conv_wave == 2 ? lcg : (D.conv_wave == 1 ? lcw : lc);
+      lcol = std::max(lcol, lds_cols(D.cb));
     }
   }
   h->upd0[h->nd] = (int)mu.size();
   h->r2c0[h->nd] = (int)mr.size();
   h->conv0[h->nd] = (int)mc.size();
   h->convw0[h->nd] = (int)mcw.size();
-  if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds || lcw > kMaxLds) {
-    set_err("tile/batch shapes need %zu/%zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, lcw, kMaxLds);
+  h->convg0[h->nd] = (int)mcg.size();
+  if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds || lcw > kMaxLds || lcg > kMaxLds) {
+    set_err("tile/batch shapes need %zu/%zu/%zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, lcw, lcg,
+            kMaxLds);
     return DFK_EINVAL;
   }
   h->lds_upd = lu;
   h->lds_r2c = lr;
   h->lds_conv = lc;
   h->lds_convw = lcw;
+  h->lds_convg = lcg;
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv);
-  hipFree(h->map_convw);
+  hipFree(h->map_convw); hipFree(h->map_convg);
   h->pcfl = h->pcm = nullptr;
-  h->map_upd = h->map_r2c = h->map_conv = h->map_convw = nullptr;
+  h->map_upd = h->map_r2c = h->map_conv = h->map_convw = h->map_convg = nullptr;
   h->ntiles = part;
   TRY(dalloc(&h->pcfl, part));
   TRY(dalloc(&h->pcm, 4 * part));
@@ -237,6 +248,7 @@ int build_layout(dfk_handle *h) {
   TRY(dalloc(&h->map_r2c, (i64)mr.size()));
   TRY(dalloc(&h->map_conv, (i64)mc.size()));
   TRY(dalloc(&h->map_convw, (i64)mcw.size()));
+  TRY(dalloc(&h->map_convg, (i64)mcg.size()));
   // unit calls launch one disk's range [upd0[d], upd0[d+1]) of the map, so
   // each disk's range is reordered on its own
   auto per_disk = [&](std::vector<int2> &m, const std::vector<int> &first) {
@@ -251,17 +263,21 @@ int build_layout(dfk_handle *h) {
     mr = xcd_order(mr);
     mc = xcd_order(mc);
     mcw = xcd_order(mcw);
+    mcg = xcd_order(mcg);
   } else {
     per_disk(mu, h->upd0);
     per_disk(mr, h->r2c0);
     per_disk(mc, h->conv0);
     per_disk(mcw, h->convw0);
+    per_disk(mcg, h->convg0);
   }
   HIPCHK(hipMemcpy(h->map_upd, mu.data(), sizeof(int2) * mu.size(), hipMemcpyHostToDevice));
   if (!mr.empty()) HIPCHK(hipMemcpy(h->map_r2c, mr.data(), sizeof(int2) * mr.size(), hipMemcpyHostToDevice));
   if (!mc.empty()) HIPCHK(hipMemcpy(h->map_conv, mc.data(), sizeof(int2) * mc.size(), hipMemcpyHostToDevice));
   if (!mcw.empty())
     HIPCHK(hipMemcpy(h->map_convw, mcw.data(), sizeof(int2) * mcw.size(), hipMemcpyHostToDevice));
+  if (!mcg.empty())
+    HIPCHK(hipMemcpy(h->map_convg, mcg.data(), sizeof(int2) * mcg.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dD, h->hD.data(), sizeof(DiskDev) * h->nd, hipMemcpyHostToDevice));
   h->layout_ok = true;
   return DFK_OK;
@@ -305,11 +321,15 @@ hipError_t launch_r2c(dfk_handle *h, int nblk, int base, int par, int nctl) {
 hipError_t launch_conv(dfk_handle *h, int d) {
   const int a = d < 0 ? 0 : d, b = d < 0 ? h->nd : d + 1;
   const int n = h->conv0[b] - h->conv0[a], nw = h->convw0[b] - h->convw0[a];
+  const int ng = h->convg0[b] - h->convg0[a];
   hipError_t e = dfk_launch_conv(h->stream, n, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[a],
                                  h->ctl, h->khat, h->tw, h->spec);
   if (e != hipSuccess) return e;
-  return dfk_launch_conv_wave(h->stream, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
-                              h->convw0[a], h->ctl, h->khat, h->tw, h->spec);
+  e = dfk_launch_conv_wave(h->stream, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
+                           h->convw0[a], h->ctl, h->khat, h->tw, h->spec);
+  if (e != hipSuccess) return e;
+  return dfk_launch_conv_group(h->stream, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
+                               h->convg0[a], h->ctl, h->khat, h->tw, h->spec);
 }
 
 // Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk,
@@ -688,6 +708,7 @@ void dfk_destroy(dfk_handle *h) {
   hipFree(h->dD); hipFree(h->st); hipFree(h->phi); hipFree(h->gR); hipFree(h->gp); hipFree(h->F);
   hipFree(h->spec); hipFree(h->khat); hipFree(h->geom); hipFree(h->trig); hipFree(h->tw);
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv); hipFree(h->map_convw);
+  hipFree(h->map_convg);
   hipFree(h->ctl); hipFree(h->samples);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
@@ -1067,13 +1088,13 @@ int dfk_set_fft_options(dfk_handle *h, int d, int rows_tw_lds, int cols_tw_lds, 
   TRY(check_disk(h, d));
   DiskDev &D = h->hD[d];
   const int v = (rows_tw_lds ? 1 : 0) | (cols_tw_lds ? 2 : 0);
-  // the wave-local column transform needs radices 2/3/4/8, >= 2 stages, M <= 1024
+  // the in-place column transforms need radices 2/3/4/8, >= 2 stages, M <= 1024
   bool fast = D.nst_col >= 2 && D.M <= 1024;
   for (int q = 0; q < D.nst_col; ++q) {
     const int r = D.rad_col[q];
     fast = fast && (r == 2 || r == 3 || r == 4 || r == 8);
   }
-  const int w = (cols_wave && fast && D.selfgrav) ? 1 : 0;
+  const int w = (fast && D.selfgrav && (cols_wave == 1 || cols_wave == 2)) ? cols_wave : 0;
   if (v != D.tw_lds || w != D.conv_wave) {
     D.tw_lds = v;
     D.conv_wave = w;

@@ -277,8 +277,9 @@ class Ensemble:
 
     def set_fft_options(self, rows_tw_lds=1, cols_tw_lds=1, cols_wave=0, d=None):
         """Twiddle tables of the row / column transforms in LDS (1) or read
-        through the caches (0); wave-local in-place column transforms (1) or
-        workgroup-cooperative ones (0); see dfk_set_fft_options."""
+        through the caches (0); column transforms workgroup-cooperative (0),
+        wave-local in place (1) or workgroup in place (2); see
+        dfk_set_fft_options."""
         for q in (range(self.n) if d is None else [d]):
             _check(lib().dfk_set_fft_options(self.h, q, int(rows_tw_lds), int(cols_tw_lds),
                                              int(cols_wave)))

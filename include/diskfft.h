@@ -149,13 +149,15 @@ int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
  *       convolution read their twiddle table: 1 = staged in LDS per
  *       workgroup (default; fastest when few workgroups run), 0 = from HBM
  *       through L1/L2 (less LDS per workgroup, more workgroups per CU);
- *   cols_wave  1 = each column of the convolution is transformed by one
- *       wavefront, in place in one LDS buffer, without workgroup barriers
- *       (throughput regime; only for 2Nu <= 1024 with radices 2, 3, 4, 8,
- *       otherwise ignored and read back as 0); 0 = the workgroup cooperates
- *       on its columns (default).
- * The twiddle placement does not change a bit of the results; the
- * wave-local kernel agrees with the cooperative one to round-off. */
+ *   cols_wave  how the column convolution is spread over threads:
+ *       0 = the workgroup cooperates on its columns with two LDS buffers per
+ *       column (default); 1 = each column is transformed by one wavefront,
+ *       in place in one LDS buffer, without workgroup barriers; 2 = the
+ *       workgroup transforms its columns one after another, in place in a
+ *       single LDS buffer.  1 and 2 need 2Nu <= 1024 with radices 2, 3, 4,
+ *       8 (otherwise the value is ignored and read back as 0).
+ * The twiddle placement does not change a bit of the results; the in-place
+ * kernels agree with the cooperative one to round-off. */
 int dfk_set_fft_options(dfk_handle *h, int disk, int rows_tw_lds, int cols_tw_lds, int cols_wave);
 int dfk_get_fft_options(dfk_handle *h, int disk, int *rows_tw_lds, int *cols_tw_lds,
                         int *cols_wave);

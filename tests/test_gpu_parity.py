@@ -91,15 +91,18 @@ def test_potential_large_batches(orc, dfk, Nu, Nphi, pr, cb):
 @pytest.mark.parametrize("Nu,Nphi,wave_ok", [(256, 256, 1), (35, 40, 0), (96, 80, 1), (33, 26, 0),
                                              (512, 1024, 1), (384, 512, 1), (3, 4, 1)])
 @pytest.mark.parametrize("opts,cb,threads", [((0, 0, 0), 0, 0), ((1, 1, 1), 3, 128),
-                                             ((1, 0, 1), 1, 64), ((0, 1, 1), 8, 256)])
+                                             ((1, 0, 1), 1, 64), ((0, 1, 1), 8, 256),
+                                             ((1, 1, 2), 2, 128), ((1, 0, 2), 1, 256),
+                                             ((0, 0, 2), 3, 64), ((1, 0, 2), 1, 192)])
 def test_potential_fft_options(orc, dfk, Nu, Nphi, wave_ok, opts, cb, threads):
     """Transform options (dfk_set_fft_options): twiddles from HBM instead of
-    LDS give the same bits; the wave-local in-place column transform (a
-    kernel of its own, which may contract multiply-adds differently) agrees
-    to round-off, with fewer, as many and more columns than waves per
-    workgroup; on the HBM-I/O stage path (radices 2/3/4/8) and the general
-    one (35 x 40: 70 = 2 x 5 x 7; 33 x 26: 66 = 2 x 3 x 11), where the wave
-    mode is off."""
+    LDS give the same bits; the in-place column transforms (kernels of their
+    own, which may contract multiply-adds differently) agree to round-off:
+    one wave per column with fewer, as many and more columns than waves per
+    workgroup, and the whole workgroup on one column after another at every
+    workgroup size; on the HBM-I/O stage path (radices 2/3/4/8) and the
+    general one (35 x 40: 70 = 2 x 5 x 7; 33 x 26: 66 = 2 x 3 x 11), where
+    the in-place modes are off."""
     print(f"case potential {Nu}x{Nphi} fft options {opts} cb {cb} threads {threads}", flush=True)
     e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
     rng = np.random.default_rng(11)
