@@ -101,11 +101,13 @@ def cpu_baseline(p, S, U, W, cfl, seconds, threads):
                        f"({dt:.1f} s), oracle built -O3 -march=x86-64-v3, OpenMP {threads} threads")
 
 
-def ensemble_legs(dfk, steps_batch, steps_seq):
+def ensemble_legs(dfk, steps_batch, steps_seq, chains=0):
     import ensemble_cfg
     disks = ensemble_cfg.draw(64, seed=2026)
     ens, ics = ensemble_cfg.build(dfk, disks)
     ntuned = autotune.apply(ens, regime="throughput")
+    if chains:
+        ens.set_chains(chains)
     zones = sum(d["Nu"] * d["Nphi"] for d in disks)
     ens.set_graph_steps(50)
     ens.run_init(1e30, cfl=0.4, mD=[ic[3] for ic in ics], sample_cap=64)
@@ -114,6 +116,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
     ens.advance(steps_batch)
     wall_b = time.perf_counter() - t0
     batched = zones * steps_batch / wall_b
+    nchains = ens.get_chains()
     names = ["control", "rows_r2c", "cols_conv", "rows_c2r", "update"]
     kms = dict(zip(names, [float(x) for x in ens.profile_step(50)]))
     kb = {k: sum(kernel_bytes(d["Nu"], d["Nphi"])[k] for d in disks) for k in names}
@@ -124,7 +127,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
     # the same disks stepped one after another, each a plain single-disk run
     seq_time, seq_zones = 0.0, 0
     if steps_seq <= 0:
-        return {"disks": 64, "zones": zones, "steps": steps_batch,
+        return {"disks": 64, "zones": zones, "steps": steps_batch, "graph_chains": nchains,
                 "batched_zone_updates_per_s": batched, "kernels": kern}
     for d, ic in zip(disks, ics):
         e = dfk.Ensemble(ic[4])
@@ -140,7 +143,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq):
         e.close()
     seq = seq_zones / seq_time
     balg = sum(b_alg(d["Nu"], d["Nphi"]) for d in disks) / zones
-    return {"disks": 64, "zones": zones, "steps": steps_batch,
+    return {"disks": 64, "zones": zones, "steps": steps_batch, "graph_chains": nchains,
             "batched_zone_updates_per_s": batched,
             "sequential_zone_updates_per_s": seq,
             "speedup_vs_sequential": batched / seq, "tuned_disks": ntuned,
@@ -164,6 +167,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the batched 64-disk leg (no single disk, no sequential leg)")
+    ap.add_argument("--chains", type=int, default=0,
+                    help="graph chains of the ensemble handle (0: the library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -174,7 +179,8 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     if args.ensemble_only:
-        print(json.dumps({"ensemble": ensemble_legs(dfk, args.ensemble_steps, 0)}), flush=True)
+        print(json.dumps({"ensemble": ensemble_legs(dfk, args.ensemble_steps, 0, args.chains)}),
+              flush=True)
         return
     Nu, Nphi = args.nu, args.nphi
     r = single_disk(dfk, Nu, Nphi, args.eps, args.cfl, args.steps, args.warmup, args.chunk)
@@ -241,7 +247,7 @@ def main():
         out["cpu_baseline"]["single_thread"] = {"value": one["value"], "sample": one["sample"]}
     if not args.no_ensemble:
         try:
-            out["ensemble"] = ensemble_legs(dfk, args.ensemble_steps, 100)
+            out["ensemble"] = ensemble_legs(dfk, args.ensemble_steps, 100, args.chains)
         except ImportError:
             out["ensemble"] = None
     print(json.dumps(out), flush=True)

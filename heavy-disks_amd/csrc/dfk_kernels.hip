@@ -664,14 +664,15 @@ __global__ void __launch_bounds__(WG)
 k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
            DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,
            const double *__restrict__ geom, const double2 *__restrict__ tw,
-           double2 *__restrict__ spec, int par, int nrow_blk, const double *__restrict__ pcfl,
-           const double *__restrict__ pcm, DfkSampleDev *__restrict__ samples, int cap) {
+           double2 *__restrict__ spec, int par, int nrow_blk, int ctl0,
+           const double *__restrict__ pcfl, const double *__restrict__ pcm,
+           DfkSampleDev *__restrict__ samples, int cap) {
   extern __shared__ double2 lds2[];
   if ((int)blockIdx.x >= nrow_blk) {
-    // trailing workgroups: the loop control of disk blockIdx.x - nrow_blk.
+    // trailing workgroups: the loop control of disk ctl0 + blockIdx.x - nrow_blk.
     // Row blocks of a disk that it stops read `done` racily; they then only
     // transform a spectrum that no later kernel of the disk reads.
-    const int d = blockIdx.x - nrow_blk;
+    const int d = ctl0 + blockIdx.x - nrow_blk;
     if (!ctl[d].done) control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
     return;
   }
@@ -1544,11 +1545,11 @@ hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const 
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, DiskCtl *ctl, const double *st, i64 ntot,
                           const double *geom, const double2 *tw, double2 *spec, int par,
-                          int nctl, const double *pcfl, const double *pcm, DfkSampleDev *samples,
-                          int cap) {
+                          int nctl, int ctl0, const double *pcfl, const double *pcm,
+                          DfkSampleDev *samples, int cap) {
   if (nblk + nctl <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_rows_r2c, dim3(nblk + nctl), dim3(nthr), lds, s, disks, map, base, ctl, st,
-                     ntot, geom, tw, spec, par, nblk, pcfl, pcm, samples, cap);
+                     ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap);
   return hipGetLastError();
 }
 

@@ -12,11 +12,11 @@ hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, Di
 hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const int2 *map,
                            int base, const DiskCtl *ctl, const double *st, long long ntot,
                            const double *geom, const double *trig, double *pcfl, double *pcm);
-// nctl > 0 appends one control workgroup per disk 0 .. nctl-1 to the launch.
+// nctl > 0 appends one control workgroup per disk ctl0 .. ctl0+nctl-1 to the launch.
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, DiskCtl *ctl, const double *st,
                           long long ntot, const double *geom, const double2 *tw, double2 *spec,
-                          int par, int nctl, const double *pcfl, const double *pcm,
+                          int par, int nctl, int ctl0, const double *pcfl, const double *pcm,
                           DfkSampleDev *samples, int cap);
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                            const int2 *map, int base, const DiskCtl *ctl, const double *khat,

@@ -129,6 +129,10 @@ struct dfk_handle {
   long long iter = 0;            // loop iterations launched since dfk_run_init
   int thr_rows = 256, thr_cols = 256, thr_upd = 256;   // threads per workgroup
   int upd_kind = 0;              // 0: LDS-tile update kernel, 1: register-streaming
+  int nchains = 3;               // disk groups stepped as parallel chains of a graph
+  std::vector<hipStream_t> side; // streams of chains 1 .. nchains-1 (default priority)
+  std::vector<hipEvent_t> join;  // their join events; ev_fork starts them
+  hipEvent_t ev_fork = nullptr;
 };
 
 namespace {
@@ -308,35 +312,42 @@ double *field(dfk_handle *h, int buf, int f, int d) {
   return h->st + (i64)(buf * 3 + f) * h->ntot + h->hD[d].so;
 }
 
-// Row r2c of every self-gravitating disk, with the loop control of all disks
-// in trailing workgroups of the same launch (nctl = h->nd) or without (0).
-hipError_t launch_r2c(dfk_handle *h, int nblk, int base, int par, int nctl) {
-  return dfk_launch_r2c(h->stream, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl,
-                        h->st, h->ntot, h->geom, h->tw, h->spec, par, nctl, h->pcfl, h->pcm,
+// Row r2c of map range [base, base + nblk), with the loop control of disks
+// ctl0 .. ctl0 + nctl - 1 in trailing workgroups of the same launch (nctl = 0:
+// none), on stream s.
+hipError_t launch_r2c(dfk_handle *h, hipStream_t s, int nblk, int base, int par, int nctl,
+                      int ctl0 = 0) {
+  return dfk_launch_r2c(s, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl,
+                        h->st, h->ntot, h->geom, h->tw, h->spec, par, nctl, ctl0, h->pcfl, h->pcm,
                         h->samples, h->cap);
 }
 
-// Column convolution of disk d (all disks for d < 0): the workgroup-
-// cooperative kernel for the disks that use it, the wave-local one for the rest.
-hipError_t launch_conv(dfk_handle *h, int d) {
-  const int a = d < 0 ? 0 : d, b = d < 0 ? h->nd : d + 1;
+// Column convolution of disks [a, b) on stream s: each of the three column
+// kernels (cooperative, wave-local, workgroup in place) for the disks that
+// use it.
+hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b) {
   const int n = h->conv0[b] - h->conv0[a], nw = h->convw0[b] - h->convw0[a];
   const int ng = h->convg0[b] - h->convg0[a];
-  hipError_t e = dfk_launch_conv(h->stream, n, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[a],
+  hipError_t e = dfk_launch_conv(s, n, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[a],
                                  h->ctl, h->khat, h->tw, h->spec);
   if (e != hipSuccess) return e;
-  e = dfk_launch_conv_wave(h->stream, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
+  e = dfk_launch_conv_wave(s, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
                            h->convw0[a], h->ctl, h->khat, h->tw, h->spec);
   if (e != hipSuccess) return e;
-  return dfk_launch_conv_group(h->stream, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
+  return dfk_launch_conv_group(s, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
                                h->convg0[a], h->ctl, h->khat, h->tw, h->spec);
+}
+
+// Column convolution of disk d (all disks for d < 0) on the handle's stream.
+hipError_t launch_conv(dfk_handle *h, int d) {
+  return launch_conv_range(h, h->stream, d < 0 ? 0 : d, d < 0 ? h->nd : d + 1);
 }
 
 // Gravity pipeline (rows r2c -> column convolution -> rows c2r) for one disk,
 // Sigma read from state buffer `par`.
 int launch_gravity_disk(dfk_handle *h, int d, int par) {
   const int nr = h->r2c0[d + 1] - h->r2c0[d];
-  HIPCHK(launch_r2c(h, nr, h->r2c0[d], par, 0));
+  HIPCHK(launch_r2c(h, h->stream, nr, h->r2c0[d], par, 0));
   HIPCHK(launch_conv(h, d));
   HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
                         h->geom, h->tw, h->phi));
@@ -345,14 +356,16 @@ int launch_gravity_disk(dfk_handle *h, int d, int par) {
 
 // Update of map range [base, base + nblk) from state buffer par; the
 // gamma = 2 kernels when every disk of the handle has gamma = 2.
-hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR, int has_gp, int par) {
+hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR, int has_gp, int par,
+                         hipStream_t s = nullptr) {
+  if (!s) s = h->stream;
   int g2 = 1;
   for (const DiskDev &D : h->hD) g2 &= (D.gamma2 != 0);
   if (h->upd_kind == 1)
-    return dfk_launch_update_stream(h->stream, nblk, garr, g2, h->dD, h->map_upd, base, h->ctl, h->st,
+    return dfk_launch_update_stream(s, nblk, garr, g2, h->dD, h->map_upd, base, h->ctl, h->st,
                                     h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
                                     h->trig, h->pcfl, h->pcm, par);
-  return dfk_launch_update(h->stream, nblk, h->thr_upd, h->lds_upd, garr, g2, h->dD, h->map_upd, base,
+  return dfk_launch_update(s, nblk, h->thr_upd, h->lds_upd, garr, g2, h->dD, h->map_upd, base,
                            h->ctl, h->st, h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
                            h->trig, h->pcfl, h->pcm, par);
 }
@@ -365,7 +378,7 @@ hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR
 int launch_step_all(dfk_handle *h, int par) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd];
   if (nr > 0) {
-    HIPCHK(launch_r2c(h, nr, 0, par, h->nd));
+    HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd));
     HIPCHK(launch_conv(h, -1));
     HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
                           h->geom, h->tw, h->phi));
@@ -376,17 +389,86 @@ int launch_step_all(dfk_handle *h, int par) {
   return DFK_OK;
 }
 
+// The same for the disks [a, b) on stream s (every one of them
+// self-gravitating, so the control rides in their r2c launch).
+int launch_step_range(dfk_handle *h, hipStream_t s, int a, int b, int par) {
+  const int nr = h->r2c0[b] - h->r2c0[a], nu = h->upd0[b] - h->upd0[a];
+  HIPCHK(launch_r2c(h, s, nr, h->r2c0[a], par, b - a, a));
+  HIPCHK(launch_conv_range(h, s, a, b));
+  HIPCHK(dfk_launch_c2r(s, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[a], h->ctl, h->spec,
+                        h->geom, h->tw, h->phi));
+  HIPCHK(launch_update(h, nu, h->upd0[a], 0, 0, 0, par, s));
+  return DFK_OK;
+}
+
+// Disk ranges of the graph chains: h->nchains contiguous groups of about
+// equal zone counts (one chain when a disk has no self-gravity or there are
+// fewer disks than chains).
+std::vector<int> chain_bounds(dfk_handle *h) {
+  int k = h->nchains;
+  for (const DiskDev &D : h->hD) if (!D.selfgrav) k = 1;
+  k = std::max(1, std::min(k, h->nd));
+  std::vector<int> cut{0};
+  i64 tot = 0, acc = 0;
+  for (const DiskDev &D : h->hD) tot += (i64)D.Nu * D.Nphi;
+  for (int d = 0; d < h->nd && (int)cut.size() < k; ++d) {
+    acc += (i64)h->hD[d].Nu * h->hD[d].Nphi;
+    // close the group once it holds its share; keep a disk for every later group
+    if (acc * k >= tot * (i64)cut.size() && h->nd - (d + 1) >= k - (int)cut.size()) cut.push_back(d + 1);
+  }
+  cut.push_back(h->nd);
+  return cut;
+}
+
 // Graph of `nsteps` iterations starting with state buffer par0 (cached).
 int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
   const int key = 2 * nsteps + par0;
   auto it = h->graphs.find(key);
   if (it != h->graphs.end()) { *out = it->second; return DFK_OK; }
+  const std::vector<int> cb = chain_bounds(h);
+  const int nc = (int)cb.size() - 1;
+  HIPCHK(hipSetDevice(h->device));
+  while ((int)h->side.size() < nc - 1) {
+    hipStream_t s;
+    hipEvent_t ev;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    h->side.push_back(s);
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    h->join.push_back(ev);
+  }
+  if (nc > 1 && !h->ev_fork) HIPCHK(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = DFK_OK;
-  for (int s = 0; s < nsteps && rc == DFK_OK; ++s) rc = launch_step_all(h, (par0 + s) & 1);
+  if (nc == 1) {
+    for (int s = 0; s < nsteps && rc == DFK_OK; ++s) rc = launch_step_all(h, (par0 + s) & 1);
+  } else {
+    // chain c steps disks [cb[c], cb[c+1]) on its own stream: the chains
+    // share nothing but the launch, so one chain's kernels run while
+    // another's drain
+    auto chk = [&](hipError_t x) {
+      if (x != hipSuccess && rc == DFK_OK) {
+        set_err("graph chain: %s", hipGetErrorString(x));
+        rc = DFK_EHIP;
+      }
+    };
+    chk(hipEventRecord(h->ev_fork, h->stream));
+    for (int c = 1; c < nc; ++c) chk(hipStreamWaitEvent(h->side[c - 1], h->ev_fork, 0));
+    for (int c = 0; c < nc && rc == DFK_OK; ++c) {
+      hipStream_t st = c == 0 ? h->stream : h->side[c - 1];
+      for (int s = 0; s < nsteps && rc == DFK_OK; ++s)
+        rc = launch_step_range(h, st, cb[c], cb[c + 1], (par0 + s) & 1);
+    }
+    for (int c = 1; c < nc; ++c) {
+      chk(hipEventRecord(h->join[c - 1], h->side[c - 1]));
+      chk(hipStreamWaitEvent(h->stream, h->join[c - 1], 0));
+    }
+  }
   hipError_t e = hipStreamEndCapture(h->stream, &g);
-  if (rc != DFK_OK) return rc;
+  if (rc != DFK_OK) {
+    if (e == hipSuccess) hipGraphDestroy(g);
+    return rc;
+  }
   if (e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); return DFK_EHIP; }
   hipGraphExec_t ex;
   e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -712,6 +794,9 @@ void dfk_destroy(dfk_handle *h) {
   hipFree(h->ctl); hipFree(h->samples);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
+  for (hipStream_t s : h->side) hipStreamDestroy(s);
+  for (hipEvent_t e : h->join) hipEventDestroy(e);
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
 }
@@ -1161,6 +1246,21 @@ int dfk_get_update_kernel(dfk_handle *h, int *kind) {
   return DFK_OK;
 }
 
+int dfk_set_chains(dfk_handle *h, int k) {
+  if (!h || k < 1 || k > 4) { set_err("chains must be 1 .. 4"); return DFK_EINVAL; }
+  if (k != h->nchains) {
+    h->nchains = k;
+    drop_graphs(h);
+  }
+  return DFK_OK;
+}
+
+int dfk_get_chains(dfk_handle *h, int *k) {
+  if (!h || !k) { set_err("null argument"); return DFK_EINVAL; }
+  *k = (int)chain_bounds(h).size() - 1;
+  return DFK_OK;
+}
+
 int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update) {
   if (!h) { set_err("null handle"); return DFK_EINVAL; }
   if (rows) *rows = h->thr_rows;
@@ -1203,7 +1303,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
           HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
           break;
         case 1:
-          HIPCHK(launch_r2c(h, nr, 0, par, h->nd));   // as in the loop: with the control
+          HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd));   // as in the loop: with the control
           break;
         case 2:
           HIPCHK(launch_conv(h, -1));

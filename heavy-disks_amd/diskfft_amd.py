@@ -33,6 +33,7 @@ SYMBOLS = [
     "dfk_run_init", "dfk_advance", "dfk_samples", "dfk_get_status", "dfk_set_shapes",
     "dfk_get_shapes", "dfk_set_fft_options", "dfk_get_fft_options", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
     "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
+    "dfk_set_chains", "dfk_get_chains",
     "dfk_snapshot_save", "dfk_snapshot_load", "dfk_set_eos",
 ]
 
@@ -104,6 +105,8 @@ def lib():
             "dfk_get_threads": (_I, [_P, _IP, _IP, _IP]),
             "dfk_set_update_kernel": (_I, [_P, _I]),
             "dfk_get_update_kernel": (_I, [_P, _IP]),
+            "dfk_set_chains": (_I, [_P, _I]),
+            "dfk_get_chains": (_I, [_P, _IP]),
             "dfk_snapshot_save": (_I, [_P, _I, ctypes.c_char_p]),
             "dfk_snapshot_load": (_I, [_P, _I, ctypes.c_char_p]),
             "dfk_set_eos": (_I, [_P, _I, _D, _D]),
@@ -309,6 +312,15 @@ class Ensemble:
     def get_update_kernel(self):
         k = _I()
         _check(lib().dfk_get_update_kernel(self.h, ctypes.byref(k)))
+        return k.value
+
+    def set_chains(self, k):
+        """Step the disks as k parallel graph chains (dfk_set_chains)."""
+        _check(lib().dfk_set_chains(self.h, k))
+
+    def get_chains(self):
+        k = _I()
+        _check(lib().dfk_get_chains(self.h, ctypes.byref(k)))
         return k.value
 
     def set_eos(self, K, gamma=2.0, d=0):

@@ -175,6 +175,17 @@ int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update);
 int dfk_set_update_kernel(dfk_handle *h, int kind);
 int dfk_get_update_kernel(dfk_handle *h, int *kind);
 
+/* Graph chains (1 .. 4; no counterpart in the reference, which steps one
+ * disk): the disks of the handle are split into that many contiguous groups
+ * of about equal zone counts, and each group is stepped by its own chain of
+ * launches in the loop's graphs, on its own stream, so that the kernels of
+ * one group fill the GPU while those of another drain.  Results are the same
+ * bits as with one chain.  Default 3 (at most one chain per disk); only
+ * handles whose disks all have self-gravity use more than one chain;
+ * dfk_get_chains reports the number in use. */
+int dfk_set_chains(dfk_handle *h, int k);
+int dfk_get_chains(dfk_handle *h, int *k);
+
 /* Device time in ms of the last dfk_advance (HIP events around the graph
  * launch on the handle's stream). */
 int dfk_last_advance_ms(dfk_handle *h, double *ms);

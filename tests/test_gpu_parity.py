@@ -330,6 +330,43 @@ def test_ensemble_matches_single_disks(orc, dfk, kind):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("chains", [2, 3])
+def test_graph_chains_same_bits(orc, dfk, chains):
+    """Disks stepped as parallel graph chains (dfk_set_chains) give the same
+    states, clocks and samples as one chain, with disks that stop at
+    different steps."""
+    print(f"case graph chains {chains}", flush=True)
+    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 30, 0.04), (40, 130, 0.01), (24, 24, 0.03)]
+    params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
+    ics, mDs = [], []
+    for a, b, c in shapes:
+        h, _ = _lka98(orc, a, b, c)
+        ics.append(h.state())
+        mDs.append(h.disk_mass())
+    caps = [20, 35, 27, 31, 12]
+    out = []
+    for k in (1, chains):
+        e = dfk.Ensemble(params)
+        e.set_chains(k)
+        for d in range(len(shapes)):
+            e.set_state(*ics[d], d=d)
+        e.set_graph_steps(8)
+        e.run_init(1e9, dtout=0.002, mD=mDs, max_steps=caps, sample_cap=64)
+        e.advance(40)
+        out.append((e.get_chains(), [e.get_state(d) for d in range(len(shapes))],
+                    [e.status(d) for d in range(len(shapes))],
+                    [e.samples(d) for d in range(len(shapes))]))
+        e.close()
+    assert out[0][0] == 1 and out[1][0] == chains
+    for d in range(len(shapes)):
+        assert out[1][2][d]["steps"] == caps[d]
+        assert out[0][2][d]["t"] == out[1][2][d]["t"]
+        for a, b in zip(out[0][1][d], out[1][1][d]):
+            assert np.array_equal(a, b)
+    for a, b in zip(out[0][3], out[1][3]):
+        assert len(a) > 2 and np.array_equal(a, b)
+
+
 def test_nonfinite_flag_stops_disk(dfk):
     Nu, Nphi = 32, 32
     print("case nonfinite flag", flush=True)
