@@ -66,14 +66,19 @@ def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
     """Set every disk of an Ensemble to its cached shapes (where present), and
     the handle's threads per workgroup to the most common tuned choice."""
     cache = load_cache(path)
-    n, votes, kinds = 0, {}, {}
+    n, votes, kinds, rows = 0, {}, {}, {}
     found = [cache.get(_key(*ens.shape(d), regime, device)) for d in range(ens.n)]
     for d, c in enumerate(found):
         if c:
+            z = ens.shape(d)[0] * ens.shape(d)[1]
             k = c.get("kind", 0)
-            kinds[k] = kinds.get(k, 0) + ens.shape(d)[0] * ens.shape(d)[1]
+            kinds[k] = kinds.get(k, 0) + z
+            r = c.get("rows_inplace", 0)
+            rows[r] = rows.get(r, 0) + z
     if kinds:
         ens.set_update_kernel(max(kinds, key=kinds.get))
+    if rows:
+        ens.set_row_transform(max(rows, key=rows.get))
     for d, c in enumerate(found):
         if c:
             ens.set_shapes(c["ti"], c["tj"], c["pr"], c["cb"], d=d)
@@ -124,9 +129,15 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     # (1) or read through the caches (0).  A launch's LDS is the largest any
     # of its disks needs, so the knob only pays where the big transforms use
     # it; for the rows it lost in mixed ensembles and is not tuned.
-    for pr in prs:
-        ms = prof(base[0], base[1], pr, base[3])
-        res["pr"][str(pr)] = float(ms[1] + ms[3])
+    # ... and the row transforms cooperative (0) or in place (1)
+    for rip in (0, 1):
+        e.set_row_transform(rip)
+        if rip and not e.get_row_transform():
+            break                      # this Nphi has no in-place row transform
+        for pr in prs:
+            ms = prof(base[0], base[1], pr, base[3])
+            res["pr"][f"{pr}:{rip}"] = float(ms[1] + ms[3])
+    e.set_row_transform(0)
     # column batches: twiddles in LDS or through the caches; workgroup-
     # cooperative columns (ping-pong buffers), one wave per column in place,
     # or the workgroup in place on one column after another
@@ -148,7 +159,8 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
             e.set_threads(*thr0)
     bt = min(res["tile"], key=res["tile"].get)
     bst = min(res["stream"], key=res["stream"].get) if res["stream"] else None
-    bpr, twr = int(min(res["pr"], key=res["pr"].get)), 1
+    (bpr, rip), twr = (int(x) for x in min(res["pr"], key=res["pr"].get).split(":")), 1
+    e.set_row_transform(rip)
     bcb, twc, cwave = (int(x) for x in min(res["cb"], key=res["cb"].get).split(":"))
     e.set_fft_options(twr, twc, cwave)
     ti, tj = (int(x) for x in bt.split("x"))
@@ -167,13 +179,13 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     best = prof(ti, tj, bpr, bcb)
     e.close()
     out = {"kind": kind, "ti": ti, "tj": tj, "pr": bpr, "cb": bcb, "threads": list(thr),
-           "fft": [twr, twc, cwave],
+           "fft": [twr, twc, cwave], "rows_inplace": rip,
            "copies": copies,
            "ms_per_kernel": [float(x) for x in best],
            "default": {"shape": list(base), "tile_ms": res["tile"].get(f"{base[0]}x{base[1]}")},
            "sweep": res}
     if verbose:
-        print(f"{Nu}x{Nphi} {regime} x{copies}: kernel {kind} tile {ti}x{tj} pr {bpr} cb {bcb} "
+        print(f"{Nu}x{Nphi} {regime} x{copies}: kernel {kind} tile {ti}x{tj} pr {bpr} rows in place {rip} cb {bcb} "
               f"twiddles in LDS (rows, cols) {(twr, twc)} wave columns {cwave} threads {thr} "
               f"ms {np.round(best, 4).tolist()}", flush=True)
     return out

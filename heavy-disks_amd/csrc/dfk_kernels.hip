@@ -360,17 +360,17 @@ __device__ __forceinline__ void io_stage_r(int r, const double2 *a, double2 *b, 
   else io_stage<3, SIGN, IO>(a, b, n, batch, Ns, tw, g);
 }
 
-// ------------------------------------------------------------------ in-place column transforms
-// Throughput-regime variants of the column convolution: a group of TG
-// threads (one wavefront, or the whole workgroup) owns a column and
-// transforms it in place in a single LDS buffer of M elements (no ping-pong
-// buffer: half the LDS of the cooperative kernel per column).  In each stage
-// every thread loads the inputs of all its butterflies (at most 1024 / TG
-// radix-R butterflies for M <= 1024), the group synchronises, then every
-// thread computes and stores: no thread overwrites a value another one still
-// has to read.  A wavefront synchronises without a workgroup barrier.  Same
-// butterflies, twiddles and I/O variants as io_stage (results agree to
-// round-off; the compiler's multiply-add contraction may differ).
+// ------------------------------------------------------------------ in-place transforms
+// Variants of the transforms in which a group of TG threads (one wavefront,
+// or the whole workgroup) transforms a batch of transforms in place in a
+// single LDS buffer (no ping-pong buffer: half the LDS of the cooperative
+// kernels).  In each stage every thread loads the inputs of all its
+// butterflies (at most NMAX / R / TG of them: batch x n <= NMAX), the group
+// synchronises, then every thread computes and stores: no thread overwrites
+// a value another one still has to read.  A wavefront synchronises without
+// a workgroup barrier.  Same butterflies, twiddles and I/O variants as
+// io_stage (results agree to round-off; the compiler's multiply-add
+// contraction may differ).
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): this wave's LDS accesses are done
@@ -383,23 +383,33 @@ __device__ __forceinline__ void group_sync() {
   else __syncthreads();
 }
 
-// One stage for thread `lane` (0 .. TG-1) of the group.
-template <int R, int SIGN, int IO, int TG>
-__device__ __forceinline__ void ip_stage(double2 *buf, int n, int p, int Ns, int lane,
+// Points a workgroup of TG threads transforms at once in place: 8 per thread
+// (one radix-8 butterfly), at least one length-1024 column.
+template <int TG>
+constexpr int ip_nmax() { return 8 * TG > 1024 ? 8 * TG : 1024; }
+
+// One stage over `batch` transforms of length n (batch x n <= NMAX) for
+// thread `lane` (0 .. TG-1) of the group.  Batch entry p is column k0 + p
+// (IO_ZP / IO_KHAT / IO_OUT) or rows row0 + 2p, row0 + 2p + 1 (IO_ROUT).
+// BATCH = false: batch is 1 (no per-butterfly batch index; fewer registers).
+template <int R, int SIGN, int IO, int TG, int NMAX, bool BATCH = true>
+__device__ __forceinline__ void ip_stage(double2 *buf, int n, int batch, int Ns, int lane,
                                          const double2 *__restrict__ tw, const StageIO &g) {
-  constexpr int EMAX = (1024 / R + TG - 1) / TG;   // butterflies per thread at n <= 1024
-  const int nb = n / R;
+  constexpr int EMAX = (NMAX / R + TG - 1) / TG;   // butterflies per thread
+  const int nb = n / R, total = BATCH ? nb * batch : nb;
   const int tstep = qdiv(n, rcpf((float)(Ns * R)));
-  const float rNs = rcpf((float)Ns);
-  const int col = g.k0 + p;
-  const bool live = col < g.Nc;
+  const float rNs = rcpf((float)Ns), rnb = BATCH ? rcpf((float)nb) : 0.0f;
   double2 v[EMAX][R];
 #pragma unroll
   for (int u = 0; u < EMAX; ++u) {
-    const int j = lane + TG * u;
-    if (j < nb) {
+    const int e = lane + TG * u;
+    if (e < total) {
+      const int p = BATCH ? qdiv(e, rnb) : 0;
+      const int j = e - p * nb;
       const int jq = qdiv(j, rNs);
       const int k = j - jq * Ns;
+      const int col = g.k0 + p;
+      const bool live = col < g.Nc;
 #pragma unroll
       for (int m = 0; m < R; ++m) {
         const int idx = j + m * nb;
@@ -407,7 +417,7 @@ __device__ __forceinline__ void ip_stage(double2 *buf, int n, int p, int Ns, int
           const bool pad = (R % 2 == 0) ? (m >= R / 2) : (idx >= g.Nu);
           v[u][m] = (live && !pad) ? g.spec[(i64)col * g.Nu + idx] : make_double2(0.0, 0.0);
         } else {
-          v[u][m] = buf[idx];
+          v[u][m] = buf[p * n + idx];
         }
       }
       if constexpr (IO != IO_ZP) {
@@ -418,10 +428,15 @@ __device__ __forceinline__ void ip_stage(double2 *buf, int n, int p, int Ns, int
   group_sync<TG>();
 #pragma unroll
   for (int u = 0; u < EMAX; ++u) {
-    const int j = lane + TG * u;
-    if (j < nb) {
+    const int e = lane + TG * u;
+    if (e < total) {
+      const int p = BATCH ? qdiv(e, rnb) : 0;
+      const int j = e - p * nb;
       const int jq = qdiv(j, rNs);
       const int k = j - jq * Ns;
+      const int col = g.k0 + p;
+      const bool live = col < g.Nc;
+      const int ra = g.row0 + 2 * p, rb = ra + 1;
       if constexpr (R == 2) bfly2<SIGN>(v[u][0], v[u][1]);
       if constexpr (R == 3) bfly3<SIGN>(v[u]);
       if constexpr (R == 4) bfly4<SIGN>(v[u]);
@@ -434,11 +449,14 @@ __device__ __forceinline__ void ip_stage(double2 *buf, int n, int p, int Ns, int
           double2 z = v[u][q];
           z.x *= kv;
           z.y *= kv;
-          buf[o] = z;
+          buf[p * n + o] = z;
         } else if constexpr (IO == IO_OUT) {
           if (live && o < g.Nu) g.spec[(i64)col * g.Nu + o] = v[u][q];
+        } else if constexpr (IO == IO_ROUT) {
+          if (ra < g.Nu) g.P[(i64)ra * n + o] = v[u][q].x * g.rsc[ra];
+          if (rb < g.Nu) g.P[(i64)rb * n + o] = v[u][q].y * g.rsc[rb];
         } else {
-          buf[o] = v[u][q];
+          buf[p * n + o] = v[u][q];
         }
       }
     }
@@ -446,34 +464,34 @@ __device__ __forceinline__ void ip_stage(double2 *buf, int n, int p, int Ns, int
   group_sync<TG>();
 }
 
-template <int SIGN, int IO, int TG>
-__device__ __forceinline__ void ip_stage_r(int r, double2 *buf, int n, int p, int Ns, int lane,
+template <int SIGN, int IO, int TG, int NMAX, bool BATCH = true>
+__device__ __forceinline__ void ip_stage_r(int r, double2 *buf, int n, int batch, int Ns, int lane,
                                            const double2 *tw, const StageIO &g) {
-  if (r == 8) ip_stage<8, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
-  else if (r == 4) ip_stage<4, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
-  else if (r == 2) ip_stage<2, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
-  else ip_stage<3, SIGN, IO, TG>(buf, n, p, Ns, lane, tw, g);
+  if (r == 8) ip_stage<8, SIGN, IO, TG, NMAX, BATCH>(buf, n, batch, Ns, lane, tw, g);
+  else if (r == 4) ip_stage<4, SIGN, IO, TG, NMAX, BATCH>(buf, n, batch, Ns, lane, tw, g);
+  else if (r == 2) ip_stage<2, SIGN, IO, TG, NMAX, BATCH>(buf, n, batch, Ns, lane, tw, g);
+  else ip_stage<3, SIGN, IO, TG, NMAX, BATCH>(buf, n, batch, Ns, lane, tw, g);
 }
 
-// The whole in-place convolution of column p (batch entry p of the
-// workgroup) in buf by a group of TG threads.
-template <int TG>
-__device__ __forceinline__ void ip_conv_column(const DiskDev &D, double2 *buf, int p, int lane,
-                                               const double2 *TW, const StageIO &g) {
+// The whole in-place convolution of columns k0 .. k0 + batch - 1 (g.k0) in
+// buf by a group of TG threads.
+template <int TG, int NMAX, bool BATCH = true>
+__device__ __forceinline__ void ip_conv(const DiskDev &D, double2 *buf, int batch, int lane,
+                                        const double2 *TW, const StageIO &g) {
   const int M = D.M, ns = D.nst_col;
-  ip_stage_r<-1, IO_ZP, TG>(D.rad_col[0], buf, M, p, 1, lane, TW, g);
+  ip_stage_r<-1, IO_ZP, TG, NMAX, BATCH>(D.rad_col[0], buf, M, batch, 1, lane, TW, g);
   int Ns = D.rad_col[0];
   for (int q = 1; q < ns - 1; ++q) {
-    ip_stage_r<-1, IO_LDS, TG>(D.rad_col[q], buf, M, p, Ns, lane, TW, g);
+    ip_stage_r<-1, IO_LDS, TG, NMAX, BATCH>(D.rad_col[q], buf, M, batch, Ns, lane, TW, g);
     Ns *= D.rad_col[q];
   }
-  ip_stage_r<-1, IO_KHAT, TG>(D.rad_col[ns - 1], buf, M, p, Ns, lane, TW, g);
+  ip_stage_r<-1, IO_KHAT, TG, NMAX, BATCH>(D.rad_col[ns - 1], buf, M, batch, Ns, lane, TW, g);
   Ns = 1;
   for (int q = 0; q < ns - 1; ++q) {
-    ip_stage_r<+1, IO_LDS, TG>(D.rad_col[q], buf, M, p, Ns, lane, TW, g);
+    ip_stage_r<+1, IO_LDS, TG, NMAX, BATCH>(D.rad_col[q], buf, M, batch, Ns, lane, TW, g);
     Ns *= D.rad_col[q];
   }
-  ip_stage_r<+1, IO_OUT, TG>(D.rad_col[ns - 1], buf, M, p, Ns, lane, TW, g);
+  ip_stage_r<+1, IO_OUT, TG, NMAX, BATCH>(D.rad_col[ns - 1], buf, M, batch, Ns, lane, TW, g);
 }
 
 // Whether a transform can use the global-I/O stages.
@@ -659,14 +677,18 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
 }
 
 // ------------------------------------------------------------------ gravity
-// Row r2c: S = R^{3/2} Sigma for rows [2 pr g, 2 pr (g+1)) of the current state.
-__global__ void __launch_bounds__(WG)
-k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
-           DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,
-           const double *__restrict__ geom, const double2 *__restrict__ tw,
-           double2 *__restrict__ spec, int par, int nrow_blk, int ctl0,
-           const double *__restrict__ pcfl, const double *__restrict__ pcm,
-           DfkSampleDev *__restrict__ samples, int cap) {
+// Row r2c: S = R^{3/2} Sigma for rows [2 pr g, 2 pr (g+1)) of the current
+// state.  TG = 0: the workgroup-cooperative transform (ping-pong buffers);
+// TG > 0: in place by the TG threads of the workgroup (one buffer of pr rows,
+// pr n <= ip_nmax<TG>(), radices 2/3/4/8; the runtime enforces both).
+template <int TG, bool BATCH = true>
+__device__ __forceinline__ void rows_r2c_body(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
+                                              int map_base, DiskCtl *__restrict__ ctl,
+                                              const double *__restrict__ st, i64 ntot,
+                                              const double *__restrict__ geom, const double2 *__restrict__ tw,
+                                              double2 *__restrict__ spec, int par, int nrow_blk, int ctl0,
+                                              const double *__restrict__ pcfl, const double *__restrict__ pcm,
+                                              DfkSampleDev *__restrict__ samples, int cap) {
   extern __shared__ double2 lds2[];
   if ((int)blockIdx.x >= nrow_blk) {
     // trailing workgroups: the loop control of disk ctl0 + blockIdx.x - nrow_blk.
@@ -682,8 +704,9 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   if (C.done) return;
   const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * wm.y;
+  constexpr int NBUF = TG ? 1 : 2;
   double2 *A = lds2, *B = lds2 + pr * n;
-  const double2 *TW = tw_table(lds2 + 2 * pr * n, tw + D.two_row, n, D.tw_lds & 1);
+  const double2 *TW = tw_table(lds2 + NBUF * pr * n, tw + D.two_row, n, D.tw_lds & 1);
   const double *S = stp(st, ntot, par, 0, D);
   const double *R32 = gtab(geom, D, GT_R32);
   // all threads stage the rows (more loads in flight than a first stage
@@ -697,7 +720,17 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
     A[e] = make_double2(xa, xb);
   }
   __syncthreads();
-  const double2 *Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
+  const double2 *Z = A;
+  if constexpr (TG == 0) {
+    Z = lds_fft<-1>(A, B, n, pr, D.nst_row, D.rad_row, TW);
+  } else {
+    StageIO g{};
+    int Ns = 1;
+    for (int q = 0; q < D.nst_row; ++q) {
+      ip_stage_r<-1, IO_LDS, TG, ip_nmax<TG>(), BATCH>(D.rad_row[q], A, n, pr, Ns, threadIdx.x, TW, g);
+      Ns *= D.rad_row[q];
+    }
+  }
   double2 *out = spec + D.spo;
   const int w2 = 2 * pr;
   for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
@@ -714,6 +747,20 @@ k_rows_r2c(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
     out[(i64)k * Nu + row] = X;
   }
 }
+
+#define R2C_PARAMS                                                                              \
+  const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,                \
+      DiskCtl *__restrict__ ctl, const double *__restrict__ st, i64 ntot,                       \
+      const double *__restrict__ geom, const double2 *__restrict__ tw,                          \
+      double2 *__restrict__ spec, int par, int nrow_blk, int ctl0,                              \
+      const double *__restrict__ pcfl, const double *__restrict__ pcm,                          \
+      DfkSampleDev *__restrict__ samples, int cap
+#define R2C_ARGS disks, map, map_base, ctl, st, ntot, geom, tw, spec, par, nrow_blk, ctl0, pcfl, pcm, samples, cap
+
+__global__ void __launch_bounds__(WG) k_rows_r2c(R2C_PARAMS) { rows_r2c_body<0>(R2C_ARGS); }
+
+template <int TG, bool BATCH>
+__global__ void __launch_bounds__(TG) k_rows_r2c_group(R2C_PARAMS) { rows_r2c_body<TG, BATCH>(R2C_ARGS); }
 
 // Column convolution: for columns [cb g, cb (g+1)): zero-padded forward
 // transform of length M, multiply by real K-hat, inverse, keep rows < Nu.
@@ -807,17 +854,22 @@ k_cols_conv_wave(const DiskDev *__restrict__ disks, const int2 *__restrict__ map
   g.spec = spec + D.spo; g.kh = khat + D.kho; g.Nu = Nu; g.Nc = Nc; g.k0 = k0;
   const int nw = blockDim.x >> 6, w = threadIdx.x >> 6;
   __syncthreads();   // the twiddle table, if staged in LDS
-  for (int p = w; p < cb; p += nw)   // wave w: columns w, w + nw, ...
-    ip_conv_column<64>(D, A + p * M, p, threadIdx.x & 63, TW, g);
+  for (int p = w; p < cb; p += nw) {   // wave w: columns w, w + nw, ...
+    StageIO gp = g;
+    gp.k0 = k0 + p;
+    ip_conv<64, 1024, false>(D, A + p * M, 1, threadIdx.x & 63, TW, gp);
+  }
 }
 
 // Column convolution in place by the whole workgroup (TG = blockDim.x
-// threads), its cb columns one after another through one LDS buffer of M
-// elements (then the twiddle table if staged): with half the LDS of the
-// cooperative kernel per column and at most 1024 / TG butterflies per thread,
-// more columns fit on a CU at once.  Same radix limits as the wave-local
-// kernel.
-template <int TG>
+// threads), its cb columns at once in cb LDS buffers of M elements (then the
+// twiddle table if staged; cb M <= ip_nmax<TG>(), which the runtime
+// enforces): half the LDS of the cooperative kernel per column, so more
+// columns fit on a CU at once.  Same radix limits as the wave-local kernel.
+// BATCH = false: one column per workgroup (cb = 1 for every disk of the
+// launch), whose addresses stay wave-uniform; in the latency regime that
+// build is ~20% faster than the batched one.
+template <int TG, bool BATCH>
 __global__ void __launch_bounds__(TG)
 k_cols_conv_group(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
                   const DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
@@ -828,28 +880,30 @@ k_cols_conv_group(const DiskDev *__restrict__ disks, const int2 *__restrict__ ma
   if (ctl[wm.x].done) return;
   const int M = D.M, Nc = D.Nc, cb = D.cb;
   const int k0 = wm.y * cb;
-  const double2 *TW = tw_table(lds2 + M, tw + D.two_col, M, D.tw_lds & 2);
+  const double2 *TW = tw_table(lds2 + cb * M, tw + D.two_col, M, D.tw_lds & 2);
   StageIO g{};
   g.spec = spec + D.spo; g.kh = khat + D.kho; g.Nu = D.Nu; g.Nc = Nc; g.k0 = k0;
   __syncthreads();   // the twiddle table, if staged in LDS
-  for (int p = 0; p < cb && k0 + p < Nc; ++p) ip_conv_column<TG>(D, lds2, p, threadIdx.x, TW, g);
+  ip_conv<TG, ip_nmax<TG>(), BATCH>(D, lds2, BATCH ? min(cb, Nc - k0) : 1, threadIdx.x, TW, g);
 }
 
 // Row c2r (FFTW convention: imaginary parts of the DC and Nyquist bins are
-// ignored) and Phi = pref * V / sqrt(R).
-__global__ void __launch_bounds__(WG)
-k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
-           const DiskCtl *__restrict__ ctl, const double2 *__restrict__ spec,
-           const double *__restrict__ geom, const double2 *__restrict__ tw,
-           double *__restrict__ phi) {
+// ignored) and Phi = pref * V / sqrt(R).  TG as for the r2c.
+template <int TG, bool BATCH = true>
+__device__ __forceinline__ void rows_c2r_body(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
+                                              int map_base, const DiskCtl *__restrict__ ctl,
+                                              const double2 *__restrict__ spec,
+                                              const double *__restrict__ geom, const double2 *__restrict__ tw,
+                                              double *__restrict__ phi) {
   extern __shared__ double2 lds2[];
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
   if (ctl[wm.x].done) return;
   const int n = D.Nphi, Nu = D.Nu, pr = D.pr, Nc = D.Nc;
   const int row0 = 2 * pr * wm.y;
+  constexpr int NBUF = TG ? 1 : 2;
   double2 *A = lds2, *B = lds2 + pr * n;
-  const double2 *TW = tw_table(lds2 + 2 * pr * n, tw + D.two_row, n, D.tw_lds & 1);
+  const double2 *TW = tw_table(lds2 + NBUF * pr * n, tw + D.two_row, n, D.tw_lds & 1);
   const double2 *in = spec + D.spo;
   for (int e = threadIdx.x; e < pr * Nc; e += blockDim.x) {
     const int k = qdiv(e, rcpf((float)pr)), p = e - k * pr;
@@ -864,6 +918,18 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
   __syncthreads();
   const double *sc = gtab(geom, D, GT_PHISC);
   double *P = phi + D.so;
+  if constexpr (TG > 0) {
+    StageIO g{};
+    g.rsc = sc; g.P = P; g.Nu = Nu; g.row0 = row0;
+    int Ns = 1;
+    for (int q = 0; q < D.nst_row - 1; ++q) {
+      ip_stage_r<+1, IO_LDS, TG, ip_nmax<TG>(), BATCH>(D.rad_row[q], A, n, pr, Ns, threadIdx.x, TW, g);
+      Ns *= D.rad_row[q];
+    }
+    ip_stage_r<+1, IO_ROUT, TG, ip_nmax<TG>(), BATCH>(D.rad_row[D.nst_row - 1], A, n, pr, Ns, threadIdx.x, TW,
+                                                      g);
+    return;
+  }
   if (io_fast(D.nst_row, D.rad_row)) {
     StageIO g{};
     g.rsc = sc; g.P = P; g.Nu = Nu; g.row0 = row0;
@@ -888,6 +954,17 @@ k_rows_c2r(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int 
     P[(i64)row * n + j] = v * sc[row];
   }
 }
+
+#define C2R_PARAMS                                                                              \
+  const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,                \
+      const DiskCtl *__restrict__ ctl, const double2 *__restrict__ spec,                        \
+      const double *__restrict__ geom, const double2 *__restrict__ tw, double *__restrict__ phi
+#define C2R_ARGS disks, map, map_base, ctl, spec, geom, tw, phi
+
+__global__ void __launch_bounds__(WG) k_rows_c2r(C2R_PARAMS) { rows_c2r_body<0>(C2R_ARGS); }
+
+template <int TG, bool BATCH>
+__global__ void __launch_bounds__(TG) k_rows_c2r_group(C2R_PARAMS) { rows_c2r_body<TG, BATCH>(C2R_ARGS); }
 
 // ------------------------------------------------------------------ K-hat setup
 // Row r2c of the M x Nphi kernel table (kalnajs.hpp:105-114), all M rows, into
@@ -1522,6 +1599,18 @@ __global__ void k_floor(const DiskDev *__restrict__ Dp, double *__restrict__ st,
 // (plain C++ wrappers so the runtime file stays free of kernel syntax)
 #include "dfk_launch.h"
 
+// The in-place group kernel for nthr threads, batched or not.
+template <class K>
+K pick_tg(int nthr, int batched, K a0, K a1, K b0, K b1, K c0, K c1, K d0, K d1) {
+  if (nthr == 64) return batched ? a1 : a0;
+  if (nthr == 128) return batched ? b1 : b0;
+  if (nthr == 192) return batched ? c1 : c0;
+  return batched ? d1 : d0;
+}
+#define PICK_GROUP(K)                                                                   \
+  pick_tg(nthr, batched, K<64, false>, K<64, true>, K<128, false>, K<128, true>,        \
+          K<192, false>, K<192, true>, K<256, false>, K<256, true>)
+
 #define CHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
 
 hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
@@ -1546,8 +1635,16 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           const int2 *map, int base, DiskCtl *ctl, const double *st, i64 ntot,
                           const double *geom, const double2 *tw, double2 *spec, int par,
                           int nctl, int ctl0, const double *pcfl, const double *pcm,
-                          DfkSampleDev *samples, int cap) {
+                          DfkSampleDev *samples, int cap, int inplace) {
   if (nblk + nctl <= 0) return hipSuccess;
+  if (inplace) {
+    const int batched = inplace == 2;
+    auto k = PICK_GROUP(k_rows_r2c_group);
+    if (nthr != 64 && nthr != 128 && nthr != 192 && nthr != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(nblk + nctl), dim3(nthr), lds, s, disks, map, base, ctl, st,
+                       ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_rows_r2c, dim3(nblk + nctl), dim3(nthr), lds, s, disks, map, base, ctl, st,
                      ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap);
   return hipGetLastError();
@@ -1572,11 +1669,9 @@ hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, c
 
 hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                                  const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                                 const double2 *tw, double2 *spec) {
+                                 const double2 *tw, double2 *spec, int batched) {
   if (nblk <= 0) return hipSuccess;
-  auto k = nthr == 64 ? k_cols_conv_group<64>
-         : nthr == 128 ? k_cols_conv_group<128>
-         : nthr == 192 ? k_cols_conv_group<192> : k_cols_conv_group<256>;
+  auto k = PICK_GROUP(k_cols_conv_group);
   if (nthr != 64 && nthr != 128 && nthr != 192 && nthr != 256) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec);
   return hipGetLastError();
@@ -1584,8 +1679,15 @@ hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, 
 
 hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
-                          const double *geom, const double2 *tw, double *phi) {
+                          const double *geom, const double2 *tw, double *phi, int inplace) {
   if (nblk <= 0) return hipSuccess;
+  if (inplace) {
+    const int batched = inplace == 2;
+    auto k = PICK_GROUP(k_rows_c2r_group);
+    if (nthr != 64 && nthr != 128 && nthr != 192 && nthr != 256) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, spec, geom, tw, phi);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_rows_c2r, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, spec, geom, tw,
                      phi);
   return hipGetLastError();

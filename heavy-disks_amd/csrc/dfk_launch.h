@@ -13,23 +13,26 @@ hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const 
                            int base, const DiskCtl *ctl, const double *st, long long ntot,
                            const double *geom, const double *trig, double *pcfl, double *pcm);
 // nctl > 0 appends one control workgroup per disk ctl0 .. ctl0+nctl-1 to the launch.
+// inplace: 0 the cooperative row transforms, 1 the in-place ones (k_rows_*_group) with
+// one complex row per workgroup (pr = 1 for every disk), 2 in place with batches.
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, DiskCtl *ctl, const double *st,
                           long long ntot, const double *geom, const double2 *tw, double2 *spec,
                           int par, int nctl, int ctl0, const double *pcfl, const double *pcm,
-                          DfkSampleDev *samples, int cap);
+                          DfkSampleDev *samples, int cap, int inplace);
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                            const int2 *map, int base, const DiskCtl *ctl, const double *khat,
                            const double2 *tw, double2 *spec);
 hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                                 const int2 *map, int base, const DiskCtl *ctl, const double *khat,
                                 const double2 *tw, double2 *spec);
+// batched: some disk of the launch has cb > 1 (else the one-column build).
 hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                                  const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                                 const double2 *tw, double2 *spec);
+                                 const double2 *tw, double2 *spec, int batched);
 hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
-                          const double *geom, const double2 *tw, double *phi);
+                          const double *geom, const double2 *tw, double *phi, int inplace);
 // g2: every disk of the launch has gamma = 2 (EOS without pow).
 hipError_t dfk_launch_update(hipStream_t s, int nblk, int nthr, size_t lds, int garr, int g2,
                              const DiskDev *disks,

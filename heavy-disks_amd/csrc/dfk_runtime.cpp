@@ -72,6 +72,18 @@ bool factor(int n, int *rad, int *nst) {
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// Whether a transform plan suits the in-place kernels: radices 2/3/4/8 only
+// and at least `min_st` stages.
+bool inplace_radices(const int *rad, int nst, int min_st) {
+  bool ok = nst >= min_st;
+  for (int q = 0; q < nst; ++q) ok = ok && (rad[q] == 2 || rad[q] == 3 || rad[q] == 4 || rad[q] == 8);
+  return ok;
+}
+
+// Points the in-place workgroup kernels transform at once with t threads
+// (the kernels' ip_nmax<t>()).
+int ip_nmax(int t) { return std::max(8 * t, 1024); }
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X
 // (workgroup b runs on XCD b % 8), and each XCD has its own L2.  Reorder a
 // work list so that XCD x receives one contiguous chunk of it: neighbouring
@@ -130,6 +142,10 @@ struct dfk_handle {
   int thr_rows = 256, thr_cols = 256, thr_upd = 256;   // threads per workgroup
   int upd_kind = 0;              // 0: LDS-tile update kernel, 1: register-streaming
   int nchains = 3;               // disk groups stepped as parallel chains of a graph
+  int row_inplace = 0;           // requested: in-place row transforms (k_rows_*_group)
+  int rows_ip = 0;               // in use (every self-gravitating disk's rows allow it)
+  int rows_batched = 0;          // in place with pr > 1 for some disk
+  int conv_batched = 0;          // some disk's in-place workgroup convolution has cb > 1
   std::vector<hipStream_t> side; // streams of chains 1 .. nchains-1 (default priority)
   std::vector<hipEvent_t> join;  // their join events; ev_fork starts them
   hipEvent_t ev_fork = nullptr;
@@ -175,6 +191,9 @@ int build_layout(dfk_handle *h) {
   i64 part = 0;
   size_t lu = 0, lr = 0, lc = 0, lcw = 0, lcg = 0;
   const size_t kMaxLds = h->lds_max;
+  h->rows_ip = h->row_inplace;
+  for (const DiskDev &D : h->hD)
+    if (D.selfgrav && !inplace_radices(D.rad_row, D.nst_row, 1)) h->rows_ip = 0;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
     if (h->upd_kind == 1) D.tj = kStreamCols;   // one wave = 60 output columns
@@ -208,16 +227,19 @@ int build_layout(dfk_handle *h) {
       // build with smaller LDS needs stays usable)
       // ping-pong buffers (+ the twiddle table where staged in LDS)
       auto lds_rows = [&](int pr) {
-        return (size_t)(2 * pr + (D.tw_lds & 1)) * D.Nphi * sizeof(double2);
+        return (size_t)((h->rows_ip ? 1 : 2) * pr + (D.tw_lds & 1)) * D.Nphi * sizeof(double2);
       };
-      // in-place columns need one buffer per column (the workgroup kernel
-      // one for all its columns, done one after another), cooperative ones two
+      // in-place columns need one buffer per column, cooperative ones two;
+      // the in-place workgroup kernels take at most ip_nmax points at once
       auto lds_cols = [&](int cb) {
-        const int nbuf = D.conv_wave == 2 ? 1 : (D.conv_wave == 1 ? cb : 2 * cb);
-        return (size_t)(nbuf + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
+        return (size_t)((D.conv_wave ? 1 : 2) * cb + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
       };
-      while (D.pr > 1 && lds_rows(D.pr) > kMaxLds) --D.pr;
-      while (D.cb > 1 && lds_cols(D.cb) > kMaxLds) --D.cb;
+      while (D.pr > 1 && (lds_rows(D.pr) > kMaxLds ||
+                          (h->rows_ip && D.pr * D.Nphi > ip_nmax(h->thr_rows))))
+        --D.pr;
+      while (D.cb > 1 && (lds_cols(D.cb) > kMaxLds ||
+                          (D.conv_wave == 2 && D.cb * D.M > ip_nmax(h->thr_cols))))
+        --D.cb;
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
       std::vector<int2> &cm = D.conv_wave == 2 ? mcg : (D.conv_wave == 1 ? mcw : mc);
       for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) cm.push_back(make_int2(d, g));
@@ -225,6 +247,11 @@ int build_layout(dfk_handle *h) {
       size_t &lcol = D.conv_wave == 2 ? lcg : (D.conv_wave == 1 ? lcw : lc);
       lcol = std::max(lcol, lds_cols(D.cb));
     }
+  }
+  h->rows_batched = h->conv_batched = 0;
+  for (const DiskDev &D : h->hD) {
+    if (D.selfgrav && D.pr > 1) h->rows_batched = 1;
+    if (D.selfgrav && D.conv_wave == 2 && D.cb > 1) h->conv_batched = 1;
   }
   h->upd0[h->nd] = (int)mu.size();
   h->r2c0[h->nd] = (int)mr.size();
@@ -319,7 +346,13 @@ hipError_t launch_r2c(dfk_handle *h, hipStream_t s, int nblk, int base, int par,
                       int ctl0 = 0) {
   return dfk_launch_r2c(s, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl,
                         h->st, h->ntot, h->geom, h->tw, h->spec, par, nctl, ctl0, h->pcfl, h->pcm,
-                        h->samples, h->cap);
+                        h->samples, h->cap, h->rows_ip ? 1 + h->rows_batched : 0);
+}
+
+// Row c2r of map range [base, base + nblk) on stream s.
+hipError_t launch_c2r(dfk_handle *h, hipStream_t s, int nblk, int base) {
+  return dfk_launch_c2r(s, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl, h->spec,
+                        h->geom, h->tw, h->phi, h->rows_ip ? 1 + h->rows_batched : 0);
 }
 
 // Column convolution of disks [a, b) on stream s: each of the three column
@@ -335,7 +368,7 @@ hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b) {
                            h->convw0[a], h->ctl, h->khat, h->tw, h->spec);
   if (e != hipSuccess) return e;
   return dfk_launch_conv_group(s, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
-                               h->convg0[a], h->ctl, h->khat, h->tw, h->spec);
+                               h->convg0[a], h->ctl, h->khat, h->tw, h->spec, h->conv_batched);
 }
 
 // Column convolution of disk d (all disks for d < 0) on the handle's stream.
@@ -349,8 +382,7 @@ int launch_gravity_disk(dfk_handle *h, int d, int par) {
   const int nr = h->r2c0[d + 1] - h->r2c0[d];
   HIPCHK(launch_r2c(h, h->stream, nr, h->r2c0[d], par, 0));
   HIPCHK(launch_conv(h, d));
-  HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], h->ctl, h->spec,
-                        h->geom, h->tw, h->phi));
+  HIPCHK(launch_c2r(h, h->stream, nr, h->r2c0[d]));
   return DFK_OK;
 }
 
@@ -380,8 +412,7 @@ int launch_step_all(dfk_handle *h, int par) {
   if (nr > 0) {
     HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd));
     HIPCHK(launch_conv(h, -1));
-    HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
-                          h->geom, h->tw, h->phi));
+    HIPCHK(launch_c2r(h, h->stream, nr, 0));
   } else {
     HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
   }
@@ -395,8 +426,7 @@ int launch_step_range(dfk_handle *h, hipStream_t s, int a, int b, int par) {
   const int nr = h->r2c0[b] - h->r2c0[a], nu = h->upd0[b] - h->upd0[a];
   HIPCHK(launch_r2c(h, s, nr, h->r2c0[a], par, b - a, a));
   HIPCHK(launch_conv_range(h, s, a, b));
-  HIPCHK(dfk_launch_c2r(s, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[a], h->ctl, h->spec,
-                        h->geom, h->tw, h->phi));
+  HIPCHK(launch_c2r(h, s, nr, h->r2c0[a]));
   HIPCHK(launch_update(h, nu, h->upd0[a], 0, 0, 0, par, s));
   return DFK_OK;
 }
@@ -1174,11 +1204,7 @@ int dfk_set_fft_options(dfk_handle *h, int d, int rows_tw_lds, int cols_tw_lds, 
   DiskDev &D = h->hD[d];
   const int v = (rows_tw_lds ? 1 : 0) | (cols_tw_lds ? 2 : 0);
   // the in-place column transforms need radices 2/3/4/8, >= 2 stages, M <= 1024
-  bool fast = D.nst_col >= 2 && D.M <= 1024;
-  for (int q = 0; q < D.nst_col; ++q) {
-    const int r = D.rad_col[q];
-    fast = fast && (r == 2 || r == 3 || r == 4 || r == 8);
-  }
+  const bool fast = D.M <= 1024 && inplace_radices(D.rad_col, D.nst_col, 2);
   const int w = (fast && D.selfgrav && (cols_wave == 1 || cols_wave == 2)) ? cols_wave : 0;
   if (v != D.tw_lds || w != D.conv_wave) {
     D.tw_lds = v;
@@ -1222,7 +1248,24 @@ int dfk_set_threads(dfk_handle *h, int rows, int cols, int update) {
   if (rows) h->thr_rows = rows;
   if (cols) h->thr_cols = cols;
   if (update) h->thr_upd = update;
-  drop_graphs(h);
+  h->layout_ok = false;   // the in-place transforms' batch limits depend on the threads
+  return build_layout(h);
+}
+
+int dfk_set_row_transform(dfk_handle *h, int inplace) {
+  if (!h || inplace < 0 || inplace > 1) { set_err("row transform must be 0 (cooperative) or 1 (in place)"); return DFK_EINVAL; }
+  if (inplace != h->row_inplace) {
+    h->row_inplace = inplace;
+    h->layout_ok = false;
+    return build_layout(h);
+  }
+  return DFK_OK;
+}
+
+int dfk_get_row_transform(dfk_handle *h, int *inplace) {
+  if (!h || !inplace) { set_err("null argument"); return DFK_EINVAL; }
+  TRY(build_layout(h));
+  *inplace = h->rows_ip;
   return DFK_OK;
 }
 
@@ -1309,8 +1352,7 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
           HIPCHK(launch_conv(h, -1));
           break;
         case 3:
-          HIPCHK(dfk_launch_c2r(h->stream, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, 0, h->ctl, h->spec,
-                                h->geom, h->tw, h->phi));
+          HIPCHK(launch_c2r(h, h->stream, nr, 0));
           break;
         case 4:
           HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par));

@@ -33,7 +33,7 @@ SYMBOLS = [
     "dfk_run_init", "dfk_advance", "dfk_samples", "dfk_get_status", "dfk_set_shapes",
     "dfk_get_shapes", "dfk_set_fft_options", "dfk_get_fft_options", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
     "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
-    "dfk_set_chains", "dfk_get_chains",
+    "dfk_set_chains", "dfk_get_chains", "dfk_set_row_transform", "dfk_get_row_transform",
     "dfk_snapshot_save", "dfk_snapshot_load", "dfk_set_eos",
 ]
 
@@ -106,6 +106,8 @@ def lib():
             "dfk_set_update_kernel": (_I, [_P, _I]),
             "dfk_get_update_kernel": (_I, [_P, _IP]),
             "dfk_set_chains": (_I, [_P, _I]),
+            "dfk_set_row_transform": (_I, [_P, _I]),
+            "dfk_get_row_transform": (_I, [_P, _IP]),
             "dfk_get_chains": (_I, [_P, _IP]),
             "dfk_snapshot_save": (_I, [_P, _I, ctypes.c_char_p]),
             "dfk_snapshot_load": (_I, [_P, _I, ctypes.c_char_p]),
@@ -312,6 +314,15 @@ class Ensemble:
     def get_update_kernel(self):
         k = _I()
         _check(lib().dfk_get_update_kernel(self.h, ctypes.byref(k)))
+        return k.value
+
+    def set_row_transform(self, inplace):
+        """Row transforms workgroup-cooperative (0) or in place (1)."""
+        _check(lib().dfk_set_row_transform(self.h, int(inplace)))
+
+    def get_row_transform(self):
+        k = _I()
+        _check(lib().dfk_get_row_transform(self.h, ctypes.byref(k)))
         return k.value
 
     def set_chains(self, k):

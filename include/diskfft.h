@@ -168,6 +168,17 @@ int dfk_get_fft_options(dfk_handle *h, int disk, int *rows_tw_lds, int *cols_tw_
 int dfk_set_threads(dfk_handle *h, int rows, int cols, int update);
 int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update);
 
+/* Row transforms of all disks of the handle (an autotuner knob; no
+ * counterpart in the reference): 0 = the workgroup cooperates on its pr
+ * complex rows with two LDS buffers (default); 1 = in place in one LDS buffer
+ * (half the LDS per row; pr x Nphi is then kept <= max(8 x threads, 1024)).
+ * In place needs every self-gravitating disk's Nphi to factor into 2, 3, 4
+ * and 8; otherwise the handle keeps the cooperative transforms, and
+ * dfk_get_row_transform reports the variant in use.  Both agree to
+ * round-off. */
+int dfk_set_row_transform(dfk_handle *h, int inplace);
+int dfk_get_row_transform(dfk_handle *h, int *inplace);
+
 /* Update-kernel variant for all disks of the handle: 0 = LDS tile (ti x tj
  * zones per workgroup; suits small, latency-bound runs), 1 = register
  * streaming (one wavefront per 60 columns x ti rows; suits large or batched,

@@ -330,6 +330,56 @@ def test_ensemble_matches_single_disks(orc, dfk, kind):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("Nu,Nphi,ok", [(256, 256, 1), (96, 80, 0), (512, 1024, 1), (3, 4, 1),
+                                        (64, 192, 1), (35, 40, 0)])
+@pytest.mark.parametrize("threads,pr,cb", [(256, 8, 4), (128, 3, 1), (64, 1, 2), (192, 16, 8)])
+def test_potential_row_transform(orc, dfk, Nu, Nphi, ok, threads, pr, cb):
+    """In-place row transforms (dfk_set_row_transform) with the in-place
+    workgroup column convolution, at every workgroup size and with batches
+    clamped to the in-place limit: round-off agreement with the cooperative
+    transforms and the oracle; Nphi with a factor 5 keeps the cooperative
+    rows."""
+    print(f"case potential {Nu}x{Nphi} rows in place threads {threads} pr {pr} cb {cb}", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    rng = np.random.default_rng(5)
+    Sig = rng.random((Nu, Nphi))
+    p_ref = e.potential(Sig)
+    e.set_row_transform(1)
+    e.set_fft_options(1, 0, 2)
+    e.set_threads(threads, threads, 256)
+    e.set_shapes(0, 0, pr, cb)
+    assert e.get_row_transform() == ok
+    _, _, pr_, cb_ = e.get_shapes(0)
+    nmax = max(8 * threads, 1024)
+    if ok:
+        assert pr_ * Nphi <= nmax
+    if e.get_fft_options(0)[2] == 2:
+        assert cb_ * 2 * Nu <= nmax
+    p_opt = e.potential(Sig)
+    assert relerr(p_opt, p_ref) < 1e-14
+    assert relerr(p_opt, k.potential(Sig)) < 1e-13
+
+
+@pytest.mark.parametrize("Nu,Nphi,eps", [(64, 96, 0.02), (256, 256, 0.01)])
+def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps):
+    """The 10-step loop gate with in-place row transforms and the in-place
+    workgroup column convolution."""
+    print(f"case loop {Nu}x{Nphi} in-place transforms", flush=True)
+    h, k = _lka98(orc, Nu, Nphi, eps)
+    e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
+    e.set_row_transform(1)
+    e.set_fft_options(1, 0, 2)
+    e.set_threads(128, 128, 256)
+    e.set_state(*h.state())
+    mD = h.disk_mass()
+    orc.run(h, k, 1, 1e9, dtout=0.25, cfl=0.3, max_steps=10, mD=mD)
+    e.run_init(1e9, dtout=0.25, cfl=0.3, mD=mD, max_steps=10)
+    e.advance(12)
+    errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
+    print("  rel errors Sigma, vR, vphi:", errs, flush=True)
+    assert e.status()["steps"] == 10 and max(errs) <= LOOP_TOL
+
+
 @pytest.mark.parametrize("chains", [2, 3])
 def test_graph_chains_same_bits(orc, dfk, chains):
     """Disks stepped as parallel graph chains (dfk_set_chains) give the same

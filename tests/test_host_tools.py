@@ -46,12 +46,16 @@ class _FakeEnsemble:
     def set_threads(self, *t):
         self.calls.append(("threads",) + t)
 
+    def set_row_transform(self, r):
+        self.calls.append(("rows", r))
+
 
 def test_apply_votes_kind_and_threads_by_zones(tmp_path):
     import autotune
     cache = {
         "128x128:throughput:gfx950": {"kind": 1, "ti": 16, "tj": 60, "pr": 4, "cb": 4,
-                                      "threads": [128, 128, 128], "fft": [1, 0, 1]},
+                                      "threads": [128, 128, 128], "fft": [1, 0, 1],
+                                      "rows_inplace": 1},
         "512x512:throughput:gfx950": {"kind": 0, "ti": 8, "tj": 32, "pr": 2, "cb": 1,
                                       "threads": [256, 256, 256]},
     }
@@ -61,6 +65,7 @@ def test_apply_votes_kind_and_threads_by_zones(tmp_path):
     n = autotune.apply(ens, path=str(path))
     assert n == 3                                    # 96x96 has no entry
     assert ("kind", 0) in ens.calls                  # 2 x 512^2 outweigh 128^2
+    assert ("rows", 0) in ens.calls and ("rows", 1) not in ens.calls
     assert ("threads", 256, 256, 256) in ens.calls
     assert ("shapes", 0, 16, 60, 4, 4) in ens.calls and ("fft", 0, 1, 0, 1) in ens.calls
     assert ("fft", 1, 1, 1, 0) in ens.calls          # defaults when the entry has none
