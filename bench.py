@@ -76,6 +76,7 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk, model=None):
     shapes["threads"] = e.get_threads()
     shapes["kind"] = e.get_update_kernel()
     shapes["fft"] = list(e.get_fft_options(0))
+    shapes["rows_inplace"] = e.get_row_transform()
     shapes["tuned"] = bool(tuned)
     e.close()
     return dict(p=p, S=S, U=U, W=W, mD=mD, wall=wall, dev_ms=dev_ms, prof=prof, shapes=shapes)
@@ -204,6 +205,8 @@ def main():
             pre = {0: "k_cols_conv", 1: "k_cols_conv_wave", 2: "k_cols_conv_group<"}[
                 r["shapes"].get("fft", [0, 0, 0])[2]]
             names = [k for k in tk if k == pre or (pre.endswith("<") and k.startswith(pre))]
+        elif r["shapes"].get("rows_inplace"):   # the in-place row kernels
+            names = [k for k in tk if k.startswith("k_" + dom + "_group<")]
         else:
             names = [k for k in tk if k == "k_" + dom]
         traffic = tk[names[0]]["traffic_bytes"] if names else None
