@@ -32,6 +32,7 @@ import autotune  # noqa: E402
 METRIC = "zone-updates/s incl. FFT gravity, 256² single vs 64-disk batch; % HBM peak"
 HBM_PEAK = 8.0e12          # B/s, MI355X
 REF_PUBLISHED = 6.12e7     # reference README: 256^2, FFTW + OpenMP 8 threads (CPU)
+ONCHIP_B_PER_ZONE = 56     # state in and out (48 B) + real K-hat (8 B): a step kept entirely on chip
 
 
 def b_alg(Nu, Nphi):
@@ -127,9 +128,13 @@ def ensemble_legs(dfk, steps_batch, steps_seq, chains=0):
     ens.close()
     # the same disks stepped one after another, each a plain single-disk run
     seq_time, seq_zones = 0.0, 0
+    balg = sum(b_alg(d["Nu"], d["Nphi"]) for d in disks) / zones
+    fracs = {"hbm_frac_alg": batched * balg / HBM_PEAK,
+             # the same for a design that kept the whole step on chip
+             "hbm_frac_onchip_bound": batched * ONCHIP_B_PER_ZONE / HBM_PEAK}
     if steps_seq <= 0:
         return {"disks": 64, "zones": zones, "steps": steps_batch, "graph_chains": nchains,
-                "batched_zone_updates_per_s": batched, "kernels": kern}
+                "batched_zone_updates_per_s": batched, "kernels": kern, **fracs}
     for d, ic in zip(disks, ics):
         e = dfk.Ensemble(ic[4])
         autotune.apply(e, regime="latency")
@@ -143,13 +148,11 @@ def ensemble_legs(dfk, steps_batch, steps_seq, chains=0):
         seq_zones += d["Nu"] * d["Nphi"] * steps_seq
         e.close()
     seq = seq_zones / seq_time
-    balg = sum(b_alg(d["Nu"], d["Nphi"]) for d in disks) / zones
     return {"disks": 64, "zones": zones, "steps": steps_batch, "graph_chains": nchains,
             "batched_zone_updates_per_s": batched,
             "sequential_zone_updates_per_s": seq,
             "speedup_vs_sequential": batched / seq, "tuned_disks": ntuned,
-            "kernels": kern,
-            "hbm_frac_alg": batched * balg / HBM_PEAK}
+            "kernels": kern, **fracs}
 
 
 def main():
@@ -227,6 +230,7 @@ def main():
                                f"CFL {args.cfl}, Kalnajs FFT self-gravity",
                    "Nu": Nu, "Nphi": Nphi, "global_batch": 1, "parallelism": "none"},
         "hbm_frac_step": zps * b_alg(Nu, Nphi) / (Nu * Nphi) / HBM_PEAK,
+        "hbm_frac_onchip_bound": zps * ONCHIP_B_PER_ZONE / HBM_PEAK,
         "device_ms_per_step": r["dev_ms"] / args.steps,
         "kernel_ms": kms,
         "shapes": r["shapes"],
