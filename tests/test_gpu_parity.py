@@ -380,6 +380,50 @@ def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps):
     assert e.status()["steps"] == 10 and max(errs) <= LOOP_TOL
 
 
+@pytest.mark.parametrize("rows_ip", [0, 1])
+def test_mixed_transform_modes_ensemble(orc, dfk, rows_ip):
+    """One handle whose disks use different column-convolution modes
+    (cooperative, wave-local, workgroup in place batched and single) and
+    batch sizes gives each disk the same bits as running it alone with the
+    same options."""
+    print(f"case mixed transform modes, rows in place {rows_ip}", flush=True)
+    # every Nphi factors into 2/3/4/8 (in-place rows are a handle-wide choice);
+    # 2 x 40 = 80 has a factor 5, so that disk keeps the cooperative columns
+    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 24, 0.04), (40, 128, 0.01), (24, 24, 0.03)]
+    opts = [(1, 1, 0, 2), (1, 0, 1, 3), (0, 0, 2, 2), (1, 0, 2, 1), (1, 1, 2, 4)]   # tw rows, tw cols, mode, cb
+    params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
+    ics, mDs = [], []
+    for a, b, c in shapes:
+        h, _ = _lka98(orc, a, b, c)
+        ics.append(h.state())
+        mDs.append(h.disk_mass())
+
+    def setup(e, ds):
+        e.set_update_kernel(1)
+        e.set_row_transform(rows_ip)
+        e.set_threads(128, 128, 256)
+        for q, d in enumerate(ds):
+            e.set_fft_options(*opts[d][:3], d=q)
+            e.set_shapes(0, 0, 2, opts[d][3], d=q)
+            e.set_state(*ics[d], d=q)
+
+    ens = dfk.Ensemble(params)
+    setup(ens, range(len(shapes)))
+    ens.run_init(1e9, mD=mDs, max_steps=12)
+    ens.advance(12)
+    for d, p in enumerate(params):
+        one = dfk.Ensemble(p)
+        setup(one, [d])
+        assert one.get_fft_options(0) == ens.get_fft_options(d)
+        assert one.get_shapes(0) == ens.get_shapes(d)
+        one.run_init(1e9, mD=mDs[d], max_steps=12)
+        one.advance(12)
+        for a, b in zip(ens.get_state(d), one.get_state()):
+            assert np.array_equal(a, b)
+        one.close()
+    ens.close()
+
+
 @pytest.mark.parametrize("chains", [2, 3])
 def test_graph_chains_same_bits(orc, dfk, chains):
     """Disks stepped as parallel graph chains (dfk_set_chains) give the same
