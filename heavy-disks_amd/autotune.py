@@ -5,7 +5,11 @@ The knobs (see dfk_set_shapes in include/diskfft.h) act on separate kernels:
   pr        complex rows (two real rows each) per workgroup of the row r2c/c2r
   cb        columns per workgroup of the column convolution
 so each is tuned on its own kernel's time, measured with HIP events on the
-library's stream (dfk_profile_step).  Two regimes:
+library's stream (dfk_profile_step).  In the latency regime the tuner then
+compares the whole iteration (row r2c with the control, convolution, row
+c2r, update) against the fused row kernels (dfk_set_fused_rows: convolution
+with the control + one kernel per block of ti full rows) over ti and threads
+per workgroup, and keeps the faster loop.  Two regimes:
   "latency"     one disk alone (few workgroups; launch/latency bound)
   "throughput"  enough copies of the disk to hold >= 8M zones (bandwidth bound),
                 which is what a disk sees inside a large batched ensemble (the
@@ -64,9 +68,12 @@ def lookup(Nu, Nphi, regime="throughput", path=CACHE, device="gfx950"):
 
 def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
     """Set every disk of an Ensemble to its cached shapes (where present), and
-    the handle's threads per workgroup to the most common tuned choice."""
+    the handle's threads per workgroup, update kernel, row transform and
+    fused row kernels to the most common tuned choice (votes weighted by
+    zones).  A disk tuned for fused rows gets its fused row-block height when
+    the handle runs them."""
     cache = load_cache(path)
-    n, votes, kinds, rows = 0, {}, {}, {}
+    n, votes, kinds, rows, fused = 0, {}, {}, {}, {}
     found = [cache.get(_key(*ens.shape(d), regime, device)) for d in range(ens.n)]
     for d, c in enumerate(found):
         if c:
@@ -75,13 +82,17 @@ def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
             kinds[k] = kinds.get(k, 0) + z
             r = c.get("rows_inplace", 0)
             rows[r] = rows.get(r, 0) + z
+            f = c.get("fused", {}).get("threads", 0)
+            fused[f] = fused.get(f, 0) + z
     if kinds:
         ens.set_update_kernel(max(kinds, key=kinds.get))
     if rows:
         ens.set_row_transform(max(rows, key=rows.get))
+    fthr = max(fused, key=fused.get) if fused else 0
     for d, c in enumerate(found):
         if c:
-            ens.set_shapes(c["ti"], c["tj"], c["pr"], c["cb"], d=d)
+            ti = c["fused"]["ti"] if fthr and c.get("fused", {}).get("threads") else c["ti"]
+            ens.set_shapes(ti, c["tj"], c["pr"], c["cb"], d=d)
             ens.set_fft_options(*c.get("fft", list(c.get("tw_lds", (1, 1))) + [0]), d=d)
             if "threads" in c:
                 t = tuple(c["threads"])
@@ -89,6 +100,8 @@ def apply(ens, path=CACHE, regime="throughput", device="gfx950"):
             n += 1
     if votes:
         ens.set_threads(*max(votes, key=votes.get))
+    if fthr:
+        ens.set_fused_rows(fthr)
     return n
 
 
@@ -177,9 +190,30 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     thr = (pick(1), pick(2), pick(4))
     e.set_threads(*thr)
     best = prof(ti, tj, bpr, bcb)
+    # latency regime: the four-launch iteration against fused row kernels
+    fused = {}
+    if regime == "latency":
+        step4 = float(sum(best[1:]))       # the control rides in the row r2c launch
+        res["fused"] = {}
+        if e.get_row_transform():
+            e.set_update_kernel(0)
+            for fti in (2, 4, 6, 8):
+                for fth in (256, 512, 1024):
+                    e.set_shapes(fti, tj if kind == 0 else 0, bpr, bcb)
+                    e.set_fused_rows(fth)
+                    if e.get_fused_rows():
+                        ms = prof(fti, 0, bpr, bcb)
+                        res["fused"][f"{fti}:{fth}"] = float(ms[2] + ms[4])
+            e.set_fused_rows(0)
+            e.set_update_kernel(kind)
+        if res["fused"]:
+            bf = min(res["fused"], key=res["fused"].get)
+            if res["fused"][bf] < step4:
+                fti, fth = (int(x) for x in bf.split(":"))
+                fused = {"threads": fth, "ti": fti, "step_ms": res["fused"][bf], "step4_ms": step4}
     e.close()
     out = {"kind": kind, "ti": ti, "tj": tj, "pr": bpr, "cb": bcb, "threads": list(thr),
-           "fft": [twr, twc, cwave], "rows_inplace": rip,
+           "fft": [twr, twc, cwave], "rows_inplace": rip, "fused": fused,
            "copies": copies,
            "ms_per_kernel": [float(x) for x in best],
            "default": {"shape": list(base), "tile_ms": res["tile"].get(f"{base[0]}x{base[1]}")},
@@ -187,7 +221,7 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     if verbose:
         print(f"{Nu}x{Nphi} {regime} x{copies}: kernel {kind} tile {ti}x{tj} pr {bpr} rows in place {rip} cb {bcb} "
               f"twiddles in LDS (rows, cols) {(twr, twc)} wave columns {cwave} threads {thr} "
-              f"ms {np.round(best, 4).tolist()}", flush=True)
+              f"ms {np.round(best, 4).tolist()} fused rows {fused or 'no'}", flush=True)
     return out
 
 

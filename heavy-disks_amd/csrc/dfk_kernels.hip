@@ -621,6 +621,18 @@ __device__ void control_disk(const DiskDev &D, DiskCtl &C, int d, const double *
   C.cur = 1 - par;   // where this iteration's update writes
 }
 
+// The loop control of disk d run by a trailing workgroup of another launch
+// (row r2c, or column convolution with fused row kernels): that launch's
+// other workgroups need none of its results.  Workgroups of a disk that it
+// stops read `done` racily; they then only transform a spectrum that no
+// later kernel of the disk reads.
+__device__ __forceinline__ void trailing_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
+                                                 int d, const double *__restrict__ pcfl,
+                                                 const double *__restrict__ pcm,
+                                                 DfkSampleDev *__restrict__ samples, int cap, int par) {
+  if (!ctl[d].done) control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
+}
+
 // Control of every disk as a launch of its own (one workgroup per disk).  In
 // the time loop the same work rides in the row-transform launch instead
 // (trailing workgroups of k_rows_r2c, which needs nothing of it).
@@ -677,6 +689,58 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
 }
 
 // ------------------------------------------------------------------ gravity
+// Row transform I/O shared by the row kernels.  r2c_store: the spectra of
+// `npair` transformed row pairs in Z (pair p = rows row0 + 2p, row0 + 2p + 1
+// packed as real and imaginary parts) split into the two rows' half spectra
+// and stored to spec[k][row] (rows < Nu).  c2r_load: the Hermitian rebuild of
+// the same packing from spec rows row0 .. row0 + 2 npair - 1 (rows >= Nu read
+// as zero; FFTW c2r convention: the imaginary parts of the DC and Nyquist
+// bins are ignored).
+__device__ __forceinline__ void r2c_store(const double2 *Z, double2 *__restrict__ out, int row0,
+                                          int npair, int n, int Nc, int Nu) {
+  const int w2 = 2 * npair;
+  for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
+    const int k = qdiv(e, rcpf((float)w2)), rr = e - k * w2;
+    const int p = rr >> 1, isb = rr & 1;
+    const int row = row0 + rr;
+    if (row >= Nu) continue;
+    const double2 z = Z[p * n + k];
+    double2 y = Z[p * n + (k == 0 ? 0 : n - k)];
+    y.y = -y.y;
+    double2 X;
+    if (!isb) X = make_double2(0.5 * (z.x + y.x), 0.5 * (z.y + y.y));
+    else X = make_double2(0.5 * (z.y - y.y), -0.5 * (z.x - y.x));
+    out[(i64)k * Nu + row] = X;
+  }
+}
+
+// Bin k of row pair p (rows a, b with spectra xa, xb) into the packed c2r input.
+__device__ __forceinline__ void c2r_put(double2 *A, int p, int k, double2 xa, double2 xb, int n, int Nc) {
+  if (k == 0 || 2 * k == n) { xa.y = 0.0; xb.y = 0.0; }
+  A[p * n + k] = make_double2(xa.x - xb.y, xa.y + xb.x);
+  if (k > 0 && n - k >= Nc) A[p * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
+}
+
+__device__ __forceinline__ void c2r_load(const double2 *__restrict__ in, double2 *A, int row0,
+                                         int npair, int n, int Nc, int Nu) {
+  for (int e = threadIdx.x; e < npair * Nc; e += blockDim.x) {
+    const int k = qdiv(e, rcpf((float)npair)), p = e - k * npair;
+    const int ra = row0 + 2 * p, rb = ra + 1;
+    double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
+    if (ra < Nu) xa = in[(i64)k * Nu + ra];
+    if (rb < Nu) xb = in[(i64)k * Nu + rb];
+    c2r_put(A, p, k, xa, xb, n, Nc);
+  }
+}
+
+// Doubles of the tile part of k_rows_fused's LDS (k_update's layout for a
+// TI x n tile), rounded up to whole double2; and of its staged state rows
+// (Sigma on rows i0-3 .. i0+TI+2, v_R and v_phi on i0-2 .. i0+TI+1).
+__host__ __device__ __forceinline__ int fused_tile_doubles(int TI, int n) {
+  return (3 * (TI + 4) * (n + 4) + 3 * (TI + 1) * (n + 4) + 1) & ~1;
+}
+__host__ __device__ __forceinline__ int fused_stage_doubles(int TI, int n) { return (3 * TI + 14) * n; }
+
 // Row r2c: S = R^{3/2} Sigma for rows [2 pr g, 2 pr (g+1)) of the current
 // state.  TG = 0: the workgroup-cooperative transform (ping-pong buffers);
 // TG > 0: in place by the TG threads of the workgroup (one buffer of pr rows,
@@ -690,12 +754,8 @@ __device__ __forceinline__ void rows_r2c_body(const DiskDev *__restrict__ disks,
                                               const double *__restrict__ pcfl, const double *__restrict__ pcm,
                                               DfkSampleDev *__restrict__ samples, int cap) {
   extern __shared__ double2 lds2[];
-  if ((int)blockIdx.x >= nrow_blk) {
-    // trailing workgroups: the loop control of disk ctl0 + blockIdx.x - nrow_blk.
-    // Row blocks of a disk that it stops read `done` racily; they then only
-    // transform a spectrum that no later kernel of the disk reads.
-    const int d = ctl0 + blockIdx.x - nrow_blk;
-    if (!ctl[d].done) control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
+  if ((int)blockIdx.x >= nrow_blk) {   // the loop control of disk ctl0 + blockIdx.x - nrow_blk
+    trailing_control(disks, ctl, ctl0 + blockIdx.x - nrow_blk, pcfl, pcm, samples, cap, par);
     return;
   }
   const int2 wm = map[map_base + blockIdx.x];
@@ -731,21 +791,7 @@ __device__ __forceinline__ void rows_r2c_body(const DiskDev *__restrict__ disks,
       Ns *= D.rad_row[q];
     }
   }
-  double2 *out = spec + D.spo;
-  const int w2 = 2 * pr;
-  for (int e = threadIdx.x; e < w2 * Nc; e += blockDim.x) {
-    const int k = qdiv(e, rcpf((float)w2)), rr = e - k * w2;
-    const int p = rr >> 1, isb = rr & 1;
-    const int row = row0 + rr;
-    if (row >= Nu) continue;
-    const double2 z = Z[p * n + k];
-    double2 y = Z[p * n + (k == 0 ? 0 : n - k)];
-    y.y = -y.y;
-    double2 X;
-    if (!isb) X = make_double2(0.5 * (z.x + y.x), 0.5 * (z.y + y.y));
-    else X = make_double2(0.5 * (z.y - y.y), -0.5 * (z.x - y.x));
-    out[(i64)k * Nu + row] = X;
-  }
+  r2c_store(Z, spec + D.spo, row0, pr, n, Nc, Nu);
 }
 
 #define R2C_PARAMS                                                                              \
@@ -764,11 +810,24 @@ __global__ void __launch_bounds__(TG) k_rows_r2c_group(R2C_PARAMS) { rows_r2c_bo
 
 // Column convolution: for columns [cb g, cb (g+1)): zero-padded forward
 // transform of length M, multiply by real K-hat, inverse, keep rows < Nu.
+// Loop control in trailing workgroups of a column-convolution launch (the
+// time loop with fused row kernels): workgroups nconv .. run the control of
+// disks ctl0, ctl0 + 1, ...
+#define CONV_CTL_PARAMS                                                                         \
+  int nconv, int ctl0, const double *__restrict__ pcfl, const double *__restrict__ pcm,         \
+      DfkSampleDev *__restrict__ samples, int cap, int par
+#define CONV_CONTROL                                                                            \
+  if ((int)blockIdx.x >= nconv) {                                                               \
+    trailing_control(disks, ctl, ctl0 + blockIdx.x - nconv, pcfl, pcm, samples, cap, par);     \
+    return;                                                                                     \
+  }
+
 __global__ void __launch_bounds__(WG)
 k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
-            const DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
-            const double2 *__restrict__ tw, double2 *__restrict__ spec) {
+            DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
+            const double2 *__restrict__ tw, double2 *__restrict__ spec, CONV_CTL_PARAMS) {
   extern __shared__ double2 lds2[];
+  CONV_CONTROL
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
   if (ctl[wm.x].done) return;
@@ -840,9 +899,10 @@ k_cols_conv(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int
 // (and so the occupancy) of the workgroup-cooperative kernel too.
 __global__ void __launch_bounds__(WG)
 k_cols_conv_wave(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
-                 const DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
-                 const double2 *__restrict__ tw, double2 *__restrict__ spec) {
+                 DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
+                 const double2 *__restrict__ tw, double2 *__restrict__ spec, CONV_CTL_PARAMS) {
   extern __shared__ double2 lds2[];
+  CONV_CONTROL
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
   if (ctl[wm.x].done) return;
@@ -872,9 +932,10 @@ k_cols_conv_wave(const DiskDev *__restrict__ disks, const int2 *__restrict__ map
 template <int TG, bool BATCH>
 __global__ void __launch_bounds__(TG)
 k_cols_conv_group(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
-                  const DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
-                  const double2 *__restrict__ tw, double2 *__restrict__ spec) {
+                  DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
+                  const double2 *__restrict__ tw, double2 *__restrict__ spec, CONV_CTL_PARAMS) {
   extern __shared__ double2 lds2[];
+  CONV_CONTROL
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
   if (ctl[wm.x].done) return;
@@ -904,17 +965,7 @@ __device__ __forceinline__ void rows_c2r_body(const DiskDev *__restrict__ disks,
   constexpr int NBUF = TG ? 1 : 2;
   double2 *A = lds2, *B = lds2 + pr * n;
   const double2 *TW = tw_table(lds2 + NBUF * pr * n, tw + D.two_row, n, D.tw_lds & 1);
-  const double2 *in = spec + D.spo;
-  for (int e = threadIdx.x; e < pr * Nc; e += blockDim.x) {
-    const int k = qdiv(e, rcpf((float)pr)), p = e - k * pr;
-    const int ra = row0 + 2 * p, rb = ra + 1;
-    double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
-    if (ra < Nu) xa = in[(i64)k * Nu + ra];
-    if (rb < Nu) xb = in[(i64)k * Nu + rb];
-    if (k == 0 || 2 * k == n) { xa.y = 0.0; xb.y = 0.0; }
-    A[p * n + k] = make_double2(xa.x - xb.y, xa.y + xb.x);
-    if (k > 0 && n - k >= Nc) A[p * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
-  }
+  c2r_load(spec + D.spo, A, row0, pr, n, Nc, Nu);
   __syncthreads();
   const double *sc = gtab(geom, D, GT_PHISC);
   double *P = phi + D.so;
@@ -1031,33 +1082,27 @@ k_khat_cols(const DiskDev *__restrict__ D_, const double2 *__restrict__ tw,
 // state on (ti+4) x (tj+4) zones, the radial face fluxes, then the post-radial
 // state and the azimuthal face fluxes.  GARR: gravity from given g_R/g_phi
 // arrays (unit entry point) instead of the potential.
-template <bool GARR, bool G2>
-__global__ void __launch_bounds__(WG)
-k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
-         const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
-         const double *__restrict__ phi, const double *__restrict__ gRa,
-         const double *__restrict__ gpa, int has_gR, int has_gp,
-         const double *__restrict__ geom, const double *__restrict__ trig,
-         double *__restrict__ pcfl, double *__restrict__ pcm, int par) {
-  extern __shared__ double lds[];
-  __shared__ double red[5 * (WG / 64)];
-  const int2 wm = map[map_base + blockIdx.x];
-  const DiskDev &D = disks[wm.x];
-  const DiskCtl &C = ctl[wm.x];
-  if (C.done) return;
-  const double dt = C.dt;
+// The body of k_update for tile `tile` of disk D (LDS from `lds`, workgroup
+// reduction scratch `red`).  ld(f, i, j): field f (0 Sigma, 1 v_R, 2 v_phi)
+// of the current state at zone (i, j); phi(i, j): the potential there, read
+// only for self-gravity; sink(lr, j, s): called with the new Sigma of every
+// zone (row i0 + lr, column j < Nphi) of the tile after it is stored, and
+// with 0 for the tile's rows past the disk.
+template <bool GARR, bool G2, class LdF, class PhiF, class SinkF>
+__device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, int tile, double *lds,
+                                                 double *red, double *__restrict__ st, i64 ntot,
+                                                 const double *__restrict__ gRa,
+                                                 const double *__restrict__ gpa, int has_gR, int has_gp,
+                                                 const double *__restrict__ geom,
+                                                 const double *__restrict__ trig,
+                                                 double *__restrict__ pcfl, double *__restrict__ pcm,
+                                                 int par, LdF ld, PhiF phi, SinkF sink) {
   const int Nu = D.Nu, Nphi = D.Nphi, TI = D.ti, TJ = D.tj;
-  const int ti = wm.y / D.ntile_j, tj = wm.y - ti * D.ntile_j;
+  const int ti = tile / D.ntile_j, tj = tile - ti * D.ntile_j;
   const int i0 = ti * TI, j0 = tj * TJ;
   const int SW = TJ + 4, SH = TI + 4;
   double *sS = lds, *sU = sS + SH * SW, *sW = sU + SH * SW;
   double *fM = sW + SH * SW, *fU = fM + (TI + 1) * SW, *fW = fU + (TI + 1) * SW;
-
-  const double *S = stp(st, ntot, par, 0, D), *U = stp(st, ntot, par, 1, D),
-               *W = stp(st, ntot, par, 2, D);
-  const double *Ph = phi + D.so;
-  const double *tR = gtab(geom, D, GT_R);
-  (void)tR;
   const double *tInvdRc = gtab(geom, D, GT_INVDRC), *tInvR = gtab(geom, D, GT_INVR),
                *tGstar = gtab(geom, D, GT_GSTAR), *tGrc = gtab(geom, D, GT_GRC),
                *tGpc = gtab(geom, D, GT_GPC);
@@ -1072,9 +1117,9 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
     const int jm = wrapj(j - 1, Nphi), jp = wrapj(j + 1, Nphi);
     const int im = (i > 0) ? i - 1 : 0, ip = (i < Nu - 1) ? i + 1 : Nu - 1;
     const i64 c = (i64)i * Nphi + j;
-    const double s = S[c], u = U[c], w = W[c];
-    const double dPdR = (pres<G2>(D, S[(i64)ip * Nphi + j]) - pres<G2>(D, S[(i64)im * Nphi + j])) * tInvdRc[i];
-    const double dPdphi = (pres<G2>(D, S[(i64)i * Nphi + jp]) - pres<G2>(D, S[(i64)i * Nphi + jm])) * D.inv2dphi;
+    const double s = ld(0, i, j), u = ld(1, i, j), w = ld(2, i, j);
+    const double dPdR = (pres<G2>(D, ld(0, ip, j)) - pres<G2>(D, ld(0, im, j))) * tInvdRc[i];
+    const double dPdphi = (pres<G2>(D, ld(0, i, jp)) - pres<G2>(D, ld(0, i, jm))) * D.inv2dphi;
     double gR = tGstar[i];
     double gp = 0.0;
     if (GARR) {
@@ -1083,8 +1128,8 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
     } else if (sg) {
       // kalnajs.hpp:71-78, one-sided at the radial edges
       const int iu = (i < Nu - 1) ? i + 1 : i, id = (i > 0) ? i - 1 : i;
-      gR += -(Ph[(i64)iu * Nphi + j] - Ph[(i64)id * Nphi + j]) * tGrc[i];
-      gp += -(Ph[(i64)i * Nphi + jp] - Ph[(i64)i * Nphi + jm]) * tGpc[i];
+      gR += -(phi(iu, j) - phi(id, j)) * tGrc[i];
+      gp += -(phi(i, jp) - phi(i, jm)) * tGpc[i];
     }
     const double rs = frcp(s);
     const double invR = tInvR[i];
@@ -1188,7 +1233,10 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   for (int e = threadIdx.x; e < TI * TJ; e += blockDim.x) {
     const int lr = qdiv(e, rcpf((float)TJ)), lc = e - lr * TJ;
     const int i = i0 + lr, j = j0 + lc;
-    if (i >= Nu || j >= Nphi) continue;
+    if (i >= Nu || j >= Nphi) {
+      if (j < Nphi) sink(lr, j, 0.0);
+      continue;
+    }
     const int l = (lr + 2) * SW + lc + 2;
     const int f = lr * FW + lc;
     const double s = sS[l];
@@ -1202,6 +1250,7 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
     const double uo = nmR * rn, wo = nmP * rn;
     const i64 c = (i64)i * Nphi + j;
     So[c] = so; Uo[c] = uo; Wo[c] = wo;
+    sink(lr, j, so);
     const double r = cfl_rate<G2>(D, so, uo, wo, tInvdR[i], tInvdl[i]);
     mx = (r > mx) ? r : mx;
     a0 += so * dA[i];
@@ -1211,10 +1260,165 @@ k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int ma
   }
   wg_reduce5(mx, a0, a1, a2, a3, red);
   if (threadIdx.x == 0) {
-    const i64 t = D.part + wm.y;
+    const i64 t = D.part + tile;
     pcfl[t] = mx;
     pcm[4 * t + 0] = a0; pcm[4 * t + 1] = a1; pcm[4 * t + 2] = a2; pcm[4 * t + 3] = a3;
   }
+}
+
+template <bool GARR, bool G2>
+__global__ void __launch_bounds__(WG)
+k_update(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+         const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
+         const double *__restrict__ phi, const double *__restrict__ gRa,
+         const double *__restrict__ gpa, int has_gR, int has_gp,
+         const double *__restrict__ geom, const double *__restrict__ trig,
+         double *__restrict__ pcfl, double *__restrict__ pcm, int par) {
+  extern __shared__ double lds[];
+  __shared__ double red[5 * (WG / 64)];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  const DiskCtl &C = ctl[wm.x];
+  if (C.done) return;
+  const double *Ph = phi + D.so;
+  const double *S = stp(st, ntot, par, 0, D), *U = stp(st, ntot, par, 1, D),
+               *W = stp(st, ntot, par, 2, D);
+  const int Nphi = D.Nphi;
+  update_tile_body<GARR, G2>(
+      D, C.dt, wm.y, lds, red, st, ntot, gRa, gpa, has_gR, has_gp, geom, trig, pcfl, pcm, par,
+      [&](int f, int i, int j) { return (f == 0 ? S : (f == 1 ? U : W))[(i64)i * Nphi + j]; },
+      [&](int i, int j) { return Ph[(i64)i * Nphi + j]; }, [](int, int, double) {});
+}
+
+// ------------------------------------------------------------------ fused row kernel
+// One loop iteration's row work for the rows [i0, i0 + TI) of a disk at full
+// width (TI = D.ti, even; the layout makes the tile TI x Nphi), once the
+// column convolution has left V-hat in `vin`:
+//   1. row c2r of the row pairs covering i0-3 .. i0+TI+2 (the rows the
+//      forces of the tile read) into LDS, as k_rows_c2r (Phi = pref V /
+//      sqrt(R), applied on use);
+//   2. the update of k_update on the tile, forces from that Phi;
+//   3. row r2c of the tile's new Sigma, as k_rows_r2c: S-hat of the new state
+//      into `sout` (a buffer other than vin: a workgroup reads V-hat rows
+//      that its neighbours overwrite with S-hat), which the next iteration's
+//      convolution transforms in place.
+// Phi, the momenta and the row spectra never leave the workgroup.  LDS: the
+// tile (as k_update), (TI/2 + 4) c2r row pairs, the staged state rows (whose
+// space the TI/2 r2c row pairs reuse), then the twiddles if staged.
+template <int TG, bool G2>
+__global__ void __launch_bounds__(TG)
+k_rows_fused(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+             const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
+             const double2 *__restrict__ vin, double2 *__restrict__ sout,
+             const double *__restrict__ geom, const double2 *__restrict__ tw,
+             const double *__restrict__ trig, double *__restrict__ pcfl, double *__restrict__ pcm,
+             int par) {
+  extern __shared__ double lds[];
+  __shared__ double red[5 * (TG / 64)];
+  constexpr int NMAX = ip_nmax<TG>();
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  const DiskCtl &C = ctl[wm.x];
+  if (C.done) return;
+  const int Nu = D.Nu, n = D.Nphi, Nc = D.Nc, TI = D.ti;
+  const int i0 = wm.y * TI;
+  double2 *F = (double2 *)(lds + fused_tile_doubles(TI, n));   // c2r row pairs
+  double *Sst = (double *)(F + (TI / 2 + 4) * n);              // staged state rows
+  double *Ust = Sst + (TI + 6) * n, *Wst = Ust + (TI + 4) * n;
+  double2 *Rb = (double2 *)Sst;   // r2c row pairs (the staged rows are dead by then)
+  const double2 *TW = tw_table((double2 *)(Sst + fused_stage_doubles(TI, n)), tw + D.two_row, n,
+                               D.tw_lds & 1);
+  const int pa = max(i0 - 4, 0) >> 1;                      // first c2r row pair
+  const int np = min(i0 / 2 + TI / 2 + 2, (Nu + 1) / 2) - pa;   // c2r row pairs
+  const int nrp = (min(TI, Nu - i0) + 1) >> 1;             // the tile's row pairs
+  const int maxb = NMAX / n;                               // transforms per in-place pass
+  // Stage everything the workgroup reads from HBM -- the V-hat bins of the
+  // c2r row pairs and the state rows of the tile's stencil -- with up to
+  // 2 + 8 loads in flight per thread before the first LDS store, so that the
+  // kernel waits on about one memory latency instead of one per pass.
+  {
+    const double2 *in = vin + D.spo;
+    const double *S = stp(st, ntot, par, 0, D), *U = stp(st, ntot, par, 1, D),
+                 *W = stp(st, ntot, par, 2, D);
+    const int nsp = np * Nc, nS = (TI + 6) * n, nst = nS + 2 * (TI + 4) * n;
+    const float rnp = rcpf((float)np), rn = rcpf((float)n);
+    constexpr int B1 = 2, B2 = 8;
+    for (int base = 0; base * B1 < nsp || base * B2 < nst; base += TG) {
+      double2 xa[B1], xb[B1];
+      double v[B2];
+#pragma unroll
+      for (int u = 0; u < B1; ++u) {
+        const int e = base * B1 + u * TG + threadIdx.x;
+        xa[u] = xb[u] = make_double2(0.0, 0.0);
+        if (e < nsp) {
+          const int k = qdiv(e, rnp), p = e - k * np;
+          const int ra = 2 * (pa + p);
+          if (ra < Nu) xa[u] = in[(i64)k * Nu + ra];
+          if (ra + 1 < Nu) xb[u] = in[(i64)k * Nu + ra + 1];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < B2; ++u) {
+        const int e = base * B2 + u * TG + threadIdx.x;
+        v[u] = 0.0;
+        if (e < nst) {
+          const int f = e < nS ? 0 : (e < nS + (TI + 4) * n ? 1 : 2);
+          const int r = qdiv(f == 0 ? e : e - nS - (f - 1) * (TI + 4) * n, rn);
+          const int i = i0 - (f == 0 ? 3 : 2) + r;
+          const int j = (f == 0 ? e : e - nS - (f - 1) * (TI + 4) * n) - r * n;
+          if (i >= 0 && i < Nu) v[u] = (f == 0 ? S : (f == 1 ? U : W))[(i64)i * n + j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < B1; ++u) {
+        const int e = base * B1 + u * TG + threadIdx.x;
+        if (e < nsp) {
+          const int k = qdiv(e, rnp);
+          c2r_put(F, e - k * np, k, xa[u], xb[u], n, Nc);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < B2; ++u) {
+        const int e = base * B2 + u * TG + threadIdx.x;
+        if (e < nst) Sst[e] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  for (int q0 = 0; q0 < np; q0 += maxb) {
+    StageIO g{};
+    int Ns = 1;
+    for (int q = 0; q < D.nst_row; ++q) {
+      ip_stage_r<+1, IO_LDS, TG, NMAX>(D.rad_row[q], F + q0 * n, n, min(maxb, np - q0), Ns,
+                                      threadIdx.x, TW, g);
+      Ns *= D.rad_row[q];
+    }
+  }
+  const double *sc = gtab(geom, D, GT_PHISC), *R32 = gtab(geom, D, GT_R32);
+  update_tile_body<false, G2>(
+      D, C.dt, wm.y, lds, red, st, ntot, nullptr, nullptr, 0, 0, geom, trig, pcfl, pcm, par,
+      [&](int f, int i, int j) {
+        return f == 0 ? Sst[(i - i0 + 3) * n + j] : (f == 1 ? Ust : Wst)[(i - i0 + 2) * n + j];
+      },
+      [&](int i, int j) {
+        const double2 z = F[((i >> 1) - pa) * n + j];
+        return ((i & 1) ? z.y : z.x) * sc[i];
+      },
+      [&](int lr, int j, double so) {
+        double *x = (double *)&Rb[(lr >> 1) * n + j];
+        x[lr & 1] = (i0 + lr < Nu) ? R32[i0 + lr] * so : 0.0;
+      });
+  __syncthreads();
+  for (int q0 = 0; q0 < nrp; q0 += maxb) {
+    StageIO g{};
+    int Ns = 1;
+    for (int q = 0; q < D.nst_row; ++q) {
+      ip_stage_r<-1, IO_LDS, TG, NMAX>(D.rad_row[q], Rb + q0 * n, n, min(maxb, nrp - q0), Ns,
+                                      threadIdx.x, TW, g);
+      Ns *= D.rad_row[q];
+    }
+  }
+  r2c_store(Rb, sout + D.spo, i0, nrp, n, Nc, Nu);
 }
 
 // ------------------------------------------------------------------ streaming update
@@ -1650,32 +1854,58 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
   return hipGetLastError();
 }
 
+#define CONV_CTL_ARGS nblk, c.ctl0, c.pcfl, c.pcm, c.samples, c.cap, c.par
+
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                           const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                           const double2 *tw, double2 *spec) {
-  if (nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cols_conv, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec);
+                           const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                           const double2 *tw, double2 *spec, const DfkConvCtl &c) {
+  if (nblk + c.nctl <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cols_conv, dim3(nblk + c.nctl), dim3(nthr), lds, s, disks, map, base, ctl, khat,
+                     tw, spec, CONV_CTL_ARGS);
   return hipGetLastError();
 }
 
 hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                                const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                                const double2 *tw, double2 *spec) {
-  if (nblk <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cols_conv_wave, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat,
-                     tw, spec);
+                                const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                                const double2 *tw, double2 *spec, const DfkConvCtl &c) {
+  if (nblk + c.nctl <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cols_conv_wave, dim3(nblk + c.nctl), dim3(nthr), lds, s, disks, map, base, ctl,
+                     khat, tw, spec, CONV_CTL_ARGS);
   return hipGetLastError();
 }
 
 hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                                 const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                                 const double2 *tw, double2 *spec, int batched) {
-  if (nblk <= 0) return hipSuccess;
+                                 const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                                 const double2 *tw, double2 *spec, int batched, const DfkConvCtl &c) {
+  if (nblk + c.nctl <= 0) return hipSuccess;
   auto k = PICK_GROUP(k_cols_conv_group);
   if (nthr != 64 && nthr != 128 && nthr != 192 && nthr != 256) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec);
+  hipLaunchKernelGGL(k, dim3(nblk + c.nctl), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec,
+                     CONV_CTL_ARGS);
   return hipGetLastError();
 }
+
+hipError_t dfk_launch_rows_fused(hipStream_t s, int nblk, int nthr, size_t lds, int g2,
+                                 const DiskDev *disks, const int2 *map, int base, const DiskCtl *ctl,
+                                 double *st, i64 ntot, const double2 *vin, double2 *sout,
+                                 const double *geom, const double2 *tw, const double *trig,
+                                 double *pcfl, double *pcm, int par) {
+  if (nblk <= 0) return hipSuccess;
+  auto k = nthr == 256 ? (g2 ? k_rows_fused<256, true> : k_rows_fused<256, false>)
+         : nthr == 512 ? (g2 ? k_rows_fused<512, true> : k_rows_fused<512, false>)
+                       : (g2 ? k_rows_fused<1024, true> : k_rows_fused<1024, false>);
+  if (nthr != 256 && nthr != 512 && nthr != 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k, dim3(nblk), dim3(nthr), lds, s, disks, map, base, ctl, st, ntot, vin, sout, geom,
+                     tw, trig, pcfl, pcm, par);
+  return hipGetLastError();
+}
+
+size_t dfk_fused_lds(int ti, int nphi, int tw_lds) {
+  return (size_t)(fused_tile_doubles(ti, nphi) + fused_stage_doubles(ti, nphi)) * sizeof(double) +
+         (size_t)(ti / 2 + 4 + (tw_lds ? 1 : 0)) * nphi * sizeof(double2);
+}
+
+int dfk_fused_max_nphi(int nthr) { return ip_nmax<256>() * (nthr / 256); }
 
 hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,

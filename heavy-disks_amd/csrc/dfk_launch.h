@@ -20,16 +20,36 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           long long ntot, const double *geom, const double2 *tw, double2 *spec,
                           int par, int nctl, int ctl0, const double *pcfl, const double *pcm,
                           DfkSampleDev *samples, int cap, int inplace);
+// Loop control riding in a column-convolution launch: nctl trailing
+// workgroups run the control of disks ctl0 .. ctl0 + nctl - 1 (nctl = 0: none).
+struct DfkConvCtl {
+  int nctl = 0, ctl0 = 0;
+  const double *pcfl = nullptr, *pcm = nullptr;
+  DfkSampleDev *samples = nullptr;
+  int cap = 0, par = 0;
+};
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                           const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                           const double2 *tw, double2 *spec);
+                           const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                           const double2 *tw, double2 *spec, const DfkConvCtl &c);
 hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                                const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                                const double2 *tw, double2 *spec);
+                                const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                                const double2 *tw, double2 *spec, const DfkConvCtl &c);
 // batched: some disk of the launch has cb > 1 (else the one-column build).
 hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
-                                 const int2 *map, int base, const DiskCtl *ctl, const double *khat,
-                                 const double2 *tw, double2 *spec, int batched);
+                                 const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                                 const double2 *tw, double2 *spec, int batched, const DfkConvCtl &c);
+// Fused row kernel (row c2r + update + row r2c) over row blocks of ti x Nphi,
+// nthr = 256, 512 or 1024 threads; V-hat read from vin, the new state's
+// S-hat written to sout.
+hipError_t dfk_launch_rows_fused(hipStream_t s, int nblk, int nthr, size_t lds, int g2,
+                                 const DiskDev *disks, const int2 *map, int base, const DiskCtl *ctl,
+                                 double *st, long long ntot, const double2 *vin, double2 *sout,
+                                 const double *geom, const double2 *tw, const double *trig,
+                                 double *pcfl, double *pcm, int par);
+// Its dynamic LDS bytes for a disk, and the largest Nphi it transforms with
+// nthr threads.
+size_t dfk_fused_lds(int ti, int nphi, int tw_lds);
+int dfk_fused_max_nphi(int nthr);
 hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
                           const double *geom, const double2 *tw, double *phi, int inplace);

@@ -117,6 +117,7 @@ struct dfk_handle {
   double *st = nullptr;         // 2 buffers x 3 fields x ntot
   double *phi = nullptr, *gR = nullptr, *gp = nullptr, *F = nullptr;
   double2 *spec = nullptr, *tw = nullptr;
+  double2 *spec2 = nullptr;     // second half spectrum of the fused row kernels (states in buffer 1)
   double *khat = nullptr, *geom = nullptr, *trig = nullptr;
   // layout (rebuilt when shapes change)
   bool layout_ok = false;
@@ -146,6 +147,11 @@ struct dfk_handle {
   int rows_ip = 0;               // in use (every self-gravitating disk's rows allow it)
   int rows_batched = 0;          // in place with pr > 1 for some disk
   int conv_batched = 0;          // some disk's in-place workgroup convolution has cb > 1
+  int fused_req = 0;             // requested: fused row kernels (row c2r + update + row r2c)
+                                 // with this many threads per workgroup (0: off)
+  int fused = 0;                 // in use (LDS tile update; every disk self-gravitating with
+                                 // in-place row radices and a fitting tile)
+  size_t lds_fused = 0;
   std::vector<hipStream_t> side; // streams of chains 1 .. nchains-1 (default priority)
   std::vector<hipEvent_t> join;  // their join events; ev_fork starts them
   hipEvent_t ev_fork = nullptr;
@@ -194,11 +200,25 @@ int build_layout(dfk_handle *h) {
   h->rows_ip = h->row_inplace;
   for (const DiskDev &D : h->hD)
     if (D.selfgrav && !inplace_radices(D.rad_row, D.nst_row, 1)) h->rows_ip = 0;
+  // fused row kernels: tiles of ti x Nphi with ti even (whole row pairs)
+  h->fused = h->fused_req > 0 && h->upd_kind == 0;
+  for (const DiskDev &D : h->hD) {
+    const int tif = std::max(2, std::min(D.ti & ~1, D.Nu + (D.Nu & 1)));
+    if (!D.selfgrav || !inplace_radices(D.rad_row, D.nst_row, 1) ||
+        D.Nphi > dfk_fused_max_nphi(h->fused_req) || dfk_fused_lds(tif, D.Nphi, D.tw_lds & 1) > kMaxLds)
+      h->fused = 0;
+  }
+  size_t lf = 0;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
     if (h->upd_kind == 1) D.tj = kStreamCols;   // one wave = 60 output columns
+    if (h->fused) {
+      D.tj = D.Nphi;
+      D.ti = std::max(2, std::min(D.ti & ~1, D.Nu + (D.Nu & 1)));
+    }
     D.tj = std::max(1, std::min(D.tj, D.Nphi));
-    D.ti = std::max(1, std::min(D.ti, D.Nu));
+    D.ti = std::max(1, std::min(D.ti, h->fused ? D.Nu + (D.Nu & 1) : D.Nu));
+    if (h->fused) lf = std::max(lf, dfk_fused_lds(D.ti, D.Nphi, D.tw_lds & 1));
     D.ntile_i = ceil_div(D.Nu, D.ti);
     D.ntile_j = ceil_div(D.Nphi, D.tj);
     D.part = part;
@@ -258,11 +278,13 @@ int build_layout(dfk_handle *h) {
   h->conv0[h->nd] = (int)mc.size();
   h->convw0[h->nd] = (int)mcw.size();
   h->convg0[h->nd] = (int)mcg.size();
-  if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds || lcw > kMaxLds || lcg > kMaxLds) {
-    set_err("tile/batch shapes need %zu/%zu/%zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, lcw, lcg,
-            kMaxLds);
+  if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds || lcw > kMaxLds || lcg > kMaxLds || lf > kMaxLds) {
+    set_err("tile/batch shapes need %zu/%zu/%zu/%zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, lcw,
+            lcg, lf, kMaxLds);
     return DFK_EINVAL;
   }
+  h->lds_fused = lf;
+  if (h->fused && !h->spec2) TRY(dalloc(&h->spec2, h->nspec));
   h->lds_upd = lu;
   h->lds_r2c = lr;
   h->lds_conv = lc;
@@ -343,11 +365,16 @@ double *field(dfk_handle *h, int buf, int f, int d) {
 // ctl0 .. ctl0 + nctl - 1 in trailing workgroups of the same launch (nctl = 0:
 // none), on stream s.
 hipError_t launch_r2c(dfk_handle *h, hipStream_t s, int nblk, int base, int par, int nctl,
-                      int ctl0 = 0) {
+                      int ctl0 = 0, double2 *spec = nullptr) {
   return dfk_launch_r2c(s, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl,
-                        h->st, h->ntot, h->geom, h->tw, h->spec, par, nctl, ctl0, h->pcfl, h->pcm,
-                        h->samples, h->cap, h->rows_ip ? 1 + h->rows_batched : 0);
+                        h->st, h->ntot, h->geom, h->tw, spec ? spec : h->spec, par, nctl, ctl0,
+                        h->pcfl, h->pcm, h->samples, h->cap, h->rows_ip ? 1 + h->rows_batched : 0);
 }
+
+// The half spectrum that holds S-hat of the states in buffer par: with fused
+// row kernels, buffer 1's states have theirs in spec2 (the fused kernel
+// reads V-hat from one and writes S-hat of the new state to the other).
+double2 *spec_of(dfk_handle *h, int par) { return (h->fused && par) ? h->spec2 : h->spec; }
 
 // Row c2r of map range [base, base + nblk) on stream s.
 hipError_t launch_c2r(dfk_handle *h, hipStream_t s, int nblk, int base) {
@@ -355,20 +382,36 @@ hipError_t launch_c2r(dfk_handle *h, hipStream_t s, int nblk, int base) {
                         h->geom, h->tw, h->phi, h->rows_ip ? 1 + h->rows_batched : 0);
 }
 
-// Column convolution of disks [a, b) on stream s: each of the three column
-// kernels (cooperative, wave-local, workgroup in place) for the disks that
-// use it.
-hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b) {
+// Column convolution of disks [a, b) on stream s, in place on `spec` (null:
+// the handle's first spectrum): each of the three column kernels
+// (cooperative, wave-local, workgroup in place) for the disks that use it.
+// ctl_par >= 0: the last of these launches also runs the loop control of
+// disks a .. b-1 (state buffer ctl_par) in trailing workgroups.
+hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b, double2 *spec = nullptr,
+                             int ctl_par = -1) {
   const int n = h->conv0[b] - h->conv0[a], nw = h->convw0[b] - h->convw0[a];
   const int ng = h->convg0[b] - h->convg0[a];
+  if (!spec) spec = h->spec;
+  DfkConvCtl none, c;
+  if (ctl_par >= 0) {
+    c.nctl = b - a;
+    c.ctl0 = a;
+    c.pcfl = h->pcfl;
+    c.pcm = h->pcm;
+    c.samples = h->samples;
+    c.cap = h->cap;
+    c.par = ctl_par;
+  }
+  const int last = ng > 0 ? 2 : (nw > 0 ? 1 : 0);
   hipError_t e = dfk_launch_conv(s, n, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[a],
-                                 h->ctl, h->khat, h->tw, h->spec);
+                                 h->ctl, h->khat, h->tw, spec, last == 0 ? c : none);
   if (e != hipSuccess) return e;
   e = dfk_launch_conv_wave(s, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
-                           h->convw0[a], h->ctl, h->khat, h->tw, h->spec);
+                           h->convw0[a], h->ctl, h->khat, h->tw, spec, last == 1 ? c : none);
   if (e != hipSuccess) return e;
   return dfk_launch_conv_group(s, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
-                               h->convg0[a], h->ctl, h->khat, h->tw, h->spec, h->conv_batched);
+                               h->convg0[a], h->ctl, h->khat, h->tw, spec, h->conv_batched,
+                               last == 2 ? c : none);
 }
 
 // Column convolution of disk d (all disks for d < 0) on the handle's stream.
@@ -407,8 +450,22 @@ hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR
 // launches.  Every running disk's state is in buffer `par`; the update writes
 // buffer 1 - par.  The loop control rides in the r2c launch (a launch of its
 // own when no disk has self-gravity).
+// Fused row kernels: two launches, the column convolution of S-hat in
+// spec_of(par) (with the loop control in trailing workgroups) and the row
+// kernel, which leaves S-hat of the new state in spec_of(1 - par).
+int launch_step_fused(dfk_handle *h, hipStream_t s, int a, int b, int par) {
+  int g2 = 1;
+  for (const DiskDev &D : h->hD) g2 &= (D.gamma2 != 0);
+  HIPCHK(launch_conv_range(h, s, a, b, spec_of(h, par), par));
+  HIPCHK(dfk_launch_rows_fused(s, h->upd0[b] - h->upd0[a], h->fused_req, h->lds_fused, g2, h->dD,
+                               h->map_upd, h->upd0[a], h->ctl, h->st, h->ntot, spec_of(h, par),
+                               spec_of(h, 1 - par), h->geom, h->tw, h->trig, h->pcfl, h->pcm, par));
+  return DFK_OK;
+}
+
 int launch_step_all(dfk_handle *h, int par) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd];
+  if (h->fused) return launch_step_fused(h, h->stream, 0, h->nd, par);
   if (nr > 0) {
     HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd));
     HIPCHK(launch_conv(h, -1));
@@ -423,6 +480,7 @@ int launch_step_all(dfk_handle *h, int par) {
 // The same for the disks [a, b) on stream s (every one of them
 // self-gravitating, so the control rides in their r2c launch).
 int launch_step_range(dfk_handle *h, hipStream_t s, int a, int b, int par) {
+  if (h->fused) return launch_step_fused(h, s, a, b, par);
   const int nr = h->r2c0[b] - h->r2c0[a], nu = h->upd0[b] - h->upd0[a];
   HIPCHK(launch_r2c(h, s, nr, h->r2c0[a], par, b - a, a));
   HIPCHK(launch_conv_range(h, s, a, b));
@@ -818,7 +876,7 @@ void dfk_destroy(dfk_handle *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   drop_graphs(h);
   hipFree(h->dD); hipFree(h->st); hipFree(h->phi); hipFree(h->gR); hipFree(h->gp); hipFree(h->F);
-  hipFree(h->spec); hipFree(h->khat); hipFree(h->geom); hipFree(h->trig); hipFree(h->tw);
+  hipFree(h->spec); hipFree(h->spec2); hipFree(h->khat); hipFree(h->geom); hipFree(h->trig); hipFree(h->tw);
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv); hipFree(h->map_convw);
   hipFree(h->map_convg);
   hipFree(h->ctl); hipFree(h->samples);
@@ -1033,6 +1091,9 @@ int dfk_run_init(dfk_handle *h, const dfk_run_params *rp, int sample_cap) {
   HIPCHK(hipMemcpy(h->ctl, c.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
   HIPCHK(dfk_launch_diag(h->stream, h->upd0[h->nd], h->dD, h->map_upd, 0, h->ctl, h->st, h->ntot,
                          h->geom, h->trig, h->pcfl, h->pcm));
+  // fused row kernels: the first iteration's convolution needs S-hat of the
+  // initial state (later ones get it from the previous row kernel)
+  if (h->fused) HIPCHK(launch_r2c(h, h->stream, h->r2c0[h->nd], 0, 0, 0, 0, spec_of(h, 0)));
   HIPCHK(hipStreamSynchronize(h->stream));
   h->iter = 0;
   h->run_ready = true;
@@ -1184,6 +1245,9 @@ int dfk_snapshot_load(dfk_handle *h, int d, const char *path) {
   const int nb_ = h->upd0[d + 1] - h->upd0[d];
   HIPCHK(dfk_launch_diag(h->stream, nb_, h->dD, h->map_upd, h->upd0[d], h->ctl, h->st, h->ntot,
                          h->geom, h->trig, h->pcfl, h->pcm));
+  if (h->fused)   // and S-hat of the restored state, which the next convolution reads
+    HIPCHK(launch_r2c(h, h->stream, h->r2c0[d + 1] - h->r2c0[d], h->r2c0[d], cur, 0, 0,
+                      spec_of(h, cur)));
   HIPCHK(hipStreamSynchronize(h->stream));
   return DFK_OK;
 }
@@ -1269,6 +1333,27 @@ int dfk_get_row_transform(dfk_handle *h, int *inplace) {
   return DFK_OK;
 }
 
+int dfk_set_fused_rows(dfk_handle *h, int threads) {
+  if (!h || (threads != 0 && threads != 256 && threads != 512 && threads != 1024)) {
+    set_err("fused rows: threads per workgroup must be 0 (off), 256, 512 or 1024");
+    return DFK_EINVAL;
+  }
+  if (threads != h->fused_req) {
+    h->fused_req = threads;
+    h->layout_ok = false;
+    h->run_ready = false;   // the loop's spectra change meaning
+    return build_layout(h);
+  }
+  return DFK_OK;
+}
+
+int dfk_get_fused_rows(dfk_handle *h, int *threads) {
+  if (!h || !threads) { set_err("null argument"); return DFK_EINVAL; }
+  TRY(build_layout(h));
+  *threads = h->fused ? h->fused_req : 0;
+  return DFK_OK;
+}
+
 int dfk_set_update_kernel(dfk_handle *h, int kind) {
   if (!h || kind < 0 || kind > 1) { set_err("update kernel must be 0 (LDS tile) or 1 (streaming)"); return DFK_EINVAL; }
   if (kind != h->upd_kind) {
@@ -1338,6 +1423,10 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd];
   hipEvent_t e0 = h->ev0, e1 = h->ev1;
   for (int k = 0; k < 5; ++k) {
+    if (h->fused && (k == 1 || k == 3)) {   // no separate row transforms in the fused loop
+      ms[k] = 0.0;
+      continue;
+    }
     HIPCHK(hipMemcpy(h->ctl, c.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
     HIPCHK(hipEventRecord(e0, h->stream));
     for (int r = 0; r < nrep; ++r) {
@@ -1345,17 +1434,27 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
         case 0:
           HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
           break;
-        case 1:
-          HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd));   // as in the loop: with the control
+        case 1:   // as in the loop: with the control
+          HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd));
           break;
-        case 2:
-          HIPCHK(launch_conv(h, -1));
+        case 2:   // fused: with the control, on the spectrum of buffer par
+          if (h->fused) HIPCHK(launch_conv_range(h, h->stream, 0, h->nd, spec_of(h, par), par));
+          else HIPCHK(launch_conv(h, -1));
           break;
         case 3:
           HIPCHK(launch_c2r(h, h->stream, nr, 0));
           break;
-        case 4:
-          HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par));
+        case 4:   // fused: the row kernel (writes the other state buffer and spectrum)
+          if (h->fused) {
+            int g2 = 1;
+            for (const DiskDev &D : h->hD) g2 &= (D.gamma2 != 0);
+            HIPCHK(dfk_launch_rows_fused(h->stream, nu, h->fused_req, h->lds_fused, g2, h->dD,
+                                         h->map_upd, 0, h->ctl, h->st, h->ntot, spec_of(h, par),
+                                         spec_of(h, 1 - par), h->geom, h->tw, h->trig, h->pcfl,
+                                         h->pcm, par));
+          } else {
+            HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par));
+          }
           break;
       }
     }
@@ -1370,6 +1469,8 @@ int dfk_profile_step(dfk_handle *h, int nrep, double *ms) {
   HIPCHK(hipMemcpy(h->ctl, saved.data(), sizeof(DiskCtl) * h->nd, hipMemcpyHostToDevice));
   HIPCHK(dfk_launch_diag(h->stream, nu, h->dD, h->map_upd, 0, h->ctl, h->st, h->ntot, h->geom,
                          h->trig, h->pcfl, h->pcm));
+  // fused: the repeated convolutions transformed S-hat of buffer par in place
+  if (h->fused) HIPCHK(launch_r2c(h, h->stream, nr, 0, par, 0, 0, spec_of(h, par)));
   HIPCHK(hipStreamSynchronize(h->stream));
   return DFK_OK;
 }

@@ -34,6 +34,7 @@ SYMBOLS = [
     "dfk_get_shapes", "dfk_set_fft_options", "dfk_get_fft_options", "dfk_last_advance_ms", "dfk_set_graph_steps", "dfk_profile_step",
     "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
     "dfk_set_chains", "dfk_get_chains", "dfk_set_row_transform", "dfk_get_row_transform",
+    "dfk_set_fused_rows", "dfk_get_fused_rows",
     "dfk_snapshot_save", "dfk_snapshot_load", "dfk_set_eos",
 ]
 
@@ -108,6 +109,8 @@ def lib():
             "dfk_set_chains": (_I, [_P, _I]),
             "dfk_set_row_transform": (_I, [_P, _I]),
             "dfk_get_row_transform": (_I, [_P, _IP]),
+            "dfk_set_fused_rows": (_I, [_P, _I]),
+            "dfk_get_fused_rows": (_I, [_P, _IP]),
             "dfk_get_chains": (_I, [_P, _IP]),
             "dfk_snapshot_save": (_I, [_P, _I, ctypes.c_char_p]),
             "dfk_snapshot_load": (_I, [_P, _I, ctypes.c_char_p]),
@@ -323,6 +326,16 @@ class Ensemble:
     def get_row_transform(self):
         k = _I()
         _check(lib().dfk_get_row_transform(self.h, ctypes.byref(k)))
+        return k.value
+
+    def set_fused_rows(self, threads):
+        """Time loop as column convolution + fused row kernels of `threads`
+        (256, 512 or 1024) threads per workgroup, or four launches (0)."""
+        _check(lib().dfk_set_fused_rows(self.h, int(threads)))
+
+    def get_fused_rows(self):
+        k = _I()
+        _check(lib().dfk_get_fused_rows(self.h, ctypes.byref(k)))
         return k.value
 
     def set_chains(self, k):

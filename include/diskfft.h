@@ -186,6 +186,22 @@ int dfk_get_row_transform(dfk_handle *h, int *inplace);
 int dfk_set_update_kernel(dfk_handle *h, int kind);
 int dfk_get_update_kernel(dfk_handle *h, int *kind);
 
+/* Fused row kernels in the time loop (an autotuner knob; no counterpart in
+ * the reference, whose loop runs potential, forces and step one after
+ * another, diskmain.cpp:100-108).  threads = 256, 512 or 1024: each
+ * iteration is two launches, the column convolution (carrying the loop
+ * control) and one workgroup of that many threads per block of ti full rows
+ * that runs the row c2r of the rows its forces read, the update (LDS tile of
+ * ti x Nphi) and the row r2c of its new Sigma, so Phi and the row spectra
+ * never reach HBM; 0 = four launches (default).  Used when the update kernel
+ * is the LDS tile (kind 0) and every disk has self-gravity, Nphi <= 8 x
+ * threads factoring into 2, 3, 4 and 8, and a tile that fits the LDS (ti is
+ * rounded down to an even number, at least 2); dfk_get_fused_rows reports
+ * the threads in use (0: four launches).  Same step to round-off.  Changing
+ * it needs a new dfk_run_init. */
+int dfk_set_fused_rows(dfk_handle *h, int threads);
+int dfk_get_fused_rows(dfk_handle *h, int *threads);
+
 /* Graph chains (1 .. 4; no counterpart in the reference, which steps one
  * disk): the disks of the handle are split into that many contiguous groups
  * of about equal zone counts, and each group is stepped by its own chain of

@@ -49,6 +49,9 @@ class _FakeEnsemble:
     def set_row_transform(self, r):
         self.calls.append(("rows", r))
 
+    def set_fused_rows(self, t):
+        self.calls.append(("fused", t))
+
 
 def test_apply_votes_kind_and_threads_by_zones(tmp_path):
     import autotune
@@ -70,6 +73,30 @@ def test_apply_votes_kind_and_threads_by_zones(tmp_path):
     assert ("shapes", 0, 16, 60, 4, 4) in ens.calls and ("fft", 0, 1, 0, 1) in ens.calls
     assert ("fft", 1, 1, 1, 0) in ens.calls          # defaults when the entry has none
     assert not any(c[0] == "shapes" and c[1] == 3 for c in ens.calls)
+
+
+def test_apply_fused_rows_vote(tmp_path):
+    """Latency entries tuned for fused row kernels: the handle runs them when
+    the disks tuned for them hold most zones, and those disks then get their
+    fused row-block height; otherwise every disk keeps its tile."""
+    import autotune
+    fz = {"threads": 512, "ti": 2, "step_ms": 0.018, "step4_ms": 0.022}
+    cache = {
+        "64x64:latency:gfx950": {"kind": 0, "ti": 8, "tj": 16, "pr": 1, "cb": 1,
+                                 "threads": [256, 128, 256], "fused": fz},
+        "96x96:latency:gfx950": {"kind": 0, "ti": 4, "tj": 32, "pr": 1, "cb": 1,
+                                 "threads": [256, 128, 256], "fused": {}},
+    }
+    path = tmp_path / "cache.json"
+    path.write_text(json.dumps(cache))
+    ens = _FakeEnsemble([(64, 64), (64, 64), (64, 64), (96, 96)])   # 12288 vs 9216 zones
+    autotune.apply(ens, path=str(path), regime="latency")
+    assert ("fused", 512) in ens.calls
+    assert ("shapes", 0, 2, 16, 1, 1) in ens.calls and ("shapes", 3, 4, 32, 1, 1) in ens.calls
+    ens = _FakeEnsemble([(64, 64), (96, 96), (96, 96)])
+    autotune.apply(ens, path=str(path), regime="latency")
+    assert not any(c[0] == "fused" for c in ens.calls)
+    assert ("shapes", 0, 8, 16, 1, 1) in ens.calls
 
 
 def _counter_csv(d, rows):
