@@ -445,11 +445,6 @@ hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR
                            h->trig, h->pcfl, h->pcm, par);
 }
 
-
-// One whole loop iteration for every disk (captured into the graph): four
-// launches.  Every running disk's state is in buffer `par`; the update writes
-// buffer 1 - par.  The loop control rides in the r2c launch (a launch of its
-// own when no disk has self-gravity).
 // Fused row kernels: two launches, the column convolution of S-hat in
 // spec_of(par) (with the loop control in trailing workgroups) and the row
 // kernel, which leaves S-hat of the new state in spec_of(1 - par).
@@ -463,6 +458,10 @@ int launch_step_fused(dfk_handle *h, hipStream_t s, int a, int b, int par) {
   return DFK_OK;
 }
 
+// One whole loop iteration for every disk (captured into the graph): four
+// launches (two with fused row kernels).  Every running disk's state is in
+// buffer `par`; the update writes buffer 1 - par.  The loop control rides in
+// the r2c launch (a launch of its own when no disk has self-gravity).
 int launch_step_all(dfk_handle *h, int par) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd];
   if (h->fused) return launch_step_fused(h, h->stream, 0, h->nd, par);
