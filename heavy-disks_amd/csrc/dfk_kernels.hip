@@ -232,6 +232,48 @@ __device__ __forceinline__ void fft_stage_generic(const double2 *__restrict__ a,
 }
 
 // Copy the n twiddles of a transform into LDS (read many times per stage).
+// B passes of staged_for starting at element b0 + threadIdx.x.
+template <int B, class T, class Ld, class St>
+__device__ __forceinline__ void staged_pass(int b0, int total, Ld &ld, St &st) {
+  const int nt = blockDim.x;
+  T v[B];
+#pragma unroll
+  for (int u = 0; u < B; ++u) v[u] = ld(min(b0 + u * nt + (int)threadIdx.x, total - 1));
+#pragma unroll
+  for (int u = 0; u < B; ++u) {
+    const int e = b0 + u * nt + threadIdx.x;
+    if (e < total) st(e, v[u]);
+  }
+}
+
+// st(e, ld(e)) for e = threadIdx.x, threadIdx.x + blockDim.x, ... < total,
+// with the loads of up to B consecutive passes issued before the first
+// store: ld gets an index clamped into [0, total) and must load without
+// branching (clamped addresses, selects), so that the loads sit in one basic
+// block and the workgroup waits on about one memory latency per batch
+// instead of one (or more) per pass.  A batch is never longer than the
+// passes left (no redundant loads when one pass covers everything).
+template <int B, class T, class Ld, class St>
+__device__ __forceinline__ void staged_for(int total, Ld ld, St st) {
+  const int nt = blockDim.x;
+  for (int b0 = 0; b0 < total;) {
+    const int left = (total - b0 + nt - 1) / nt;   // passes left, the same for every thread
+    if (B >= 8 && left >= 8) {
+      staged_pass<8, T>(b0, total, ld, st);
+      b0 += 8 * nt;
+    } else if (B >= 4 && left >= 4) {
+      staged_pass<4, T>(b0, total, ld, st);
+      b0 += 4 * nt;
+    } else if (B >= 2 && left >= 2) {
+      staged_pass<2, T>(b0, total, ld, st);
+      b0 += 2 * nt;
+    } else {
+      staged_pass<1, T>(b0, total, ld, st);
+      b0 += nt;
+    }
+  }
+}
+
 __device__ __forceinline__ void stage_twiddles(double2 *__restrict__ dst,
                                                const double2 *__restrict__ src, int n) {
   for (int e = threadIdx.x; e < n; e += blockDim.x) dst[e] = src[e];
@@ -721,16 +763,40 @@ __device__ __forceinline__ void c2r_put(double2 *A, int p, int k, double2 xa, do
   if (k > 0 && n - k >= Nc) A[p * n + (n - k)] = make_double2(xa.x + xb.y, -xa.y + xb.x);
 }
 
+struct Bin2 { double2 a, b; };   // one bin of a row pair's two spectra
+
+// BATCHED: the loads of several passes in flight at once (staged_for; for
+// workgroups with several row pairs).  Otherwise one pass after another
+// (a workgroup of one row pair has a single pass, and the shorter code
+// starts faster).
+template <bool BATCHED = true>
 __device__ __forceinline__ void c2r_load(const double2 *__restrict__ in, double2 *A, int row0,
                                          int npair, int n, int Nc, int Nu) {
-  for (int e = threadIdx.x; e < npair * Nc; e += blockDim.x) {
-    const int k = qdiv(e, rcpf((float)npair)), p = e - k * npair;
-    const int ra = row0 + 2 * p, rb = ra + 1;
-    double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
-    if (ra < Nu) xa = in[(i64)k * Nu + ra];
-    if (rb < Nu) xb = in[(i64)k * Nu + rb];
-    c2r_put(A, p, k, xa, xb, n, Nc);
+  if constexpr (!BATCHED) {
+    for (int e = threadIdx.x; e < npair * Nc; e += blockDim.x) {
+      const int k = qdiv(e, rcpf((float)npair)), p = e - k * npair;
+      const int ra = row0 + 2 * p, rb = ra + 1;
+      double2 xa = make_double2(0.0, 0.0), xb = make_double2(0.0, 0.0);
+      if (ra < Nu) xa = in[(i64)k * Nu + ra];
+      if (rb < Nu) xb = in[(i64)k * Nu + rb];
+      c2r_put(A, p, k, xa, xb, n, Nc);
+    }
+    return;
   }
+  const float rp = rcpf((float)npair);
+  staged_for<4, Bin2>(
+      npair * Nc,
+      [&](int e) {
+        const int k = qdiv(e, rp), p = e - k * npair;
+        const int ra = row0 + 2 * p, rb = ra + 1;
+        const double2 xa = in[(i64)k * Nu + min(ra, Nu - 1)], xb = in[(i64)k * Nu + min(rb, Nu - 1)];
+        const double2 z = make_double2(0.0, 0.0);
+        return Bin2{ra < Nu ? xa : z, rb < Nu ? xb : z};
+      },
+      [&](int e, Bin2 x) {
+        const int k = qdiv(e, rp);
+        c2r_put(A, e - k * npair, k, x.a, x.b, n, Nc);
+      });
 }
 
 // Doubles of the tile part of k_rows_fused's LDS (k_update's layout for a
@@ -770,15 +836,28 @@ __device__ __forceinline__ void rows_r2c_body(const DiskDev *__restrict__ disks,
   const double *S = stp(st, ntot, par, 0, D);
   const double *R32 = gtab(geom, D, GT_R32);
   // all threads stage the rows (more loads in flight than a first stage
-  // reading HBM through its few butterflies: measured faster here)
-  for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
-    const int p = qdiv(e, rcpf((float)n)), j = e - p * n;
-    const int ra = row0 + 2 * p, rb = ra + 1;
-    double xa = 0.0, xb = 0.0;
-    if (ra < Nu) xa = R32[ra] * S[(i64)ra * n + j];
-    if (rb < Nu) xb = R32[rb] * S[(i64)rb * n + j];
-    A[e] = make_double2(xa, xb);
-  }
+  // reading HBM through its few butterflies: measured faster here); with
+  // several row pairs the loads of up to 8 passes at once
+  const float rn = rcpf((float)n);
+  if constexpr (!BATCH) {
+    for (int e = threadIdx.x; e < pr * n; e += blockDim.x) {
+      const int p = qdiv(e, rn), j = e - p * n;
+      const int ra = row0 + 2 * p, rb = ra + 1;
+      double xa = 0.0, xb = 0.0;
+      if (ra < Nu) xa = R32[ra] * S[(i64)ra * n + j];
+      if (rb < Nu) xb = R32[rb] * S[(i64)rb * n + j];
+      A[e] = make_double2(xa, xb);
+    }
+  } else staged_for<8, double2>(
+      pr * n,
+      [&](int e) {
+        const int p = qdiv(e, rn), j = e - p * n;
+        const int ra = row0 + 2 * p, rb = ra + 1;
+        const int ca = min(ra, Nu - 1), cb_ = min(rb, Nu - 1);
+        const double xa = R32[ca] * S[(i64)ca * n + j], xb = R32[cb_] * S[(i64)cb_ * n + j];
+        return make_double2(ra < Nu ? xa : 0.0, rb < Nu ? xb : 0.0);
+      },
+      [&](int e, double2 v) { A[e] = v; });
   __syncthreads();
   const double2 *Z = A;
   if constexpr (TG == 0) {
@@ -965,7 +1044,7 @@ __device__ __forceinline__ void rows_c2r_body(const DiskDev *__restrict__ disks,
   constexpr int NBUF = TG ? 1 : 2;
   double2 *A = lds2, *B = lds2 + pr * n;
   const double2 *TW = tw_table(lds2 + NBUF * pr * n, tw + D.two_row, n, D.tw_lds & 1);
-  c2r_load(spec + D.spo, A, row0, pr, n, Nc, Nu);
+  c2r_load<BATCH>(spec + D.spo, A, row0, pr, n, Nc, Nu);
   __syncthreads();
   const double *sc = gtab(geom, D, GT_PHISC);
   double *P = phi + D.so;
@@ -1343,40 +1422,39 @@ k_rows_fused(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, in
     const int nsp = np * Nc, nS = (TI + 6) * n, nst = nS + 2 * (TI + 4) * n;
     const float rnp = rcpf((float)np), rn = rcpf((float)n);
     constexpr int B1 = 2, B2 = 8;
+    const int nUW = (TI + 4) * n;
     for (int base = 0; base * B1 < nsp || base * B2 < nst; base += TG) {
+      // loads from clamped addresses, no branches: all of them issue before
+      // the first wait
       double2 xa[B1], xb[B1];
       double v[B2];
 #pragma unroll
       for (int u = 0; u < B1; ++u) {
-        const int e = base * B1 + u * TG + threadIdx.x;
-        xa[u] = xb[u] = make_double2(0.0, 0.0);
-        if (e < nsp) {
-          const int k = qdiv(e, rnp), p = e - k * np;
-          const int ra = 2 * (pa + p);
-          if (ra < Nu) xa[u] = in[(i64)k * Nu + ra];
-          if (ra + 1 < Nu) xb[u] = in[(i64)k * Nu + ra + 1];
-        }
+        const int e = min(base * B1 + u * TG + (int)threadIdx.x, nsp - 1);
+        const int k = qdiv(e, rnp), ra = 2 * (pa + e - k * np);
+        xa[u] = in[(i64)k * Nu + min(ra, Nu - 1)];
+        xb[u] = in[(i64)k * Nu + min(ra + 1, Nu - 1)];
       }
 #pragma unroll
       for (int u = 0; u < B2; ++u) {
-        const int e = base * B2 + u * TG + threadIdx.x;
-        v[u] = 0.0;
-        if (e < nst) {
-          const int f = e < nS ? 0 : (e < nS + (TI + 4) * n ? 1 : 2);
-          const int r = qdiv(f == 0 ? e : e - nS - (f - 1) * (TI + 4) * n, rn);
-          const int i = i0 - (f == 0 ? 3 : 2) + r;
-          const int j = (f == 0 ? e : e - nS - (f - 1) * (TI + 4) * n) - r * n;
-          if (i >= 0 && i < Nu) v[u] = (f == 0 ? S : (f == 1 ? U : W))[(i64)i * n + j];
-        }
+        const int e = min(base * B2 + u * TG + (int)threadIdx.x, nst - 1);
+        const int f = e < nS ? 0 : (e < nS + nUW ? 1 : 2);
+        const int el = f == 0 ? e : e - nS - (f - 1) * nUW;
+        const int r = qdiv(el, rn);
+        const int i = min(max(i0 - (f == 0 ? 3 : 2) + r, 0), Nu - 1);
+        v[u] = (f == 0 ? S : (f == 1 ? U : W))[(i64)i * n + el - r * n];
       }
 #pragma unroll
       for (int u = 0; u < B1; ++u) {
         const int e = base * B1 + u * TG + threadIdx.x;
         if (e < nsp) {
-          const int k = qdiv(e, rnp);
-          c2r_put(F, e - k * np, k, xa[u], xb[u], n, Nc);
+          const int k = qdiv(e, rnp), p = e - k * np, ra = 2 * (pa + p);
+          const double2 z = make_double2(0.0, 0.0);
+          c2r_put(F, p, k, ra < Nu ? xa[u] : z, ra + 1 < Nu ? xb[u] : z, n, Nc);
         }
       }
+      // staged rows outside the disk are never read (the tile body clamps
+      // its rows into the disk and skips the others)
 #pragma unroll
       for (int u = 0; u < B2; ++u) {
         const int e = base * B2 + u * TG + threadIdx.x;
