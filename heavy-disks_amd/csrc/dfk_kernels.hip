@@ -1020,11 +1020,21 @@ k_cols_conv_group(const DiskDev *__restrict__ disks, const int2 *__restrict__ ma
   if (ctl[wm.x].done) return;
   const int M = D.M, Nc = D.Nc, cb = D.cb;
   const int k0 = wm.y * cb;
-  const double2 *TW = tw_table(lds2 + cb * M, tw + D.two_col, M, D.tw_lds & 2);
   StageIO g{};
   g.spec = spec + D.spo; g.kh = khat + D.kho; g.Nu = D.Nu; g.Nc = Nc; g.k0 = k0;
-  __syncthreads();   // the twiddle table, if staged in LDS
-  ip_conv<TG, ip_nmax<TG>(), BATCH>(D, lds2, BATCH ? min(cb, Nc - k0) : 1, threadIdx.x, TW, g);
+  const int nb = BATCH ? min(cb, Nc - k0) : 1;
+  // one call per twiddle location, so that each inlined transform reads its
+  // twiddles through a pointer of known address space (LDS or global loads;
+  // through one pointer that may be either, every twiddle read is a flat
+  // load, which also waits for the workgroup's outstanding LDS accesses)
+  if (D.tw_lds & 2) {
+    double2 *T = lds2 + cb * M;
+    stage_twiddles(T, tw + D.two_col, M);
+    __syncthreads();
+    ip_conv<TG, ip_nmax<TG>(), BATCH>(D, lds2, nb, threadIdx.x, T, g);
+  } else {
+    ip_conv<TG, ip_nmax<TG>(), BATCH>(D, lds2, nb, threadIdx.x, tw + D.two_col, g);
+  }
 }
 
 // Row c2r (FFTW convention: imaginary parts of the DC and Nyquist bins are
