@@ -186,8 +186,9 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     for t in (64, 128, 256):
         e.set_threads(t, t, t)
         res["threads"][str(t)] = [float(x) for x in prof(ti, tj, bpr, bcb)]
-    pick = lambda idx: int(min(res["threads"], key=lambda k: res["threads"][k][idx]))  # noqa: E731
-    thr = (pick(1), pick(2), pick(4))
+    # the row r2c and c2r share one thread count: pick it on their sum
+    pick = lambda *idx: int(min(res["threads"], key=lambda k: sum(res["threads"][k][i] for i in idx)))  # noqa: E731
+    thr = (pick(1, 3), pick(2), pick(4))
     e.set_threads(*thr)
     best = prof(ti, tj, bpr, bcb)
     # latency regime: the four-launch iteration against fused row kernels
