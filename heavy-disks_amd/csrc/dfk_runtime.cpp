@@ -599,18 +599,28 @@ int drain(dfk_handle *h) {
   return DFK_OK;
 }
 
-// Per-tile partials of disk d's current state, reduced on the host in tile order.
+// Per-tile partials of disk d's current state, reduced on the host in tile
+// order.  Read-only for a run in progress or finished: the diag kernel finds
+// the state through the control block's `cur`, the only field written (the
+// disk's clock, step count and output schedule stay), and the disk's tile
+// partials, which the loop's next control reads, are restored afterwards.
 int diag_disk(dfk_handle *h, int d, double *mx, double sums[4]) {
   TRY(build_layout(h));
-  TRY(unit_ctl(h, d, 0.0));
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&h->ctl[d].cur, &h->hcur[d], sizeof(int), hipMemcpyHostToDevice));
   const int nb = h->upd0[d + 1] - h->upd0[d];
+  const DiskDev &D = h->hD[d];
+  std::vector<double> pc(nb), pm(4 * (size_t)nb), pc0(nb), pm0(4 * (size_t)nb);
+  HIPCHK(hipMemcpy(pc0.data(), h->pcfl + D.part, sizeof(double) * nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(pm0.data(), h->pcm + 4 * D.part, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost));
   HIPCHK(dfk_launch_diag(h->stream, nb, h->dD, h->map_upd, h->upd0[d], h->ctl, h->st, h->ntot,
                          h->geom, h->trig, h->pcfl, h->pcm));
   HIPCHK(hipStreamSynchronize(h->stream));
-  const DiskDev &D = h->hD[d];
-  std::vector<double> pc(nb), pm(4 * (size_t)nb);
   HIPCHK(hipMemcpy(pc.data(), h->pcfl + D.part, sizeof(double) * nb, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(pm.data(), h->pcm + 4 * D.part, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(h->pcfl + D.part, pc0.data(), sizeof(double) * nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->pcm + 4 * D.part, pm0.data(), sizeof(double) * 4 * nb, hipMemcpyHostToDevice));
   double m = 1e-30;
   double s[4] = {0, 0, 0, 0};
   for (int t = 0; t < nb; ++t) {

@@ -88,7 +88,9 @@ int dfk_geometry(dfk_handle *h, int disk, double *R, double *Rf, double *Vol);
 int dfk_set_state(dfk_handle *h, int disk, const double *Sig, const double *vR, const double *vp);
 int dfk_get_state(dfk_handle *h, int disk, double *Sig, double *vR, double *vp);
 
-/* DiskHydro::timestep (hydro.hpp:48-61). */
+/* DiskHydro::timestep (hydro.hpp:48-61).  Like dfk_modes and dfk_disk_mass
+ * it reads the current state only: called during or after a run, it leaves
+ * the disk's loop clock, step count and later steps as they are. */
 int dfk_timestep(dfk_handle *h, int disk, double cfl, double *dt);
 
 /* KalnajsSolver::potential (kalnajs.hpp:48-63): Phi from Sigma (Nu*Nphi). */

@@ -1,0 +1,110 @@
+"""GPU: the 64-disk ensemble of the benchmark at full size (7.7 M zones,
+grids 128^2 .. 512^2 drawn with seed 2026, tuned throughput shapes), checked
+through properties that do not depend on its size:
+
+  - every disk's mass sum(Sigma dA) is conserved to round-off (radial boundary
+    fluxes are zero and phi is periodic, hydro.hpp:103-106, 143-179; no zone
+    reaches the floor);
+  - every zone stays finite and no disk raises a flag;
+  - one disk of each grid gives the same bits run alone with the same
+    options (the kernels are the same per disk);
+  - a 128 x 128 disk of the ensemble matches the oracle's 10 steps from the
+    same initial state within the north-star 1e-10.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+STEPS = 20
+
+
+@pytest.fixture(scope="module")
+def dfk():
+    import diskfft_amd
+    if diskfft_amd.device_count() < 1:
+        pytest.fail("no HIP device visible: the GPU tests need an MI355X")
+    return diskfft_amd
+
+
+@pytest.fixture(scope="module")
+def ensemble(dfk):
+    sys.path.insert(0, os.path.join(ROOT, "heavy-disks_amd"))
+    import autotune
+    import ensemble_cfg
+    disks = ensemble_cfg.draw(64, seed=2026)
+    print(f"case full ensemble: 64 disks, {sum(d['Nu'] * d['Nphi'] for d in disks)} zones",
+          flush=True)
+    ens, ics = ensemble_cfg.build(dfk, disks)
+    assert autotune.apply(ens, regime="throughput") == 64
+    m0 = [ens.disk_mass(d) for d in range(64)]
+    ens.set_graph_steps(10)
+    ens.run_init(1e30, cfl=0.4, mD=[ic[3] for ic in ics], max_steps=STEPS, sample_cap=64)
+    ens.advance(STEPS + 2)
+    yield dfk, disks, ics, ens, m0
+    ens.close()
+
+
+def test_full_ensemble_conserves_mass_and_stays_finite(ensemble):
+    dfk, disks, ics, ens, m0 = ensemble
+    worst = 0.0
+    for d in range(64):
+        st = ens.status(d)
+        assert st["done"] and st["steps"] == STEPS and st["t"] > 0 and not st["flags"], (d, st)
+        assert all(np.all(np.isfinite(x)) for x in ens.get_state(d)), d
+        rel = abs(ens.disk_mass(d) - m0[d]) / m0[d]
+        worst = max(worst, rel)
+    print(f"  worst relative mass change over {STEPS} steps: {worst:.2e}", flush=True)
+    assert worst < 1e-12
+
+
+def test_full_ensemble_disks_match_single_runs(ensemble):
+    dfk, disks, ics, ens, m0 = ensemble
+    picked = {}
+    for d, x in enumerate(disks):
+        picked.setdefault((x["Nu"], x["Nphi"]), d)
+    for (Nu, Nphi), d in sorted(picked.items()):
+        print(f"  disk {d} ({Nu}x{Nphi}) alone", flush=True)
+        one = dfk.Ensemble(ics[d][4])
+        one.set_update_kernel(ens.get_update_kernel())
+        one.set_row_transform(ens.get_row_transform())
+        one.set_threads(*ens.get_threads())
+        one.set_shapes(*ens.get_shapes(d))
+        one.set_fft_options(*ens.get_fft_options(d))
+        assert one.get_shapes(0) == ens.get_shapes(d)
+        one.set_state(*ics[d][:3])
+        one.run_init(1e30, cfl=0.4, mD=ics[d][3], max_steps=STEPS, sample_cap=64)
+        one.advance(STEPS + 2)
+        assert one.status()["t"] == ens.status(d)["t"]
+        for a, b in zip(one.get_state(), ens.get_state(d)):
+            assert np.array_equal(a, b)
+        one.close()
+
+
+def test_full_ensemble_disk_matches_oracle(orc, ensemble):
+    dfk, disks, ics, ens, m0 = ensemble
+    d = next(k for k, x in enumerate(disks) if (x["Nu"], x["Nphi"]) == (128, 128))
+    p = ics[d][4]
+    h = orc.Hydro(p["Nu"], p["Nphi"], p["Rin"], p["Rout"], p["K"], p["gamma"], p["gmstar"])
+    k = orc.Kalnajs(p["Nu"], p["Nphi"], p["Rin"], p["Rout"], p["eps"], p["G"])
+    h.set_state(*ics[d][:3])
+    orc.run(h, k, 1, 1e30, cfl=0.4, max_steps=10, mD=ics[d][3])
+    one = dfk.Ensemble(p)
+    one.set_update_kernel(ens.get_update_kernel())
+    one.set_row_transform(ens.get_row_transform())
+    one.set_threads(*ens.get_threads())
+    one.set_shapes(*ens.get_shapes(d))
+    one.set_fft_options(*ens.get_fft_options(d))
+    one.set_state(*ics[d][:3])
+    one.run_init(1e30, cfl=0.4, mD=ics[d][3], max_steps=10)
+    one.advance(12)
+    errs = [relerr(a, b) for a, b in zip(one.get_state(), h.state())]
+    print(f"  disk {d} (128x128) 10 steps vs oracle:", errs, flush=True)
+    assert one.status()["steps"] == 10 and max(errs) <= 1e-10
+    one.close()

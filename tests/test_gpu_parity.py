@@ -461,6 +461,41 @@ def test_graph_chains_same_bits(orc, dfk, chains):
         assert len(a) > 2 and np.array_equal(a, b)
 
 
+def test_diagnostics_leave_the_run_alone(orc, dfk):
+    """dfk_disk_mass, dfk_modes and dfk_timestep in the middle of a run (and
+    after it) change neither the disk's clock nor its later steps and
+    samples: the run then ends on the bits of an uninterrupted one."""
+    print("case diagnostics mid-run", flush=True)
+    Nu, Nphi, eps = 64, 64, 0.02
+    h, _ = _lka98(orc, Nu, Nphi, eps)
+    ic, mD = h.state(), h.disk_mass()
+    out = []
+    for probe in (False, True):
+        e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
+        e.set_state(*ic)
+        e.run_init(1e9, dtout=0.01, mD=mD, max_steps=30)
+        e.advance(15)
+        if probe:
+            st = e.status()
+            assert st["steps"] == 15 and st["t"] > 0
+            m = e.disk_mass()
+            c2, _ = e.modes(2, mD)
+            dt = e.timestep(0.3)
+            assert m > 0 and c2 > 0 and dt > 0
+            assert e.status() == st
+        e.advance(20)
+        st = e.status()
+        if probe:
+            e.disk_mass()
+            assert e.status() == st
+        out.append((e.get_state(), st, e.samples()))
+        e.close()
+    assert out[0][1]["steps"] == 30 and out[0][1] == out[1][1]
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert len(out[0][2]) > 2 and np.array_equal(out[0][2], out[1][2])
+
+
 def test_nonfinite_flag_stops_disk(dfk):
     Nu, Nphi = 32, 32
     print("case nonfinite flag", flush=True)
