@@ -179,7 +179,7 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     ti, tj = (int(x) for x in bt.split("x"))
     kind = 0
     if bst is not None and res["stream"][bst] < res["tile"][bt]:
-        kind, ti, tj = 1, int(bst), 60
+        kind, ti, tj = 1, int(bst), 58   # the streaming kernel's output columns per wave
     e.set_update_kernel(kind)
     # threads per workgroup of each kernel, at the chosen shapes
     res["threads"] = {}

@@ -13,7 +13,7 @@
 //   k_update    forces (kalnajs.hpp:65-80) + source (hydro.hpp:73-96) + radial
 //               MUSCL sweep (:99-140) + azimuthal sweep (:143-179) + floor
 //               (:68-69) fused on an LDS tile (k_update_stream: the same
-//               fused on a wavefront streaming down a 60-column strip);
+//               fused on a wavefront streaming down a 58-column strip);
 //               epilogue writes the new state's CFL maximum and C_0/C_2
 //               partial sums per tile.
 // k_control is the loop control as a launch of its own (handles without
@@ -1512,13 +1512,17 @@ k_rows_fused(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, in
 // ------------------------------------------------------------------ streaming update
 // Register-streaming variant of k_update for bandwidth-bound (large or
 // batched) runs.  One wavefront per work unit = (disk, column group g, radial
-// strip k).  Lane L owns azimuthal column c = 60 g - 2 + L (periodic); lanes
-// 2..61 produce output.  The wave walks source rows i0-2 .. i1+1 of its
+// strip k).  Lane L owns azimuthal column c = 58 g - 3 + L (periodic); lanes
+// 3..60 produce output.  The wave walks source rows i0-2 .. i1+1 of its
 // strip [i0, i1) (down on even strips, up on odd ones): the radial stencil
 // (Sigma/Phi windows, the last four post-source rows, the previous radial
-// face) stays in registers, azimuthal neighbours come from cross-lane
-// shuffles, and row tables are wave-uniform.  Same arithmetic as k_update.
-#define STREAM_OUT 60
+// face) stays in registers, azimuthal neighbours -- of the source step's
+// pressure and potential as well as of the azimuthal sweep -- come from
+// cross-lane shifts (a 3-lane halo: the source values of lanes 1..62 are
+// exact, those of the edge lanes reach no output), and row tables are
+// wave-uniform.  Same arithmetic as k_update.
+#define STREAM_OUT 58
+#define STREAM_HALO 3
 
 // Whole-wavefront lane shifts by one through DPP (wave_shr:1 / wave_shl:1 on
 // each 32-bit half): lane L receives lane L-1 (up) or L+1 (down); the lane
@@ -1553,10 +1557,10 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   const int i0 = ks * TS, i1 = min(i0 + TS, Nu);
   const int j0 = cg * STREAM_OUT;
   const int lane = threadIdx.x;
-  int c = (j0 - 2 + lane) % Nphi;
+  int c = (j0 - STREAM_HALO + lane) % Nphi;
   if (c < 0) c += Nphi;
-  const int cm = (c == 0) ? Nphi - 1 : c - 1, cp = (c == Nphi - 1) ? 0 : c + 1;
-  const bool out_ok = lane >= 2 && lane < 2 + STREAM_OUT && (j0 + lane - 2) < Nphi;
+  const bool out_ok = lane >= STREAM_HALO && lane < STREAM_HALO + STREAM_OUT &&
+                      (j0 + lane - STREAM_HALO) < Nphi;
 
   const double *S = stp(st, ntot, par, 0, D), *U = stp(st, ntot, par, 1, D),
                *W = stp(st, ntot, par, 2, D);
@@ -1599,19 +1603,13 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
 
     // loads of the next row are issued one iteration ahead (rows clamped
     // into the disk; values of rows outside it are never used)
-    double nU, nW, nSe, nSw, nPe = 0.0, nPw = 0.0, nSa, nPa = 0.0;
+    double nU, nW, nSa, nPa = 0.0;
     auto fetch = [&](int r) {
       const i64 ro = (i64)clampr(r) * Nphi, ra = (i64)clampr(r + STEP) * Nphi;
       nU = U[ro + c];
       nW = W[ro + c];
-      nSe = S[ro + cp];
-      nSw = S[ro + cm];
       nSa = S[ra + c];
-      if (sg) {
-        nPe = Ph[ro + cp];
-        nPw = Ph[ro + cm];
-        nPa = Ph[ra + c];
-      }
+      if (sg) nPa = Ph[ra + c];
     };
     // rows in groups of four, so that the unrolled body renames the ring
     // instead of moving it; the up to three rows past the last one compute
@@ -1623,8 +1621,12 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
 #pragma unroll
     for (int k = 0; k < 4; ++k, s += STEP) {
       // row s + STEP (ahead) and row s - STEP (behind)
-      const double Sa = nSa, Pa = nPa, u = nU, w = nW, Se = nSe, Sw = nSw, Pe = nPe, Pw = nPw;
+      const double Sa = nSa, Pa = nPa, u = nU, w = nW;
       fetch(s + STEP);
+      // azimuthal neighbours of row s from the neighbouring lanes (the edge
+      // lanes 0 and 63 get zeros; their results reach no output lane)
+      const double Se = dn1(S0), Sw = up1(S0);
+      const double Pe = sg ? dn1(P0) : 0.0, Pw = sg ? up1(P0) : 0.0;
       const double Sp = DOWN ? Sa : Sb, Sm = DOWN ? Sb : Sa;
       const double Pp = DOWN ? Pa : Pb, Pm = DOWN ? Pb : Pa;
       // ---- source at row s (hydro.hpp:73-96, forces kalnajs.hpp:65-80)

@@ -89,7 +89,7 @@ int ip_nmax(int t) { return std::max(8 * t, 1024); }
 // work list so that XCD x receives one contiguous chunk of it: neighbouring
 // tiles (which share halo rows) then meet in the same L2.
 const int kXcds = 8;
-const int kStreamCols = 60;   // output columns per wave of the streaming update (kernel's STREAM_OUT)
+const int kStreamCols = 58;   // output columns per wave of the streaming update (kernel's STREAM_OUT)
 std::vector<int2> xcd_order(const std::vector<int2> &v) {
   const int L = (int)v.size();
   std::vector<int2> out(L);
@@ -211,7 +211,7 @@ int build_layout(dfk_handle *h) {
   size_t lf = 0;
   for (int d = 0; d < h->nd; ++d) {
     DiskDev &D = h->hD[d];
-    if (h->upd_kind == 1) D.tj = kStreamCols;   // one wave = 60 output columns
+    if (h->upd_kind == 1) D.tj = kStreamCols;   // one wave = 58 output columns
     if (h->fused) {
       D.tj = D.Nphi;
       D.ti = std::max(2, std::min(D.ti & ~1, D.Nu + (D.Nu & 1)));

@@ -183,7 +183,7 @@ int dfk_get_row_transform(dfk_handle *h, int *inplace);
 
 /* Update-kernel variant for all disks of the handle: 0 = LDS tile (ti x tj
  * zones per workgroup; suits small, latency-bound runs), 1 = register
- * streaming (one wavefront per 60 columns x ti rows; suits large or batched,
+ * streaming (one wavefront per 58 columns x ti rows; suits large or batched,
  * bandwidth-bound runs).  Both compute the same step. */
 int dfk_set_update_kernel(dfk_handle *h, int kind);
 int dfk_get_update_kernel(dfk_handle *h, int *kind);
