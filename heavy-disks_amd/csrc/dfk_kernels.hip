@@ -16,6 +16,9 @@
 //               fused on a wavefront streaming down a 58-column strip);
 //               epilogue writes the new state's CFL maximum and C_0/C_2
 //               partial sums per tile.
+// k_rows_fused (an option for small disks) runs the row c2r, the update and
+// the next row r2c for a block of full rows in one launch; the loop control
+// then rides in the column-convolution launch.
 // k_control is the loop control as a launch of its own (handles without
 // self-gravity, profiling).
 // All FFTs are this file's own LDS Stockham transforms (radix 8, 4, 2, 3
