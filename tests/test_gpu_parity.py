@@ -496,6 +496,37 @@ def test_diagnostics_leave_the_run_alone(orc, dfk):
     assert len(out[0][2]) > 2 and np.array_equal(out[0][2], out[1][2])
 
 
+def test_chains_and_graph_steps_change_mid_run(orc, dfk):
+    """dfk_set_chains and dfk_set_graph_steps between two dfk_advance calls
+    recapture the loop's graphs and the run goes on, on the bits of a run
+    that never changed them."""
+    print("case chains / graph steps mid-run", flush=True)
+    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 30, 0.04), (40, 130, 0.01), (24, 24, 0.03)]
+    params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
+    ics = [_lka98(orc, a, b, c)[0] for a, b, c in shapes]
+    out = []
+    for change in (False, True):
+        e = dfk.Ensemble(params)
+        for d, h in enumerate(ics):
+            e.set_state(*h.state(), d=d)
+        e.set_chains(1)
+        e.set_graph_steps(7)
+        e.run_init(1e9, dtout=0.002, mD=[h.disk_mass() for h in ics], max_steps=30, sample_cap=64)
+        e.advance(13)
+        if change:
+            e.set_chains(3)
+            e.set_graph_steps(4)
+            assert e.get_chains() > 1
+        e.advance(20)
+        out.append([(e.get_state(d), e.status(d), e.samples(d)) for d in range(len(shapes))])
+        e.close()
+    for (sa, ta, pa), (sb, tb, pb) in zip(*out):
+        assert ta == tb and ta["steps"] == 30
+        for a, b in zip(sa, sb):
+            assert np.array_equal(a, b)
+        assert len(pa) > 2 and np.array_equal(pa, pb)
+
+
 def test_nonfinite_flag_stops_disk(dfk):
     Nu, Nphi = 32, 32
     print("case nonfinite flag", flush=True)
