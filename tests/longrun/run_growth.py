@@ -11,6 +11,8 @@ Cases (name: model, grid, eps, tlim or steps, CFL):
                   samples every 0.01, gamma / Omega_p of the first (inner-edge) growth
                   from 10x to 100x the initial |C2|/|C0|, sat over the second half
 The oracle runs with OpenMP on this host's cores (<= 16).
+--fused-rows THREADS runs the GPU side with the fused row kernels (row tile of
+2 rows, THREADS per workgroup) in place of the four-launch step.
 """
 import argparse
 import json
@@ -39,7 +41,7 @@ def case(name):
             }[name]
 
 
-def run_case(name, cache_dir=None, skip_gpu=False):
+def run_case(name, cache_dir=None, skip_gpu=False, fused=0):
     c = case(name)
     Nu, Nphi = c["N"]
     kind = dfk.MODEL_LKA98 if c["model"] == "lka98" else dfk.MODEL_POWERLAW
@@ -53,6 +55,9 @@ def run_case(name, cache_dir=None, skip_gpu=False):
     else:
         e = dfk.Ensemble(p)
         e.set_state(S, U, W)
+        if fused:
+            e.set_shapes(ti=2)
+            e.set_fused_rows(fused)
     t0 = time.perf_counter()
     dtout = c.get("dtout", 0.25)
     if not skip_gpu:
@@ -60,6 +65,9 @@ def run_case(name, cache_dir=None, skip_gpu=False):
         ((sg, stg),) = e.run(c["tlim"], dtout=dtout, cfl=c["cfl"], mD=mD, max_steps=c["steps"],
                              chunk=20000)
         print(f"  {name} gpu: {stg['steps']} steps to t = {stg['t']:.2f}", flush=True)
+        if e.get_fused_rows() != fused:
+            raise RuntimeError(f"{name}: fused row kernels requested {fused}, "
+                               f"ran with {e.get_fused_rows()}")
     tg = time.perf_counter() - t0
     cache = os.path.join(cache_dir, f"oracle_{name}.npz") if cache_dir else None
     if cache and os.path.exists(cache):
@@ -88,7 +96,7 @@ def run_case(name, cache_dir=None, skip_gpu=False):
     rel = {q: abs(gg[q] - go[q]) / abs(go[q]) if go[q] else None
            for q in ("gamma", "omega_p", "sat")}
     Sg, _, _ = e.get_state()
-    out = dict(case=name, config=c, K=p["K"], gpu=gg, oracle=go, rel_diff=rel,
+    out = dict(case=name, config=c, K=p["K"], fused_rows=fused, gpu=gg, oracle=go, rel_diff=rel,
                steps_gpu=stg["steps"], steps_oracle=steps, t_gpu=stg["t"], t_oracle=tf,
                c2_final_gpu=stg["c2_final"], c2_final_oracle=c2f,
                mass_drift_gpu=float(sg[-1, 1] - 1.0), mass_drift_oracle=float(so[-1, 1] - 1.0),
@@ -107,13 +115,15 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--oracle-cache", default=None,
                     help="directory holding (or receiving) the oracle's runs of the cases")
+    ap.add_argument("--fused-rows", type=int, default=0,
+                    help="GPU side with the fused row kernels of this many threads")
     ap.add_argument("--oracle-only", action="store_true",
                     help="run (and cache) the oracle side only; no GPU needed")
     args = ap.parse_args()
     oracle.set_threads(max(1, min(16, os.cpu_count() or 1)))
     res = []
     for name in args.cases:
-        res.append(run_case(name, args.oracle_cache, args.oracle_only))
+        res.append(run_case(name, args.oracle_cache, args.oracle_only, args.fused_rows))
         with open(args.out, "w") as f:
             json.dump(res, f)
 
