@@ -9,7 +9,7 @@ through properties that do not depend on its size:
   - one disk of each grid gives the same bits run alone with the same
     options (the kernels are the same per disk);
   - a 128 x 128 disk of the ensemble matches the oracle's 10 steps from the
-    same initial state within the north-star 1e-10.
+    same initial state within 1e-10 relative.
 """
 import os
 import sys

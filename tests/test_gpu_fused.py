@@ -4,7 +4,7 @@ C ABI, against the oracle and against the four-launch loop.
 Each iteration is then the column convolution (carrying the loop control)
 and one kernel per block of ti full rows running the row c2r of the rows its
 forces read, the update and the row r2c of the new Sigma.  Bar: the
-north-star 1e-10 after 10 steps against the oracle (diskmain.cpp:92-110),
+1e-10 relative after 10 steps against the oracle (diskmain.cpp:92-110),
 and the same against the four-launch loop on the same device: the formulas
 are the same, but the two builds contract different products to FMAs, and
 v_R (the smallest field, so the largest relative error in the max-norm)

@@ -1,6 +1,6 @@
 """GPU parity: the HIP library (through its C ABI) against the CPU oracle.
 
-Bar (north star): Sigma and velocities after 10 steps within 1e-10 relative
+Bar: Sigma and velocities after 10 steps within 1e-10 relative
 (max-norm over the field, relative to the field's max).  The kernels use the
 reference's formulas with reciprocal tables instead of some divisions and FMA
 contraction, so agreement is round-off level; the tests assert 1e-10 for the
@@ -13,7 +13,7 @@ from helpers import grid, relerr, rough_state
 
 pytestmark = pytest.mark.gpu
 
-LOOP_TOL = 1e-10      # north-star tolerance, Sigma/vR/vphi after 10 steps
+LOOP_TOL = 1e-10      # loop tolerance, Sigma/vR/vphi after 10 steps
 OP_TOL = 1e-12        # single operator (one transform, one stage, one step)
 
 
@@ -206,7 +206,7 @@ def _lka98(orc, Nu, Nphi, eps, amp=1e-3, selfgrav=1):
                                          (256, 256, 0.005), (48, 36, 0.02), (512, 1024, 0.01),
                                          (3, 4, 0.02)])
 def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps, kind):
-    """The north-star gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative."""
+    """The loop gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative."""
     print(f"case 10-step loop {Nu}x{Nphi} eps={eps} update kernel {kind}", flush=True)
     h, k = _lka98(orc, Nu, Nphi, eps)
     e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))

@@ -1,6 +1,6 @@
 """GPU loop over many steps: snapshot/resume and the m = 2 growth rate.
 
-The north star asks gamma and Omega_p within 1% of the FP64 CPU path from the
+The bar: gamma and Omega_p within 1% of the FP64 CPU path from the
 same state.  The reference's own 128 x 128 setup (diskfft/README.md:27-30:
 LKA98 ring, tlim 9, amp 1e-3) runs in a couple of seconds on both sides.
 """
