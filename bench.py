@@ -55,6 +55,23 @@ def kernel_bytes(Nu, Nphi):
     }
 
 
+def warm_graphs(e, steps, warmup):
+    """Untimed: launches once every graph that `e.advance(steps)` will launch
+    from the current loop position (a graph is captured per length and
+    starting state buffer on first use), then `warmup` more steps; returns
+    with the loop on the same state buffer as at the call (even step
+    counts), so the timed advance(steps) finds all its graphs instantiated
+    and uploaded.  Returns the number of steps taken."""
+    done = 0
+    for _ in range(2 if steps % 2 else 1):   # odd: the graphs from both buffers
+        e.advance(steps)
+        done += steps
+    w = warmup + warmup % 2
+    if w:
+        e.advance(w)
+    return done + w
+
+
 def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk, model=None):
     m = dfk.model_defaults(dfk.MODEL_POWERLAW if model is None else model)
     p, S, U, W, mD, _ = dfk.model_ic(m, Nu, Nphi, eps)
@@ -65,13 +82,13 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk, model=None):
     e.set_state(S, U, W)
     e.set_graph_steps(chunk)
     e.run_init(1e30, dtout=0.25, cfl=cfl, mD=mD, sample_cap=chunk + 8)
-    e.advance(warmup)
+    pre = warm_graphs(e, steps, warmup)
     t0 = time.perf_counter()
     e.advance(steps)
     wall = time.perf_counter() - t0
     dev_ms = e.last_advance_ms()
     st = e.status()
-    assert st["steps"] == warmup + steps and not st["flags"], st
+    assert st["steps"] == pre + steps and not st["flags"], st
     prof = e.profile_step(200)
     shapes = dict(zip(("ti", "tj", "pr", "cb"), e.get_shapes(0)))
     shapes["threads"] = e.get_threads()
@@ -81,6 +98,20 @@ def single_disk(dfk, Nu, Nphi, eps, cfl, steps, warmup, chunk, model=None):
     shapes["tuned"] = bool(tuned)
     e.close()
     return dict(p=p, S=S, U=U, W=W, mD=mD, wall=wall, dev_ms=dev_ms, prof=prof, shapes=shapes)
+
+
+def host_cpus():
+    """(nproc, CPUs this process may run on, threads for the CPU baseline):
+    the baseline uses every CPU available to the process, capped by
+    OMP_NUM_THREADS where the host sets it (the CPU share of one GPU on a
+    shared GPU host)."""
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return nproc, avail, min(avail, share) if share > 0 else avail
 
 
 def cpu_baseline(p, S, U, W, cfl, seconds, threads):
@@ -113,7 +144,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq, chains=0):
     zones = sum(d["Nu"] * d["Nphi"] for d in disks)
     ens.set_graph_steps(50)
     ens.run_init(1e30, cfl=0.4, mD=[ic[3] for ic in ics], sample_cap=64)
-    ens.advance(50)
+    warm_graphs(ens, steps_batch, 0)
     t0 = time.perf_counter()
     ens.advance(steps_batch)
     wall_b = time.perf_counter() - t0
@@ -141,7 +172,7 @@ def ensemble_legs(dfk, steps_batch, steps_seq, chains=0):
         e.set_state(ic[0], ic[1], ic[2])
         e.set_graph_steps(50)
         e.run_init(1e30, cfl=0.4, mD=ic[3], sample_cap=64)
-        e.advance(50)
+        warm_graphs(e, steps_seq, 0)
         t0 = time.perf_counter()
         e.advance(steps_seq)
         seq_time += time.perf_counter() - t0
@@ -247,9 +278,11 @@ def main():
                                          "diskdisk default model)", "value": lka,
                              "vs_published_cpu": lka / REF_PUBLISHED}
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        nproc, avail, threads = host_cpus()
         out["cpu_baseline"] = cpu_baseline(r["p"], r["S"], r["U"], r["W"], args.cfl,
                                            args.cpu_seconds, threads)
+        out["cpu_baseline"].update(nproc=nproc, cpus_available=avail,
+                                   omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
         one = cpu_baseline(r["p"], r["S"], r["U"], r["W"], args.cfl, args.cpu_seconds / 2, 1)
         out["cpu_baseline"]["single_thread"] = {"value": one["value"], "sample": one["sample"]}
     if not args.no_ensemble:

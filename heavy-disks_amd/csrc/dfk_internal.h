@@ -37,6 +37,7 @@ enum {
   GT_DA,         // R_i (Rf_{i+1}-Rf_i) dphi  (C_m area weight)
   GT_GRC,        // 1/(2 du R_i) interior, 1/(du R_i) at i=0, Nu-1 (self-gravity g_R)
   GT_GPC,        // 1/(2 dphi R_i)                                  (self-gravity g_phi)
+  GT_DL,         // R_i dphi  (CFL azimuthal length, formed as hydro.hpp:56 forms it)
   GT_COUNT
 };
 

@@ -590,12 +590,25 @@ __device__ void wg_reduce5(double &mx, double &s0, double &s1, double &s2, doubl
   }
 }
 
-// CFL rate of one zone (hydro.hpp:52-57); NaN rates never win the max.
+// Running maximum of the CFL rate (|vR| + cs)/dR + (|vphi| + cs)/(R dphi)
+// of hydro.hpp:52-57, with the rate of the maximum formed exactly as the
+// reference forms it: the same sqrt, two IEEE divisions and a sum (dR and
+// R dphi from tables holding the reference's own products), so that dt =
+// cfl / max is the reference's to the bit for the same state (the max is
+// order-free).  The two divisions (about 25 instructions) are taken only
+// for zones that can raise the maximum: the rate is first estimated with
+// the reciprocal tables, within a few ulp of the exact one, and a zone
+// whose exact rate exceeds mx has an estimate >= mx (1 - 1e-14).  NaN rates
+// never win, as in the reference's `sR + sp > dtinv`.
 template <bool G2 = false>
-__device__ __forceinline__ double cfl_rate(const DiskDev &D, double s, double u, double w,
-                                           double invdR, double invdl) {
-  double cs = sqrt(cs2f<G2>(D, smax(s, 1e-12)));
-  return (fabs(u) + cs) * invdR + (fabs(w) + cs) * invdl;
+__device__ __forceinline__ void cfl_max(const DiskDev &D, double &mx, double s, double u, double w,
+                                        double invdR, double invdl, double dR, double dl) {
+  const double cs = sqrt(cs2f<G2>(D, smax(s, 1e-12)));
+  const double au = fabs(u) + cs, aw = fabs(w) + cs;
+  if (au * invdR + aw * invdl >= mx * (1.0 - 1e-14)) {
+    const double r = au / dR + aw / dl;
+    mx = (r > mx) ? r : mx;
+  }
 }
 
 __device__ __forceinline__ double nonfinite(double s, double u, double w) {
@@ -708,7 +721,7 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
   const double *S = stp(st, ntot, cur, 0, D), *U = stp(st, ntot, cur, 1, D),
                *W = stp(st, ntot, cur, 2, D);
   const double *invdR = gtab(geom, D, GT_INVDR), *invdl = gtab(geom, D, GT_INVDL),
-               *dA = gtab(geom, D, GT_DA);
+               *dR = gtab(geom, D, GT_AFACE), *dl = gtab(geom, D, GT_DL), *dA = gtab(geom, D, GT_DA);
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int e = threadIdx.x; e < D.ti * D.tj; e += blockDim.x) {
@@ -717,8 +730,7 @@ k_diag(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_
     if (i >= Nu || j >= Nphi) continue;
     const i64 c = (i64)i * Nphi + j;
     const double s = S[c], u = U[c], w = W[c];
-    const double r = cfl_rate(D, s, u, w, invdR[i], invdl[i]);
-    mx = (r > mx) ? r : mx;
+    cfl_max(D, mx, s, u, w, invdR[i], invdl[i], dR[i], dl[i]);
     const double sa = s * dA[i];
     a0 += sa;
     a1 += s * c2[j] * dA[i];
@@ -1319,7 +1331,7 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
   // ---- phase 5: azimuthal gather + floor, write, next-step CFL and C_m partials
   double *So = stpw(st, ntot, 1 - par, 0, D), *Uo = stpw(st, ntot, 1 - par, 1, D),
          *Wo = stpw(st, ntot, 1 - par, 2, D);
-  const double *dA = gtab(geom, D, GT_DA);
+  const double *dA = gtab(geom, D, GT_DA), *tDl = gtab(geom, D, GT_DL);
   const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int e = threadIdx.x; e < TI * TJ; e += blockDim.x) {
@@ -1343,8 +1355,7 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
     const i64 c = (i64)i * Nphi + j;
     So[c] = so; Uo[c] = uo; Wo[c] = wo;
     sink(lr, j, so);
-    const double r = cfl_rate<G2>(D, so, uo, wo, tInvdR[i], tInvdl[i]);
-    mx = (r > mx) ? r : mx;
+    cfl_max<G2>(D, mx, so, uo, wo, tInvdR[i], tInvdl[i], tAface[i], tDl[i]);
     a0 += so * dA[i];
     a1 += so * c2[j] * dA[i];
     a2 -= so * s2[j] * dA[i];
@@ -1575,7 +1586,7 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
                *tGpc = gtab(geom, D, GT_GPC), *tInvdR = gtab(geom, D, GT_INVDR),
                *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL),
                *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE),
-               *tdA = gtab(geom, D, GT_DA);
+               *tdA = gtab(geom, D, GT_DA), *tDl = gtab(geom, D, GT_DL);
   const double c2j = trig[D.cos_off + c], s2j = trig[D.cos_off + Nphi + c];
   const bool sg = !GARR && D.selfgrav != 0;
 
@@ -1731,8 +1742,7 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
             So[o] = so;
             Uo[o] = uo;
             Wo[o] = wo;
-            const double r = cfl_rate<G2>(D, so, uo, wo, tInvdR[g], tInvdl[g]);
-            mx = (r > mx) ? r : mx;
+            cfl_max<G2>(D, mx, so, uo, wo, tInvdR[g], tInvdl[g], tAface[g], tDl[g]);
             a0 += so * tdA[g];
             a1 += so * c2j * tdA[g];
             a2 -= so * s2j * tdA[g];

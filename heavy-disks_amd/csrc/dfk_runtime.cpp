@@ -89,6 +89,9 @@ int ip_nmax(int t) { return std::max(8 * t, 1024); }
 // work list so that XCD x receives one contiguous chunk of it: neighbouring
 // tiles (which share halo rows) then meet in the same L2.
 const int kXcds = 8;
+// Largest max|Im K-hat| / max|Re K-hat| accepted at create (round-off of the
+// transform of a real-even table is ~1e-16 x log2 of its size).
+const double kKhatImagTol = 1e-13;
 const int kStreamCols = 58;   // output columns per wave of the streaming update (kernel's STREAM_OUT)
 std::vector<int2> xcd_order(const std::vector<int2> &v) {
   const int L = (int)v.size();
@@ -665,6 +668,7 @@ void host_tables(const DiskDev &D, const dfk_disk_params &p, std::vector<double>
     T(GT_DA, i) = R[i] * (Rf[i + 1] - Rf[i]) * dphi;                            // diskmain.cpp:75
     T(GT_GRC, i) = (i == 0 || i == Nu - 1) ? 1.0 / (du * R[i]) : 1.0 / (2.0 * du * R[i]);  // kalnajs.hpp:74-77
     T(GT_GPC, i) = 1.0 / (2.0 * dphi * R[i]);                                   // kalnajs.hpp:72,78
+    T(GT_DL, i) = R[i] * dphi;                                                  // hydro.hpp:56
   }
   trig.assign((size_t)2 * Nphi, 0.0);
   for (int j = 0; j < Nphi; ++j) {
@@ -873,8 +877,24 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
     }
   }
   if ((rc = build_layout(h))) return fail(rc);
-  for (int d = 0; d < ndisks; ++d)
-    if (params[d].selfgrav && (rc = build_khat(h, d, nullptr))) return fail(rc);
+  for (int d = 0; d < ndisks; ++d) {
+    if (!params[d].selfgrav) continue;
+    // The loop multiplies by the real part of K-hat only.  K(du, dphi) is
+    // real and even in both arguments, so its transform is real up to
+    // round-off; check that on the transform just built.
+    std::vector<double2> full;
+    if ((rc = build_khat(h, d, &full))) return fail(rc);
+    double re = 0.0, im = 0.0;
+    for (const double2 &v : full) {
+      re = std::max(re, std::fabs(v.x));
+      im = std::max(im, std::fabs(v.y));
+    }
+    if (!(im <= kKhatImagTol * re)) {
+      set_err("disk %d: K-hat is not real (max|Im| / max|Re| = %.3g > %.0e)", d, im / re,
+              kKhatImagTol);
+      return fail(DFK_EHIP);
+    }
+  }
   *out = h;
   return DFK_OK;
 }
@@ -1169,14 +1189,17 @@ int dfk_get_status(dfk_handle *h, int d, dfk_status *st) {
 }
 
 namespace {
+// Snapshot header: the disk, its loop clock and every loop parameter of the
+// saved run (so that a load continues that run, whatever dfk_run_init set).
 struct SnapHeader {
-  char magic[8];             // "DFKSNAP1"
+  char magic[8];             // "DFKSNAP2"
   int Nu, Nphi;
   dfk_disk_params params;
-  double t, tnext, tlim, dtout, cfl;
-  long long step;
-  int flags, pad_;
+  double t, tnext, tlim, dtout, cfl, mD;
+  long long step, max_steps;
+  int flags, stop_on_nonfinite;
 };
+const char kSnapMagic[8] = {'D', 'F', 'K', 'S', 'N', 'A', 'P', '2'};
 }  // namespace
 
 int dfk_snapshot_save(dfk_handle *h, int d, const char *path) {
@@ -1189,13 +1212,16 @@ int dfk_snapshot_save(dfk_handle *h, int d, const char *path) {
   TRY(read_ctl(h, d, &c));
   SnapHeader hd;
   memset(&hd, 0, sizeof hd);
-  memcpy(hd.magic, "DFKSNAP1", 8);
+  memcpy(hd.magic, kSnapMagic, 8);
   hd.Nu = h->hD[d].Nu;
   hd.Nphi = h->hD[d].Nphi;
   hd.params = h->P[d];
   hd.t = c.t; hd.tnext = c.tnext; hd.tlim = c.tlim; hd.dtout = c.dtout; hd.cfl = c.cfl;
+  hd.mD = h->mD[d];
   hd.step = c.step;
+  hd.max_steps = c.max_steps;
   hd.flags = c.flags;
+  hd.stop_on_nonfinite = c.stop_on_nonfinite;
   const size_t N = (size_t)hd.Nu * hd.Nphi;
   std::vector<double> buf(3 * N);
   h->hcur[d] = c.cur;
@@ -1215,7 +1241,7 @@ int dfk_snapshot_load(dfk_handle *h, int d, const char *path) {
   FILE *f = fopen(path, "rb");
   if (!f) { set_err("cannot open %s", path); return DFK_EINVAL; }
   SnapHeader hd;
-  bool ok = fread(&hd, sizeof hd, 1, f) == 1 && memcmp(hd.magic, "DFKSNAP1", 8) == 0;
+  bool ok = fread(&hd, sizeof hd, 1, f) == 1 && memcmp(hd.magic, kSnapMagic, 8) == 0;
   const dfk_disk_params &p = h->P[d];
   if (ok && (hd.Nu != p.Nu || hd.Nphi != p.Nphi || hd.params.K != p.K || hd.params.eps != p.eps ||
              hd.params.gamma != p.gamma || hd.params.gmstar != p.gmstar || hd.params.Rin != p.Rin ||
@@ -1243,6 +1269,12 @@ int dfk_snapshot_load(dfk_handle *h, int d, const char *path) {
   c.tnext = hd.tnext;
   c.step = hd.step;
   c.flags = hd.flags;
+  c.tlim = hd.tlim;
+  c.dtout = hd.dtout;
+  c.cfl = hd.cfl;
+  c.max_steps = hd.max_steps;
+  c.stop_on_nonfinite = hd.stop_on_nonfinite;
+  h->mD[d] = hd.mD;
   c.done = 0;
   // the disk's sample history restarts at the restored state (samples of the
   // state it replaces, e.g. run_init's t = 0 output, are dropped)

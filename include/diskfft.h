@@ -129,12 +129,15 @@ int dfk_get_status(dfk_handle *h, int disk, dfk_status *st);
 
 /* Snapshot / resume (no counterpart in the reference, which writes only the
  * diagnostic series).  save writes disk `disk`'s constructor parameters, loop
- * clock (t, next output time, steps) and current state (Sigma, vR, vphi) to
- * a binary file.  load, called after dfk_run_init, restores state and clock of
- * a disk with the same grid and parameters (DFK_EINVAL otherwise); the run
- * then continues exactly as the saved one would have (dt is a function of the
- * state only).  The disk's undrained samples are discarded by load: its
- * sample series restarts with the first output after the restored time. */
+ * clock (t, next output time, steps), loop parameters (tlim, dtout, cfl, mD,
+ * max_steps, stop_on_nonfinite) and current state (Sigma, vR, vphi) to a
+ * binary file.  load, called after dfk_run_init (at any point of a run, on
+ * any disk of a handle), restores state, clock and loop parameters of a disk
+ * with the same grid and constructor parameters (DFK_EINVAL otherwise), in
+ * place of those dfk_run_init set for it; the run then continues exactly as
+ * the saved one would have (dt is a function of the state only).  The disk's
+ * undrained samples are discarded by load: its sample series restarts with
+ * the first output after the restored time. */
 int dfk_snapshot_save(dfk_handle *h, int disk, const char *path);
 int dfk_snapshot_load(dfk_handle *h, int disk, const char *path);
 

@@ -37,6 +37,9 @@ def test_diskdisk_matches_reference_driver(tmp_path, args):
     # same output times and step count; values to the printed precision
     assert np.array_equal(mod_g[:, 0], mod_o[:, 0])
     assert np.max(np.abs(mod_g[:, 1:3] - mod_o[:, 1:3]) / np.abs(mod_o[:, 1:3])) < 2e-6
+    # arg(C2), printed %.6e: to that precision, modulo 2 pi
+    dph = np.angle(np.exp(1j * (mod_g[:, 3] - mod_o[:, 3])))
+    assert np.max(np.abs(dph) / np.maximum(np.abs(mod_o[:, 3]), 1e-3)) < 2e-6
     assert out_g[-1].split("steps=")[1] == out_o[-1].split("steps=")[1]
     for lg, lo in zip(out_g[:-1], out_o[:-1]):
         assert lg.split("|C0|")[0] == lo.split("|C0|")[0]      # "t=..." fields

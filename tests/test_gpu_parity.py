@@ -133,15 +133,35 @@ def test_forces_match_oracle(orc, dfk, Nu, Nphi):
 
 
 # ------------------------------------------------------------------ hydro operators
-@pytest.mark.parametrize("Nu,Nphi", [(16, 24), (37, 52), (128, 128)])
-def test_timestep_matches_oracle(orc, dfk, Nu, Nphi):
-    print(f"case timestep {Nu}x{Nphi}", flush=True)
+@pytest.mark.parametrize("Nu,Nphi", [(16, 24), (37, 52), (128, 128), (256, 256), (512, 1024)])
+@pytest.mark.parametrize("seed", [5, 6, 7])
+def test_timestep_matches_oracle(orc, dfk, Nu, Nphi, seed):
+    """DiskHydro::timestep (hydro.hpp:48-61) is the same double: the rate of
+    the maximum is formed with the reference's sqrt and IEEE divisions and the
+    max is order-free.  Also for states with zones below the 1e-12 clamp."""
+    print(f"case timestep {Nu}x{Nphi} seed {seed}", flush=True)
     e, h, _ = _pair(orc, dfk, Nu, Nphi, selfgrav=0)
-    st = rough_state(Nu, Nphi, seed=5)
+    st = rough_state(Nu, Nphi, seed=seed)
+    if seed == 7:
+        st[0][1::3, ::5] = 1e-14
+    e.set_state(*st)
+    h.set_state(*st)
+    for cfl in (0.3, 0.4):
+        dg, do = e.timestep(cfl), h.timestep(cfl)
+        assert dg == do, (dg, do)
+
+
+def test_timestep_general_eos(orc, dfk):
+    """gamma = 1.5 takes pow (hydro.hpp:44): the device pow and the host's may
+    differ in the last bit, so dt agrees to round-off there."""
+    print("case timestep general EOS", flush=True)
+    Nu, Nphi = 64, 96
+    e, h, _ = _pair(orc, dfk, Nu, Nphi, selfgrav=0, K=0.2, gamma=1.5)
+    st = rough_state(Nu, Nphi, seed=8)
     e.set_state(*st)
     h.set_state(*st)
     dg, do = e.timestep(0.3), h.timestep(0.3)
-    assert abs(dg - do) / do < 1e-14
+    assert abs(dg - do) / do < 1e-15
 
 
 @pytest.mark.parametrize("stage", [1, 2, 3, 4])
@@ -268,6 +288,58 @@ def test_general_eos_loop_and_mixed_ensemble(orc, dfk, kind):
             assert np.array_equal(a, b)
 
 
+def _powerlaw_ic(orc, dfk, Nu, Nphi, eps=0.01):
+    """The benchmark's power-law disk (r in [0.1, 1]), its equilibrium from the
+    oracle's gravity, so both sides start from the same arrays."""
+    m = dfk.model_defaults(dfk.MODEL_POWERLAW)
+    k = orc.Kalnajs(Nu, Nphi, m.Rin, m.Rout, eps, m.G)
+    p, S, U, W, mD, _ = dfk.model_ic(m, Nu, Nphi, eps,
+                                     potential_forces=lambda x: k.forces(k.potential(x)))
+    h = orc.Hydro(Nu, Nphi, p["Rin"], p["Rout"], p["K"], p["gamma"], p["gmstar"])
+    h.set_state(S, U, W)
+    return p, (S, U, W), mD, h, k
+
+
+@pytest.mark.parametrize("kind", [0, 1, "tuned"])
+@pytest.mark.parametrize("Nu,Nphi", [(256, 256), (512, 1024)])
+def test_powerlaw_ten_step_loop_parity(orc, dfk, Nu, Nphi, kind):
+    """The loop gate on the benchmark's own disk: power-law model, eps 0.01,
+    CFL 0.4, 256 x 256 (the single-disk benchmark) and 512 x 1024 (the
+    bandwidth case), with each update kernel and with the tuned shapes the
+    benchmark runs.  Sigma, vR, vphi after 10 steps within 1e-10; the first
+    step's dt (same state on both sides) is the same double, so t after one
+    step is equal."""
+    import autotune
+    print(f"case power-law 10-step loop {Nu}x{Nphi} kernel {kind}", flush=True)
+    p, ic, mD, h, k = _powerlaw_ic(orc, dfk, Nu, Nphi)
+    e = dfk.Ensemble(p)
+    if kind == "tuned":
+        assert autotune.apply(e, regime="latency") or autotune.apply(e, regime="throughput")
+    else:
+        e.set_update_kernel(kind)
+    e.set_state(*ic)
+    e.run_init(1e30, dtout=0.25, cfl=0.4, mD=mD, max_steps=1)
+    e.advance(2)
+    h1 = orc.Hydro(Nu, Nphi, p["Rin"], p["Rout"], p["K"], p["gamma"], p["gmstar"])
+    h1.set_state(*ic)
+    _, t1, _, _ = orc.run(h1, k, 1, 1e30, dtout=0.25, cfl=0.4, max_steps=1, mD=mD)
+    assert e.status()["t"] == t1
+    e.set_state(*ic)
+    so, tf, c2f, steps = orc.run(h, k, 1, 1e30, dtout=0.25, cfl=0.4, max_steps=10, mD=mD)
+    e.run_init(1e30, dtout=0.25, cfl=0.4, mD=mD, max_steps=10)
+    e.advance(12)
+    st = e.status()
+    assert st["done"] and st["steps"] == 10 and steps == 10
+    assert abs(st["t"] - tf) / tf < 1e-13
+    errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
+    print("  rel errors Sigma, vR, vphi:", errs, flush=True)
+    assert max(errs) <= LOOP_TOL
+    sg = e.samples()
+    # |C2| of the seeded noise is a cancelling sum (~6e-6 of |C0|): compared
+    # relative to |C0|
+    assert sg.shape == so.shape and abs(sg[0, 2] - so[0, 2]) < 1e-12 * so[0, 1]
+
+
 def test_loop_samples_and_clip(orc, dfk):
     """Output schedule and last-step clip of diskmain.cpp:92-110 over ~300 steps."""
     Nu, Nphi, eps, tlim = 64, 64, 0.02, 0.6
@@ -283,6 +355,8 @@ def test_loop_samples_and_clip(orc, dfk):
     assert sg.shape == so.shape
     assert np.max(np.abs(sg[:, 0] - so[:, 0])) < 1e-12
     assert relerr(sg[:, 2], so[:, 2]) < 1e-9
+    # arg C2 (the fourth column of diskfft_modes.dat), compared modulo 2 pi
+    assert np.max(np.abs(np.angle(np.exp(1j * (sg[:, 3] - so[:, 3]))))) < 1e-9
     assert abs(st["c2_final"] - c2f) / c2f < 1e-9
     errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
     print(f"  {steps} steps, rel errors:", errs, flush=True)
@@ -328,6 +402,50 @@ def test_ensemble_matches_single_disks(orc, dfk, kind):
         assert ens.status(d)["steps"] == caps[d]
         for a, b in zip(ens.get_state(d), one.get_state()):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_mixed_selfgrav_ensemble(orc, dfk, kind):
+    """A handle mixing self-gravitating disks and disks without self-gravity
+    (one graph chain; every disk's loop control rides in the row-transform
+    launch) gives each disk the bits of its own run, and a disk without
+    self-gravity matches the oracle's 10 steps."""
+    print(f"case mixed selfgrav ensemble, update kernel {kind}", flush=True)
+    shapes = [(32, 48, 0.02, 1), (64, 64, 0.01, 0), (48, 30, 0.04, 1), (40, 130, 0.01, 0)]
+    params = [dfk.disk_params(a, b, eps=c, selfgrav=g) for a, b, c, g in shapes]
+    hs = [_lka98(orc, a, b, c, selfgrav=g) for a, b, c, g in shapes]
+    ics = [h.state() for h, _ in hs]
+    mDs = [h.disk_mass() for h, _ in hs]
+    caps = [20, 10, 27, 31]
+    ens = dfk.Ensemble(params)
+    ens.set_update_kernel(kind)
+    ens.set_chains(3)
+    assert ens.get_chains() == 1
+    for d in range(len(shapes)):
+        ens.set_state(*ics[d], d=d)
+    ens.run_init(1e9, dtout=0.01, mD=mDs, max_steps=caps, sample_cap=64)
+    ens.advance(40)
+    for d, p in enumerate(params):
+        one = dfk.Ensemble(p)
+        one.set_update_kernel(kind)
+        one.set_shapes(*ens.get_shapes(d))
+        one.set_state(*ics[d])
+        one.run_init(1e9, dtout=0.01, mD=mDs[d], max_steps=caps[d], sample_cap=64)
+        one.advance(40)
+        assert ens.status(d) == one.status()
+        assert ens.status(d)["steps"] == caps[d]
+        for a, b in zip(ens.get_state(d), one.get_state()):
+            assert np.array_equal(a, b)
+        assert np.array_equal(ens.samples(d), one.samples())
+        one.close()
+    h, k = hs[1]
+    orc.run(h, k, 0, 1e9, dtout=0.01, cfl=0.3, max_steps=10, mD=mDs[1])
+    e = dfk.Ensemble(params[1])
+    e.set_state(*ics[1])
+    e.run_init(1e9, dtout=0.01, mD=mDs[1], max_steps=10)
+    e.advance(12)
+    errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
+    assert max(errs) <= LOOP_TOL, errs
 
 
 @pytest.mark.parametrize("Nu,Nphi,ok", [(256, 256, 1), (96, 80, 0), (512, 1024, 1), (3, 4, 1),
