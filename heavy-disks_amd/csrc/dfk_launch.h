@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dfk_internal.h"
+#include "../../include/diskfft.h"
 
 hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
                               const double *pcfl, const double *pcm, DfkSampleDev *samples,
@@ -73,5 +74,13 @@ hipError_t dfk_launch_forces(hipStream_t s, const DiskDev *D, long long N, const
 hipError_t dfk_launch_stage(hipStream_t s, const DiskDev *D, long long N, int stage, double *st,
                             long long ntot, int cur, double dt, const double *gR,
                             const double *gp, const double *geom, double *F);
+
+// Scratch of the no-allocation entry point (dfk_noalloc.hip).  init: once
+// per device, with that device current (dfk_create); returns 0, or 1 with a
+// message in err.  ctl: the device's zeroed control block (null before
+// init); spectrum: its half-spectrum area of DFK_NOALLOC_MAX_SPECTRUM double2.
+int dfk_noalloc_init(int device, char *err, size_t errlen);
+void *dfk_noalloc_ctl(int device);
+void *dfk_noalloc_spectrum(int device);
 
 #endif

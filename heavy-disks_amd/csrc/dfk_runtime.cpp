@@ -158,6 +158,7 @@ struct dfk_handle {
   std::vector<hipStream_t> side; // streams of chains 1 .. nchains-1 (default priority)
   std::vector<hipEvent_t> join;  // their join events; ev_fork starts them
   hipEvent_t ev_fork = nullptr;
+  std::string noalloc_err;      // why the no-allocation scratch is unavailable ("" if it is)
 };
 
 namespace {
@@ -390,11 +391,13 @@ hipError_t launch_c2r(dfk_handle *h, hipStream_t s, int nblk, int base) {
 // (cooperative, wave-local, workgroup in place) for the disks that use it.
 // ctl_par >= 0: the last of these launches also runs the loop control of
 // disks a .. b-1 (state buffer ctl_par) in trailing workgroups.
+// ctl: the control blocks the kernels read (null: the handle's).
 hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b, double2 *spec = nullptr,
-                             int ctl_par = -1) {
+                             int ctl_par = -1, DiskCtl *ctl = nullptr) {
   const int n = h->conv0[b] - h->conv0[a], nw = h->convw0[b] - h->convw0[a];
   const int ng = h->convg0[b] - h->convg0[a];
   if (!spec) spec = h->spec;
+  if (!ctl) ctl = h->ctl;
   DfkConvCtl none, c;
   if (ctl_par >= 0) {
     c.nctl = b - a;
@@ -407,13 +410,13 @@ hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b, double2
   }
   const int last = ng > 0 ? 2 : (nw > 0 ? 1 : 0);
   hipError_t e = dfk_launch_conv(s, n, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[a],
-                                 h->ctl, h->khat, h->tw, spec, last == 0 ? c : none);
+                                 ctl, h->khat, h->tw, spec, last == 0 ? c : none);
   if (e != hipSuccess) return e;
   e = dfk_launch_conv_wave(s, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
-                           h->convw0[a], h->ctl, h->khat, h->tw, spec, last == 1 ? c : none);
+                           h->convw0[a], ctl, h->khat, h->tw, spec, last == 1 ? c : none);
   if (e != hipSuccess) return e;
   return dfk_launch_conv_group(s, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
-                               h->convg0[a], h->ctl, h->khat, h->tw, spec, h->conv_batched,
+                               h->convg0[a], ctl, h->khat, h->tw, spec, h->conv_batched,
                                last == 2 ? c : none);
 }
 
@@ -785,6 +788,12 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
   if (hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
       lmax > 0)
     h->lds_max = (size_t)lmax;
+  {
+    // the no-allocation entry point's scratch address on this device (its
+    // failure only disables that entry point, which then reports it)
+    char msg[256] = "";
+    if (dfk_noalloc_init(device, msg, sizeof msg)) h->noalloc_err = msg;
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     set_err("stream create failed");
     return fail(DFK_EHIP);
@@ -1014,6 +1023,44 @@ int dfk_potential(dfk_handle *h, int d, const double *Sigma, double *Phi) {
   TRY(launch_gravity_disk(h, d, scratch));
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(Phi, h->phi + h->hD[d].so, n, hipMemcpyDeviceToHost));
+  return DFK_OK;
+}
+
+int dfk_potential_noalloc(dfk_handle *h, int d, const double *Sigma, double *Phi, void *stream) {
+  TRY(check_disk(h, d));
+  if (!Sigma || !Phi) { set_err("null pointer"); return DFK_EINVAL; }
+  const DiskDev &D = h->hD[d];
+  if (!D.selfgrav) { set_err("disk %d was created with selfgrav = 0 (no K-hat)", d); return DFK_ESTATE; }
+  if (!h->layout_ok) { set_err("dfk_potential_noalloc: the handle's last shape change failed"); return DFK_ESTATE; }
+  if ((i64)D.Nu * D.Nc > DFK_NOALLOC_MAX_SPECTRUM) {
+    set_err("dfk_potential_noalloc: disk %d needs %lld spectrum entries, the scratch holds %d",
+            d, (i64)D.Nu * D.Nc, DFK_NOALLOC_MAX_SPECTRUM);
+    return DFK_EINVAL;
+  }
+  DiskCtl *ctl = (DiskCtl *)dfk_noalloc_ctl(h->device);
+  double2 *spec = (double2 *)dfk_noalloc_spectrum(h->device);
+  if (!ctl || !spec) {
+    set_err("dfk_potential_noalloc: scratch unavailable (%s)", h->noalloc_err.c_str());
+    return DFK_EHIP;
+  }
+  HIPCHK(hipSetDevice(h->device));
+  hipStream_t s = (hipStream_t)stream;   // NULL: the device's null stream
+  // The kernels address disk d's fields at its pool offsets (state D.so,
+  // spectrum D.spo) and its control block at ctl[d]; these bases put those
+  // addresses on the caller's arrays, the scratch spectrum and the scratch
+  // control block (all zero: not done).  Address arithmetic only.
+  auto shift = [](const void *p, i64 bytes) { return (uintptr_t)p - (uintptr_t)bytes; };
+  const double *st = (const double *)shift(Sigma, D.so * (i64)sizeof(double));
+  double *phi = (double *)shift(Phi, D.so * (i64)sizeof(double));
+  double2 *spb = (double2 *)shift(spec, D.spo * (i64)sizeof(double2));
+  DiskCtl *cb = (DiskCtl *)shift(ctl, (i64)d * (i64)sizeof(DiskCtl));
+  const int nr = h->r2c0[d + 1] - h->r2c0[d];
+  const int ip = h->rows_ip ? 1 + h->rows_batched : 0;
+  HIPCHK(dfk_launch_r2c(s, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], cb, st, 0,
+                        h->geom, h->tw, spb, 0, 0, 0, nullptr, nullptr, nullptr, 0, ip));
+  HIPCHK(launch_conv_range(h, s, d, d + 1, spb, -1, cb));
+  HIPCHK(dfk_launch_c2r(s, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], cb, spb,
+                        h->geom, h->tw, phi, ip));
   return DFK_OK;
 }
 

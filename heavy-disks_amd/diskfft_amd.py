@@ -35,7 +35,7 @@ SYMBOLS = [
     "dfk_set_threads", "dfk_get_threads", "dfk_set_update_kernel", "dfk_get_update_kernel",
     "dfk_set_chains", "dfk_get_chains", "dfk_set_row_transform", "dfk_get_row_transform",
     "dfk_set_fused_rows", "dfk_get_fused_rows",
-    "dfk_snapshot_save", "dfk_snapshot_load", "dfk_set_eos",
+    "dfk_snapshot_save", "dfk_snapshot_load", "dfk_set_eos", "dfk_potential_noalloc",
 ]
 
 
@@ -115,6 +115,7 @@ def lib():
             "dfk_snapshot_save": (_I, [_P, _I, ctypes.c_char_p]),
             "dfk_snapshot_load": (_I, [_P, _I, ctypes.c_char_p]),
             "dfk_set_eos": (_I, [_P, _I, _D, _D]),
+            "dfk_potential_noalloc": (_I, [_P, _I, _P, _P, _P]),
         }
         for k, (res, args) in sig.items():
             f = getattr(L, k)
@@ -204,6 +205,13 @@ class Ensemble:
         out = np.empty(s)
         _check(lib().dfk_potential(self.h, d, _dp(a), _dp(out)))
         return out
+
+    def potential_noalloc(self, sigma_dev, phi_dev, stream=None, d=0):
+        """dfk_potential_noalloc: Phi from Sigma, both device addresses (ints)
+        on the handle's device; enqueued on `stream` (a hipStream_t address,
+        None: the device's null stream), nothing allocated."""
+        _check(lib().dfk_potential_noalloc(self.h, d, _P(sigma_dev), _P(phi_dev),
+                                           None if stream is None else _P(stream)))
 
     def forces(self, Phi, d=0):
         s = self.shape(d)

@@ -96,6 +96,23 @@ int dfk_timestep(dfk_handle *h, int disk, double cfl, double *dt);
 /* KalnajsSolver::potential (kalnajs.hpp:48-63): Phi from Sigma (Nu*Nphi). */
 int dfk_potential(dfk_handle *h, int disk, const double *Sigma, double *Phi);
 
+/* dfk_potential on device memory, allocating nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Sigma and Phi are device pointers on the handle's device (Nu*Nphi doubles,
+ * row-major); stream is a hipStream_t (NULL: the device's null stream).  The call
+ * only enqueues the three transform kernels on that stream and returns; Phi
+ * is ready when the stream reaches that point.  Same kernels and bits as
+ * dfk_potential.  The half spectrum lives in a scratch area of the library
+ * (one per device, held until the process ends) of DFK_NOALLOC_MAX_SPECTRUM
+ * complex entries: a disk with Nu*(Nphi/2+1) above that gets DFK_EINVAL.
+ * Calls that overlap on one device (two streams, two threads, two handles)
+ * race on that scratch; calls ordered on one stream do not.  Needs a handle
+ * whose last call succeeded (shapes set, K-hat built); it does not touch the
+ * handle's run, state or loop clock. */
+#define DFK_NOALLOC_MAX_SPECTRUM 1048576
+int dfk_potential_noalloc(dfk_handle *h, int disk, const double *Sigma, double *Phi, void *stream);
+
 /* KalnajsSolver::forces (kalnajs.hpp:65-80). */
 int dfk_forces(dfk_handle *h, int disk, const double *Phi, double *gR, double *gphi);
 
