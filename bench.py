@@ -134,13 +134,17 @@ def cpu_baseline(p, S, U, W, cfl, seconds, threads):
                        f"({dt:.1f} s), oracle built -O3 -march=x86-64-v3, OpenMP {threads} threads")
 
 
-def ensemble_legs(dfk, steps_batch, steps_seq, chains=0):
+def ensemble_legs(dfk, steps_batch, steps_seq, chains=0, cols_mode=None):
     import ensemble_cfg
     disks = ensemble_cfg.draw(64, seed=2026)
     ens, ics = ensemble_cfg.build(dfk, disks)
     ntuned = autotune.apply(ens, regime="throughput")
     if chains:
         ens.set_chains(chains)
+    if cols_mode is not None:   # column-convolution mode of every disk, over the tuned one
+        for d in range(64):
+            r, c, _ = ens.get_fft_options(d)
+            ens.set_fft_options(r, c, cols_mode, d=d)
     zones = sum(d["Nu"] * d["Nphi"] for d in disks)
     ens.set_graph_steps(50)
     ens.run_init(1e30, cfl=0.4, mD=[ic[3] for ic in ics], sample_cap=64)
@@ -204,6 +208,8 @@ def main():
                     help="only the batched 64-disk leg (no single disk, no sequential leg)")
     ap.add_argument("--chains", type=int, default=0,
                     help="graph chains of the ensemble handle (0: the library default)")
+    ap.add_argument("--cols-mode", type=int, default=None,
+                    help="column-convolution mode of every ensemble disk (dfk_set_fft_options)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -214,8 +220,8 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
     if args.ensemble_only:
-        print(json.dumps({"ensemble": ensemble_legs(dfk, args.ensemble_steps, 0, args.chains)}),
-              flush=True)
+        print(json.dumps({"ensemble": ensemble_legs(dfk, args.ensemble_steps, 0, args.chains,
+                                                     args.cols_mode)}), flush=True)
         return
     Nu, Nphi = args.nu, args.nphi
     r = single_disk(dfk, Nu, Nphi, args.eps, args.cfl, args.steps, args.warmup, args.chunk)

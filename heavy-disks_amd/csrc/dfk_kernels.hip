@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dfk_internal.h"
+#include "dfk_regfft.h"
 
 #define WG 256
 
@@ -1052,6 +1053,83 @@ k_cols_conv_group(const DiskDev *__restrict__ disks, const int2 *__restrict__ ma
   }
 }
 
+// Column convolution by the two-pass register transform of dfk_regfft.h
+// (the disks whose runtime marked their columns mode 3): M = N1 N2 = 2 Nu
+// fixed at compile time, one 64-lane wavefront per workgroup and N2 lanes
+// per column (64 / N2 columns per workgroup), so every synchronisation is a
+// wavefront one.  Per column: pass 1 reads rows < Nu (for a fixed n1, the
+// lanes read consecutive rows), one LDS transpose per direction in H parts,
+// pass 3 writes rows < Nu.  Index arithmetic is compile-time but for the
+// column.
+template <int N1, int N2, int H>
+__device__ __forceinline__ void conv_reg_body(const DiskDev &D, int grp, double2 *lds2,
+                                              const double *__restrict__ khat,
+                                              const double2 *__restrict__ tw,
+                                              double2 *__restrict__ spec) {
+  constexpr int Nu = N1 * N2 / 2, CPW = 64 / N2, LE = rfft::conv::lds_elems_h<N1, N2, H>();
+  const int c = threadIdx.x / N2, t = threadIdx.x % N2;
+  const int col = grp * CPW + c;
+  const bool live = col < D.Nc;
+  const int cc = live ? col : D.Nc - 1;   // columns past the spectrum read the last one, store nothing
+  double2 *L = lds2 + c * LE;
+  const double2 *TW = tw + D.two_col;
+  double2 *io = spec + D.spo + (i64)cc * Nu;
+  const double *kh = khat + D.kho + (i64)cc * (Nu + 1);
+  double2 v[N1];
+  rfft::conv::pass1<N1, N2>(t, [&](int r) {
+    const double2 q = io[r];
+    return live ? q : make_double2(0.0, 0.0);
+  }, TW, v);
+  double2 x[N2];
+  rfft::sfor<0, H>([&](auto h_) {
+    constexpr int hh = decltype(h_)::value;
+    rfft::conv::fwd_write<N1, N2, H>(t, hh, v, L);
+    wave_lds_sync();
+    if (t < N1) rfft::conv::fwd_read_c<N1, N2, H, hh>(t, L, x);
+    wave_lds_sync();
+  });
+  if (t < N1) {
+    double kv[N2];
+    rfft::conv::pass2_khat<N1, N2>(t, [&](int q) { return kh[q]; }, kv);
+    rfft::conv::pass2<N1, N2>(t, x, kv, TW);
+  }
+  double2 z[N1];
+  rfft::sfor<0, H>([&](auto h_) {
+    constexpr int hh = decltype(h_)::value;
+    rfft::conv::inv_write<N1, N2, H>(t, hh, x, L);
+    wave_lds_sync();
+    rfft::conv::inv_read_c<N1, N2, H, hh>(t, L, z);
+    if (hh + 1 < H) wave_lds_sync();
+  });
+  rfft::conv::pass3<N1, N2>(t, z, [&](int r, double2 q) {
+    if (live) io[r] = q;
+  });
+}
+
+// One launch for every disk of the mode, whatever its transform length: the
+// workgroup branches once on its disk's M (workgroups are dispatched in map
+// order, disk by disk, so a CU mostly runs one instance at a time).
+__global__ void __launch_bounds__(64)
+k_cols_conv_reg(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
+                DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
+                const double2 *__restrict__ tw, double2 *__restrict__ spec, CONV_CTL_PARAMS) {
+  extern __shared__ double2 lds2[];
+  CONV_CONTROL
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  switch (D.M) {
+    case 128: conv_reg_body<8, 16, 1>(D, wm.y, lds2, khat, tw, spec); break;
+    case 192: conv_reg_body<12, 16, 1>(D, wm.y, lds2, khat, tw, spec); break;
+    case 256: conv_reg_body<16, 16, 1>(D, wm.y, lds2, khat, tw, spec); break;
+    case 384: conv_reg_body<12, 32, 2>(D, wm.y, lds2, khat, tw, spec); break;
+    case 512: conv_reg_body<16, 32, 2>(D, wm.y, lds2, khat, tw, spec); break;
+    case 768: conv_reg_body<24, 32, 2>(D, wm.y, lds2, khat, tw, spec); break;
+    case 1024: conv_reg_body<32, 32, 2>(D, wm.y, lds2, khat, tw, spec); break;
+    default: break;   // the runtime marks no other length mode 3
+  }
+}
+
 // Row c2r (FFTW convention: imaginary parts of the DC and Nyquist bins are
 // ignored) and Phi = pref * V / sqrt(R).  TG as for the r2c.
 template <int TG, bool BATCH = true>
@@ -1985,6 +2063,44 @@ hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, 
   if (nthr != 64 && nthr != 128 && nthr != 192 && nthr != 256) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k, dim3(nblk + c.nctl), dim3(nthr), lds, s, disks, map, base, ctl, khat, tw, spec,
                      CONV_CTL_ARGS);
+  return hipGetLastError();
+}
+
+// (N1, N2, H) of the two-pass register convolution for transform length M, or 0s
+static void conv_reg_split(int M, int *n1, int *n2, int *h) {
+  *h = 2;
+  switch (M) {
+    case 128: *n1 = 8; *n2 = 16; *h = 1; return;
+    case 192: *n1 = 12; *n2 = 16; *h = 1; return;
+    case 256: *n1 = 16; *n2 = 16; *h = 1; return;
+    case 384: *n1 = 12; *n2 = 32; return;
+    case 512: *n1 = 16; *n2 = 32; return;
+    case 768: *n1 = 24; *n2 = 32; return;
+    case 1024: *n1 = 32; *n2 = 32; return;
+    default: *n1 = *n2 = *h = 0;
+  }
+}
+
+int dfk_conv_reg_cols(int M) {
+  int n1, n2, h;
+  conv_reg_split(M, &n1, &n2, &h);
+  return n2 ? 64 / n2 : 0;
+}
+
+size_t dfk_conv_reg_lds(int M) {
+  int n1, n2, h;
+  conv_reg_split(M, &n1, &n2, &h);
+  if (!n2) return 0;
+  const int a = n1 * (n2 / h + 1), b = n2 * (n1 / h + 1);
+  return (size_t)(64 / n2) * (a > b ? a : b) * sizeof(double2);
+}
+
+hipError_t dfk_launch_conv_reg(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+                               const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                               const double2 *tw, double2 *spec, const DfkConvCtl &c) {
+  if (nblk + c.nctl <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cols_conv_reg, dim3(nblk + c.nctl), dim3(64), lds, s, disks, map, base, ctl,
+                     khat, tw, spec, CONV_CTL_ARGS);
   return hipGetLastError();
 }
 

@@ -39,6 +39,15 @@ hipError_t dfk_launch_conv_wave(hipStream_t s, int nblk, int nthr, size_t lds, c
 hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                                  const int2 *map, int base, DiskCtl *ctl, const double *khat,
                                  const double2 *tw, double2 *spec, int batched, const DfkConvCtl &c);
+// Two-pass register column convolution (transform lengths M = 128, 192, 256,
+// 384, 512, 768, 1024): columns per 64-thread workgroup (0: M unsupported),
+// dynamic LDS bytes a disk of that M needs, and the launch (lds: the largest
+// need among the launch's disks).
+int dfk_conv_reg_cols(int M);
+size_t dfk_conv_reg_lds(int M);
+hipError_t dfk_launch_conv_reg(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
+                               const int2 *map, int base, DiskCtl *ctl, const double *khat,
+                               const double2 *tw, double2 *spec, const DfkConvCtl &c);
 // Fused row kernel (row c2r + update + row r2c) over row blocks of ti x Nphi,
 // nthr = 256, 512 or 1024 threads; V-hat read from vin, the new state's
 // S-hat written to sout.

@@ -127,9 +127,9 @@ struct dfk_handle {
   double *pcfl = nullptr, *pcm = nullptr;
   i64 ntiles = 0;
   int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr, *map_convw = nullptr,
-       *map_convg = nullptr;
-  std::vector<int> upd0, r2c0, conv0, convw0, convg0;   // per-disk first block (size nd+1)
-  size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0, lds_convw = 0, lds_convg = 0;
+       *map_convg = nullptr, *map_convr = nullptr;
+  std::vector<int> upd0, r2c0, conv0, convw0, convg0, convr0;   // per-disk first block (size nd+1)
+  size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0, lds_convw = 0, lds_convg = 0, lds_convr = 0;
   size_t lds_max = 64 * 1024;   // per workgroup, from the device at create (gfx950: 160 KiB)
   DiskCtl *ctl = nullptr;
   // run
@@ -192,14 +192,15 @@ int build_layout(dfk_handle *h) {
   drop_graphs(h);
   h->run_ready = false;
   // update, rows, columns: cooperative, wave-local and workgroup in-place
-  std::vector<int2> mu, mr, mc, mcw, mcg;
+  std::vector<int2> mu, mr, mc, mcw, mcg, mreg;
+  h->convr0.assign(h->nd + 1, 0);
   h->upd0.assign(h->nd + 1, 0);
   h->r2c0.assign(h->nd + 1, 0);
   h->conv0.assign(h->nd + 1, 0);
   h->convw0.assign(h->nd + 1, 0);
   h->convg0.assign(h->nd + 1, 0);
   i64 part = 0;
-  size_t lu = 0, lr = 0, lc = 0, lcw = 0, lcg = 0;
+  size_t lu = 0, lr = 0, lc = 0, lcw = 0, lcg = 0, lcr = 0;
   const size_t kMaxLds = h->lds_max;
   h->rows_ip = h->row_inplace;
   for (const DiskDev &D : h->hD)
@@ -246,6 +247,7 @@ int build_layout(dfk_handle *h) {
     h->conv0[d] = (int)mc.size();
     h->convw0[d] = (int)mcw.size();
     h->convg0[d] = (int)mcg.size();
+    h->convr0[d] = (int)mreg.size();
     if (D.selfgrav) {
       // batches shrink until they fit the LDS budget (a cached shape from a
       // build with smaller LDS needs stays usable)
@@ -265,11 +267,16 @@ int build_layout(dfk_handle *h) {
                           (D.conv_wave == 2 && D.cb * D.M > ip_nmax(h->thr_cols))))
         --D.cb;
       for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
-      std::vector<int2> &cm = D.conv_wave == 2 ? mcg : (D.conv_wave == 1 ? mcw : mc);
-      for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) cm.push_back(make_int2(d, g));
       lr = std::max(lr, lds_rows(D.pr));
-      size_t &lcol = D.conv_wave == 2 ? lcg : (D.conv_wave == 1 ? lcw : lc);
-      lcol = std::max(lcol, lds_cols(D.cb));
+      if (D.conv_wave == 3) {   // columns per workgroup fixed by the transform length
+        for (int g = 0; g < ceil_div(D.Nc, dfk_conv_reg_cols(D.M)); ++g) mreg.push_back(make_int2(d, g));
+        lcr = std::max(lcr, dfk_conv_reg_lds(D.M));
+      } else {
+        std::vector<int2> &cm = D.conv_wave == 2 ? mcg : (D.conv_wave == 1 ? mcw : mc);
+        for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) cm.push_back(make_int2(d, g));
+        size_t &lcol = D.conv_wave == 2 ? lcg : (D.conv_wave == 1 ? lcw : lc);
+        lcol = std::max(lcol, lds_cols(D.cb));
+      }
     }
   }
   h->rows_batched = h->conv_batched = 0;
@@ -282,6 +289,7 @@ int build_layout(dfk_handle *h) {
   h->conv0[h->nd] = (int)mc.size();
   h->convw0[h->nd] = (int)mcw.size();
   h->convg0[h->nd] = (int)mcg.size();
+  h->convr0[h->nd] = (int)mreg.size();
   if (lu > kMaxLds || lr > kMaxLds || lc > kMaxLds || lcw > kMaxLds || lcg > kMaxLds || lf > kMaxLds) {
     set_err("tile/batch shapes need %zu/%zu/%zu/%zu/%zu/%zu bytes of LDS (max %zu)", lu, lr, lc, lcw,
             lcg, lf, kMaxLds);
@@ -294,10 +302,11 @@ int build_layout(dfk_handle *h) {
   h->lds_conv = lc;
   h->lds_convw = lcw;
   h->lds_convg = lcg;
+  h->lds_convr = lcr;
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv);
-  hipFree(h->map_convw); hipFree(h->map_convg);
+  hipFree(h->map_convw); hipFree(h->map_convg); hipFree(h->map_convr);
   h->pcfl = h->pcm = nullptr;
-  h->map_upd = h->map_r2c = h->map_conv = h->map_convw = h->map_convg = nullptr;
+  h->map_upd = h->map_r2c = h->map_conv = h->map_convw = h->map_convg = h->map_convr = nullptr;
   h->ntiles = part;
   TRY(dalloc(&h->pcfl, part));
   TRY(dalloc(&h->pcm, 4 * part));
@@ -306,6 +315,7 @@ int build_layout(dfk_handle *h) {
   TRY(dalloc(&h->map_conv, (i64)mc.size()));
   TRY(dalloc(&h->map_convw, (i64)mcw.size()));
   TRY(dalloc(&h->map_convg, (i64)mcg.size()));
+  TRY(dalloc(&h->map_convr, (i64)mreg.size()));
   // unit calls launch one disk's range [upd0[d], upd0[d+1]) of the map, so
   // each disk's range is reordered on its own
   auto per_disk = [&](std::vector<int2> &m, const std::vector<int> &first) {
@@ -321,12 +331,14 @@ int build_layout(dfk_handle *h) {
     mc = xcd_order(mc);
     mcw = xcd_order(mcw);
     mcg = xcd_order(mcg);
+    mreg = xcd_order(mreg);
   } else {
     per_disk(mu, h->upd0);
     per_disk(mr, h->r2c0);
     per_disk(mc, h->conv0);
     per_disk(mcw, h->convw0);
     per_disk(mcg, h->convg0);
+    per_disk(mreg, h->convr0);
   }
   HIPCHK(hipMemcpy(h->map_upd, mu.data(), sizeof(int2) * mu.size(), hipMemcpyHostToDevice));
   if (!mr.empty()) HIPCHK(hipMemcpy(h->map_r2c, mr.data(), sizeof(int2) * mr.size(), hipMemcpyHostToDevice));
@@ -335,6 +347,8 @@ int build_layout(dfk_handle *h) {
     HIPCHK(hipMemcpy(h->map_convw, mcw.data(), sizeof(int2) * mcw.size(), hipMemcpyHostToDevice));
   if (!mcg.empty())
     HIPCHK(hipMemcpy(h->map_convg, mcg.data(), sizeof(int2) * mcg.size(), hipMemcpyHostToDevice));
+  if (!mreg.empty())
+    HIPCHK(hipMemcpy(h->map_convr, mreg.data(), sizeof(int2) * mreg.size(), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->dD, h->hD.data(), sizeof(DiskDev) * h->nd, hipMemcpyHostToDevice));
   h->layout_ok = true;
   return DFK_OK;
@@ -408,16 +422,22 @@ hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b, double2
     c.cap = h->cap;
     c.par = ctl_par;
   }
-  const int last = ng > 0 ? 2 : (nw > 0 ? 1 : 0);
+  // the control rides in the last launch that has work (the cooperative one,
+  // empty but for the control, when none has)
+  const int nr = h->convr0[b] - h->convr0[a];
+  const int last = nr > 0 ? 3 : (ng > 0 ? 2 : (nw > 0 ? 1 : 0));
   hipError_t e = dfk_launch_conv(s, n, h->thr_cols, h->lds_conv, h->dD, h->map_conv, h->conv0[a],
                                  ctl, h->khat, h->tw, spec, last == 0 ? c : none);
   if (e != hipSuccess) return e;
   e = dfk_launch_conv_wave(s, nw, h->thr_cols, h->lds_convw, h->dD, h->map_convw,
                            h->convw0[a], ctl, h->khat, h->tw, spec, last == 1 ? c : none);
   if (e != hipSuccess) return e;
-  return dfk_launch_conv_group(s, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
-                               h->convg0[a], ctl, h->khat, h->tw, spec, h->conv_batched,
-                               last == 2 ? c : none);
+  e = dfk_launch_conv_group(s, ng, h->thr_cols, h->lds_convg, h->dD, h->map_convg,
+                            h->convg0[a], ctl, h->khat, h->tw, spec, h->conv_batched,
+                            last == 2 ? c : none);
+  if (e != hipSuccess) return e;
+  return dfk_launch_conv_reg(s, nr, h->lds_convr, h->dD, h->map_convr, h->convr0[a], ctl, h->khat,
+                             h->tw, spec, last == 3 ? c : none);
 }
 
 // Column convolution of disk d (all disks for d < 0) on the handle's stream.
@@ -916,7 +936,7 @@ void dfk_destroy(dfk_handle *h) {
   hipFree(h->dD); hipFree(h->st); hipFree(h->phi); hipFree(h->gR); hipFree(h->gp); hipFree(h->F);
   hipFree(h->spec); hipFree(h->spec2); hipFree(h->khat); hipFree(h->geom); hipFree(h->trig); hipFree(h->tw);
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv); hipFree(h->map_convw);
-  hipFree(h->map_convg);
+  hipFree(h->map_convg); hipFree(h->map_convr);
   hipFree(h->ctl); hipFree(h->samples);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
@@ -1355,9 +1375,11 @@ int dfk_set_fft_options(dfk_handle *h, int d, int rows_tw_lds, int cols_tw_lds, 
   TRY(check_disk(h, d));
   DiskDev &D = h->hD[d];
   const int v = (rows_tw_lds ? 1 : 0) | (cols_tw_lds ? 2 : 0);
-  // the in-place column transforms need radices 2/3/4/8, >= 2 stages, M <= 1024
+  // the in-place column transforms need radices 2/3/4/8, >= 2 stages, M <= 1024;
+  // the two-pass register one a transform length it is built for
   const bool fast = D.M <= 1024 && inplace_radices(D.rad_col, D.nst_col, 2);
-  const int w = (fast && D.selfgrav && (cols_wave == 1 || cols_wave == 2)) ? cols_wave : 0;
+  int w = (fast && D.selfgrav && (cols_wave == 1 || cols_wave == 2)) ? cols_wave : 0;
+  if (cols_wave == 3 && D.selfgrav && dfk_conv_reg_cols(D.M) > 0) w = 3;
   if (v != D.tw_lds || w != D.conv_wave) {
     D.tw_lds = v;
     D.conv_wave = w;

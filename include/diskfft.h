@@ -177,9 +177,13 @@ int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
  *       in place in one LDS buffer, without workgroup barriers; 2 = the
  *       workgroup transforms its columns one after another, in place in a
  *       single LDS buffer.  1 and 2 need 2Nu <= 1024 with radices 2, 3, 4,
- *       8 (otherwise the value is ignored and read back as 0).
+ *       8 (otherwise the value is ignored and read back as 0); 3 = a two-pass
+ *       register transform per direction (2Nu = N1 N2, in-register N1- and
+ *       N2-point DFTs, one LDS transpose between them, 64 / N2 columns per
+ *       wavefront), for 2Nu in {128, 192, 256, 384, 512, 768, 1024} only
+ *       (otherwise read back as 0); it ignores cb and cols_tw_lds.
  * The twiddle placement does not change a bit of the results; the in-place
- * kernels agree with the cooperative one to round-off. */
+ * and register kernels agree with the cooperative one to round-off. */
 int dfk_set_fft_options(dfk_handle *h, int disk, int rows_tw_lds, int cols_tw_lds, int cols_wave);
 int dfk_get_fft_options(dfk_handle *h, int disk, int *rows_tw_lds, int *cols_tw_lds,
                         int *cols_wave);

@@ -121,6 +121,29 @@ def test_potential_fft_options(orc, dfk, Nu, Nphi, wave_ok, opts, cb, threads):
     assert relerr(p_opt, k.potential(Sig)) < 1e-13
 
 
+@pytest.mark.parametrize("Nu,Nphi,ok", [(64, 64, 1), (96, 80, 1), (128, 128, 1), (192, 192, 1),
+                                        (256, 256, 1), (384, 512, 1), (512, 512, 1),
+                                        (512, 1024, 1), (64, 30, 1), (35, 40, 0), (100, 64, 0)])
+def test_potential_register_columns(orc, dfk, Nu, Nphi, ok):
+    """The two-pass register column convolution (cols_wave = 3) at every
+    transform length it is built for (2Nu = 128 .. 1024): round-off agreement
+    with the cooperative transform and the oracle; other lengths keep the
+    cooperative one (read back as 0), the same bits."""
+    print(f"case potential {Nu}x{Nphi} register columns", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    rng = np.random.default_rng(Nu + 3 * Nphi)
+    Sig = rng.random((Nu, Nphi))
+    p_ref = e.potential(Sig)
+    e.set_fft_options(1, 0, 3)
+    assert e.get_fft_options(0)[2] == 3 * ok
+    p_reg = e.potential(Sig)
+    if ok:
+        assert relerr(p_reg, p_ref) < 1e-14
+    else:
+        assert np.array_equal(p_reg, p_ref)
+    assert relerr(p_reg, k.potential(Sig)) < 1e-13
+
+
 @pytest.mark.parametrize("Nu,Nphi", [(16, 16), (35, 40), (128, 128), (33, 26)])
 def test_forces_match_oracle(orc, dfk, Nu, Nphi):
     print(f"case forces {Nu}x{Nphi}", flush=True)
@@ -478,15 +501,17 @@ def test_potential_row_transform(orc, dfk, Nu, Nphi, ok, threads, pr, cb):
     assert relerr(p_opt, k.potential(Sig)) < 1e-13
 
 
+@pytest.mark.parametrize("cols", [2, 3])
 @pytest.mark.parametrize("Nu,Nphi,eps", [(64, 96, 0.02), (256, 256, 0.01)])
-def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps):
+def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps, cols):
     """The 10-step loop gate with in-place row transforms and the in-place
-    workgroup column convolution."""
-    print(f"case loop {Nu}x{Nphi} in-place transforms", flush=True)
+    workgroup (2) or two-pass register (3) column convolution."""
+    print(f"case loop {Nu}x{Nphi} in-place transforms, columns mode {cols}", flush=True)
     h, k = _lka98(orc, Nu, Nphi, eps)
     e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
     e.set_row_transform(1)
-    e.set_fft_options(1, 0, 2)
+    e.set_fft_options(1, 0, cols)
+    assert e.get_fft_options(0)[2] == cols
     e.set_threads(128, 128, 256)
     e.set_state(*h.state())
     mD = h.disk_mass()
@@ -507,8 +532,10 @@ def test_mixed_transform_modes_ensemble(orc, dfk, rows_ip):
     print(f"case mixed transform modes, rows in place {rows_ip}", flush=True)
     # every Nphi factors into 2/3/4/8 (in-place rows are a handle-wide choice);
     # 2 x 40 = 80 has a factor 5, so that disk keeps the cooperative columns
-    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 24, 0.04), (40, 128, 0.01), (24, 24, 0.03)]
-    opts = [(1, 1, 0, 2), (1, 0, 1, 3), (0, 0, 2, 2), (1, 0, 2, 1), (1, 1, 2, 4)]   # tw rows, tw cols, mode, cb
+    shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 24, 0.04), (40, 128, 0.01), (24, 24, 0.03),
+              (96, 48, 0.02), (64, 32, 0.01)]
+    opts = [(1, 1, 0, 2), (1, 0, 1, 3), (0, 0, 2, 2), (1, 0, 2, 1), (1, 1, 2, 4),
+            (1, 0, 3, 1), (0, 0, 3, 2)]   # tw rows, tw cols, mode, cb
     params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
     ics, mDs = [], []
     for a, b, c in shapes:
