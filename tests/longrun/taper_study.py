@@ -7,7 +7,8 @@ For each tanh taper width w (Sigma ~ r^slope x tapers at 0.15 / 0.9, M_d = M*):
 
   1. Toomre Q.  K is scanned geometrically (as dfk_model_state does) and the
      largest min-Q reachable is recorded over the whole grid [Rin, Rout] and
-     over the disk body [0.2, 0.8]; whether min Q = 1.3 is reachable on each.
+     over the disk body [0.2, 0.8]; whether min Q = 1.3 (--qmin) is reachable
+     on each (for M_d / M* = --mass-ratio, default 1).
   2. Dynamics.  The oracle runs the model (K from Q_min = 1.3 over the whole
      grid where reachable, else over the body) for `tlim` time units (20 outer
      orbits: t = 89) at CFL 0.4, eps 0.01, sampling |C2|/|C0| and arg C2 every
@@ -38,10 +39,11 @@ import diskfft_amd as dfk  # noqa: E402
 import oracle  # noqa: E402
 
 
-def q_scan(w, Nu, Nphi, eps, k, q_in, q_out):
+def q_scan(w, Nu, Nphi, eps, k, q_in, q_out, mass_ratio):
     """Largest min Q over [q_in, q_out] reachable by any K (gravity of the
     axisymmetric disk from the oracle's potential)."""
-    m = dfk.model_defaults(dfk.MODEL_POWERLAW, taper_w=w, q_in=q_in, q_out=q_out, Qmin=1e9)
+    m = dfk.model_defaults(dfk.MODEL_POWERLAW, taper_w=w, q_in=q_in, q_out=q_out, Qmin=1e9,
+                           mass_ratio=mass_ratio)
     dfk.model_ic(m, Nu, Nphi, eps, potential_forces=lambda x: k.forces(k.potential(x)))
     return float(m.Qmin_got), float(m.K)
 
@@ -52,15 +54,15 @@ def m2_profile(Sig, sig0):
     return np.abs(F[:, 2]) / (0.5 * Sig.shape[1] * np.maximum(sig0, 1e-300))
 
 
-def run_width(w, Nu, Nphi, eps, tlim, whole):
+def run_width(w, Nu, Nphi, eps, tlim, whole, qmin=1.3, mass_ratio=1.0):
     k = oracle.Kalnajs(Nu, Nphi, 0.1, 1.0, eps, 1.0, fast=True)
-    out = {"taper_w": w, "Nu": Nu, "Nphi": Nphi, "eps": eps}
-    out["q_whole_max"], out["K_whole_best"] = q_scan(w, Nu, Nphi, eps, k, 0.0, 10.0)
-    out["q_body_max"], out["K_body_best"] = q_scan(w, Nu, Nphi, eps, k, 0.2, 0.8)
-    out["q13_whole_reachable"] = out["q_whole_max"] >= 1.3
-    out["q13_body_reachable"] = out["q_body_max"] >= 1.3
+    out = {"taper_w": w, "Nu": Nu, "Nphi": Nphi, "eps": eps, "Qmin": qmin, "mass_ratio": mass_ratio}
+    out["q_whole_max"], out["K_whole_best"] = q_scan(w, Nu, Nphi, eps, k, 0.0, 10.0, mass_ratio)
+    out["q_body_max"], out["K_body_best"] = q_scan(w, Nu, Nphi, eps, k, 0.2, 0.8, mass_ratio)
+    out["q13_whole_reachable"] = out["q_whole_max"] >= qmin
+    out["q13_body_reachable"] = out["q_body_max"] >= qmin
     use_whole = whole and out["q13_whole_reachable"]
-    m = dfk.model_defaults(dfk.MODEL_POWERLAW, taper_w=w,
+    m = dfk.model_defaults(dfk.MODEL_POWERLAW, taper_w=w, Qmin=qmin, mass_ratio=mass_ratio,
                            q_in=0.0 if use_whole else 0.2, q_out=10.0 if use_whole else 0.8)
     p, S, U, W, mD, Q = dfk.model_ic(m, Nu, Nphi, eps,
                                      potential_forces=lambda x: k.forces(k.potential(x)))
@@ -136,11 +138,14 @@ def main():
     ap.add_argument("--tlim", type=float, default=89.0)
     ap.add_argument("--body-q", action="store_true",
                     help="always take Q_min over the body [0.2, 0.8]")
+    ap.add_argument("--qmin", type=float, default=1.3)
+    ap.add_argument("--mass-ratio", type=float, default=1.0)
     args = ap.parse_args()
     oracle.set_threads(max(1, min(16, os.cpu_count() or 1)))
     res = []
     for w in [float(x) for x in args.widths.split(",")]:
-        res.append(run_width(w, args.nu, args.nphi, args.eps, args.tlim, not args.body_q))
+        res.append(run_width(w, args.nu, args.nphi, args.eps, args.tlim, not args.body_q,
+                             args.qmin, args.mass_ratio))
         with open(args.out, "w") as f:
             json.dump(res, f)
 

@@ -98,3 +98,25 @@ def test_unreachable_q_reports_best(orc, dfk):
     m = dfk.model_defaults(dfk.MODEL_POWERLAW, Qmin=50.0)
     dfk.model_ic(m, Nu, Nphi, eps, potential_forces=_oracle_grav(orc, m, Nu, Nphi, eps))
     assert 0 < m.Qmin_got < 50.0 and m.K > 0
+
+
+@pytest.mark.parametrize("w,whole_ok,body_ok", [(0.02, False, True), (0.05, False, False),
+                                                (0.1, False, False), (0.2, True, True)])
+def test_taper_width_and_q_reachability(orc, dfk, w, whole_ok, body_ok):
+    """The taper study's first finding (tests/longrun/taper_study.py; 128 x 128,
+    M_d = M*, eps 0.01): min Toomre Q = 1.3 over the whole grid is reachable by
+    some K only for the smoothest taper, w = 0.2 (where Sigma at both grid
+    edges is still 38% of the power law); over the disk body [0.2, 0.8] only
+    for the sharp default w = 0.02 (pressure at sharp tapers drives kappa^2
+    negative before min Q reaches 1.3, at wider ones the tapers reach into the
+    body).  Recomputed here with the host model code and the oracle's gravity."""
+    Nu, Nphi, eps = 128, 128, 0.01
+    got = {}
+    for name, q_in, q_out in (("whole", 0.0, 10.0), ("body", 0.2, 0.8)):
+        m = dfk.model_defaults(dfk.MODEL_POWERLAW, taper_w=w, q_in=q_in, q_out=q_out, Qmin=1e9)
+        dfk.model_ic(m, Nu, Nphi, eps, potential_forces=_oracle_grav(orc, m, Nu, Nphi, eps))
+        got[name] = m.Qmin_got
+    print(f"w={w}: largest min Q whole grid {got['whole']:.3f}, body {got['body']:.3f}")
+    assert (got["whole"] >= 1.3) == whole_ok
+    assert (got["body"] >= 1.3) == body_ok
+    assert dfk.model_defaults(dfk.MODEL_POWERLAW).taper_w == 0.02
