@@ -22,7 +22,8 @@
 // k_control is the loop control as a launch of its own (handles without
 // self-gravity, profiling).
 // All FFTs are this file's own LDS Stockham transforms (radix 8, 4, 2, 3
-// unrolled; 5, 7, 11, 13 by a direct small DFT).
+// unrolled; any other prime by a direct DFT stage), plus the column
+// convolution's two-pass register transform (dfk_regfft.h).
 #include <hip/hip_runtime.h>
 
 #include "dfk_internal.h"
@@ -202,7 +203,8 @@ __device__ __forceinline__ void fft_stage(const double2 *__restrict__ a, double2
 }
 
 // Stage of any other prime radix R: direct R-point DFT reading its inputs
-// from LDS (few registers; used only for lengths with factors 5, 7, 11, 13).
+// from LDS (few registers; R^2 products per R points, so a large prime
+// factor makes its transform slow but not wrong).
 template <int SIGN>
 __device__ __forceinline__ void fft_stage_generic(const double2 *__restrict__ a, double2 *__restrict__ b, int n,
                                   int batch, int Ns, int R, const double2 *__restrict__ tw) {
@@ -1241,8 +1243,8 @@ k_khat_cols(const DiskDev *__restrict__ D_, const double2 *__restrict__ tw,
   const DiskDev &D = *D_;
   const int M = D.M, Nu = D.Nu, Nc = D.Nc, cb = D.cb;
   const int k0 = blockIdx.x * cb;
-  double2 *A = lds2, *B = lds2 + cb * M, *TW = lds2 + 2 * cb * M;
-  stage_twiddles(TW, tw + D.two_col, M);
+  double2 *A = lds2, *B = lds2 + cb * M;
+  const double2 *TW = tw_table(lds2 + 2 * cb * M, tw + D.two_col, M, D.tw_lds & 2);
   for (int e = threadIdx.x; e < cb * M; e += blockDim.x) {
     const int c = e / M, p = e - c * M;
     const int k = k0 + c;

@@ -54,20 +54,28 @@ void set_err(const char *fmt, ...) {
     if (r_ != DFK_OK) return r_; \
   } while (0)
 
-// Radix factorisation used by both transforms (8s first, then 4, 2, 3, 5, 7, 11, 13).
+// Radix factorisation used by both transforms: 8s first, then 4, 2, and the
+// odd primes in increasing order (3 unrolled in the kernels, every other
+// prime by their direct-DFT stage).
 bool factor(int n, int *rad, int *nst) {
   int m = n, k = 0;
-  while (m % 8 == 0) { rad[k++] = 8; m /= 8; }
-  while (m % 4 == 0) { rad[k++] = 4; m /= 4; }
-  while (m % 2 == 0) { rad[k++] = 2; m /= 2; }
-  for (int q : {3, 5, 7, 11, 13})
+  auto take = [&](int q) {
     while (m % q == 0) {
       if (k >= DFK_MAX_STAGES) return false;
       rad[k++] = q;
       m /= q;
     }
+    return true;
+  };
+  if (!take(8) || !take(4) || !take(2)) return false;
+  for (int q = 3; q * q <= m; q += 2)
+    if (!take(q)) return false;
+  if (m > 1) {   // the prime left over
+    if (k >= DFK_MAX_STAGES) return false;
+    rad[k++] = m;
+  }
   *nst = k;
-  return m == 1 && k <= DFK_MAX_STAGES;
+  return true;
 }
 
 int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -89,6 +97,9 @@ int ip_nmax(int t) { return std::max(8 * t, 1024); }
 // work list so that XCD x receives one contiguous chunk of it: neighbouring
 // tiles (which share halo rows) then meet in the same L2.
 const int kXcds = 8;
+// Largest grid: a transform of 4096 points and its ping-pong buffer take
+// 128 KiB of LDS (the twiddles are then read through the caches).
+const int kMaxNu = 2048, kMaxNphi = 4096;
 // Largest max|Im K-hat| / max|Re K-hat| accepted at create (round-off of the
 // transform of a real-even table is ~1e-16 x log2 of its size).
 const double kKhatImagTol = 1e-13;
@@ -204,7 +215,8 @@ int build_layout(dfk_handle *h) {
   const size_t kMaxLds = h->lds_max;
   h->rows_ip = h->row_inplace;
   for (const DiskDev &D : h->hD)
-    if (D.selfgrav && !inplace_radices(D.rad_row, D.nst_row, 1)) h->rows_ip = 0;
+    if (D.selfgrav && (!inplace_radices(D.rad_row, D.nst_row, 1) || D.Nphi > ip_nmax(h->thr_rows)))
+      h->rows_ip = 0;
   // fused row kernels: tiles of ti x Nphi with ti even (whole row pairs)
   h->fused = h->fused_req > 0 && h->upd_kind == 0;
   for (const DiskDev &D : h->hD) {
@@ -730,6 +742,7 @@ int build_khat(dfk_handle *h, int d, std::vector<double2> *full) {
   // batch shapes for the M-row table: same pr / cb rules as the loop
   D.pr = std::max(1, std::min(1024 / std::max(D.Nphi, 1), ceil_div(D.M, 2)));
   D.cb = std::max(1, std::min(1024 / std::max(D.M, 1), D.Nc));
+  D.tw_lds = 0;   // twiddles through the caches: the largest transforms fill the LDS
   std::vector<double> kt;
   ktable(D, kt);
   double *dk = nullptr;
@@ -743,8 +756,8 @@ int build_khat(dfk_handle *h, int d, std::vector<double2> *full) {
   if (e == hipSuccess) e = hipMemcpy(dd, &D, sizeof D, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = dfk_launch_khat(h->stream, dd, ceil_div(D.M, 2 * D.pr), ceil_div(D.Nc, D.cb),
-                        (size_t)(2 * D.pr + 1) * D.Nphi * sizeof(double2),
-                        (size_t)(2 * D.cb + 1) * D.M * sizeof(double2), dk, h->tw, tmp, h->khat);
+                        (size_t)(2 * D.pr) * D.Nphi * sizeof(double2),
+                        (size_t)(2 * D.cb) * D.M * sizeof(double2), dk, h->tw, tmp, h->khat);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   if (e == hipSuccess && full) {
     full->resize((size_t)D.Nc * D.M);
@@ -780,9 +793,11 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
     const dfk_disk_params &p = params[d];
     if (p.Nu < 3 || p.Nphi < 4) { set_err("disk %d: need Nu >= 3 and Nphi >= 4 (got %d x %d)", d, p.Nu, p.Nphi); return DFK_EINVAL; }
     if (!(p.Rin > 0) || !(p.Rout > p.Rin)) { set_err("disk %d: need 0 < Rin < Rout", d); return DFK_EINVAL; }
-    if (p.Nu > 512 || p.Nphi > 1024) {
-      // one row (Nphi) or one padded column (2 Nu), ping-pong + twiddles, must fit 64 KB of LDS
-      set_err("disk %d: need Nu <= 512 and Nphi <= 1024 (got %d x %d)", d, p.Nu, p.Nphi);
+    if (p.Nu > kMaxNu || p.Nphi > kMaxNphi) {
+      // one row (Nphi) or one padded column (2 Nu) and its ping-pong buffer
+      // must fit a workgroup's LDS (160 KiB on gfx950)
+      set_err("disk %d: need Nu <= %d and Nphi <= %d (got %d x %d)", d, kMaxNu, kMaxNphi, p.Nu,
+              p.Nphi);
       return DFK_EINVAL;
     }
   }
@@ -837,8 +852,8 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
     D.dphi = 2.0 * M_PI / p.Nphi;
     D.inv2dphi = 1.0 / (2.0 * D.dphi);
     if (D.selfgrav && (!factor(D.Nphi, D.rad_row, &D.nst_row) || !factor(D.M, D.rad_col, &D.nst_col))) {
-      set_err("disk %d: transform lengths %d and %d must factor into 2, 3, 5, 7, 11, 13", d,
-              D.Nphi, D.M);
+      set_err("disk %d: transform lengths %d and %d have more than %d factors", d, D.Nphi, D.M,
+              DFK_MAX_STAGES);
       return fail(DFK_EUNSUPPORTED);
     }
     for (int n : {D.Nphi, D.M}) {
@@ -867,7 +882,11 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
   const bool small = h->ntot < (1LL << 19);
   for (int d = 0; d < ndisks; ++d) {
     DiskDev &D = h->hD[d];
-    D.tw_lds = 3;   // twiddles of row and column transforms staged in LDS
+    // twiddles of the row and column transforms staged in LDS, where a
+    // transform, its ping-pong buffer and the table fit
+    D.tw_lds = 0;
+    if ((size_t)3 * D.Nphi * sizeof(double2) <= h->lds_max) D.tw_lds |= 1;
+    if ((size_t)3 * D.M * sizeof(double2) <= h->lds_max) D.tw_lds |= 2;
     D.conv_wave = 0;
     if (small) {
       D.ti = 8; D.tj = 16; D.pr = 1; D.cb = 1;
@@ -1374,7 +1393,10 @@ int dfk_set_shapes(dfk_handle *h, int d, int ti, int tj, int pr, int cb) {
 int dfk_set_fft_options(dfk_handle *h, int d, int rows_tw_lds, int cols_tw_lds, int cols_wave) {
   TRY(check_disk(h, d));
   DiskDev &D = h->hD[d];
-  const int v = (rows_tw_lds ? 1 : 0) | (cols_tw_lds ? 2 : 0);
+  // twiddles in LDS only where a transform, its ping-pong buffer and the table fit
+  const bool rfit = (size_t)3 * D.Nphi * sizeof(double2) <= h->lds_max;
+  const bool cfit = (size_t)3 * D.M * sizeof(double2) <= h->lds_max;
+  const int v = (rows_tw_lds && rfit ? 1 : 0) | (cols_tw_lds && cfit ? 2 : 0);
   // the in-place column transforms need radices 2/3/4/8, >= 2 stages, M <= 1024;
   // the two-pass register one a transform length it is built for
   const bool fast = D.M <= 1024 && inplace_radices(D.rad_col, D.nst_col, 2);

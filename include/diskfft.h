@@ -25,13 +25,14 @@ extern "C" {
 #define DFK_EINVAL 1        /* bad argument (sizes, null pointers, disk index) */
 #define DFK_EHIP 2          /* HIP runtime error or no device                  */
 #define DFK_ENOMEM 3        /* device allocation failed                        */
-#define DFK_EUNSUPPORTED 4  /* transform length with a prime factor > 13       */
+#define DFK_EUNSUPPORTED 4  /* transform length with more than 16 prime factors */
 #define DFK_ESTATE 5        /* call out of order (e.g. advance before run_init)*/
 
 /* Constructor arguments of DiskHydro (hydro.hpp:28-29) and KalnajsSolver
  * (kalnajs.hpp:25-26) for one disk. */
 typedef struct {
-  int Nu, Nphi;          /* radial (log-spaced) and azimuthal zones; 3 <= Nu <= 512, 4 <= Nphi <= 1024 */
+  int Nu, Nphi;          /* radial (log-spaced) and azimuthal zones; 3 <= Nu <= 2048, 4 <= Nphi <= 4096,
+                            any factorisation (large prime factors make the transforms slow) */
   double Rin, Rout;      /* grid radii */
   double K, gamma;       /* barotropic EOS P = K Sigma^gamma */
   double gmstar;         /* G M_star */
@@ -171,6 +172,8 @@ int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
  *       convolution read their twiddle table: 1 = staged in LDS per
  *       workgroup (default; fastest when few workgroups run), 0 = from HBM
  *       through L1/L2 (less LDS per workgroup, more workgroups per CU);
+ *       transforms too long for a table beside their buffers in LDS
+ *       (Nphi or 2Nu above 3413 on gfx950) always read it through the caches;
  *   cols_wave  how the column convolution is spread over threads:
  *       0 = the workgroup cooperates on its columns with two LDS buffers per
  *       column (default); 1 = each column is transformed by one wavefront,

@@ -17,12 +17,14 @@ ORC_BIN = os.path.join(ROOT, "oracle", "oracle_diskdisk")
 def _run(exe, args, cwd):
     r = subprocess.run([exe] + args, cwd=cwd, capture_output=True, text=True, timeout=240)
     assert r.returncode == 0, r.stderr
-    return r.stdout.splitlines(), np.loadtxt(os.path.join(cwd, "diskfft_modes.dat"))
+    return r.stdout.splitlines(), np.loadtxt(os.path.join(cwd, "diskfft_modes.dat"), ndmin=2)
 
 
 @pytest.mark.parametrize("args", [["64", "64", "0.02", "1.0", "1", "1e-3", "4"],
                                   ["48", "36", "0.01", "0.6", "0", "2e-3"],
-                                  ["96", "80", "0.005", "0.8"]])
+                                  ["96", "80", "0.005", "0.8"],
+                                  ["64", "34", "0.02", "0.3"],          # 34 = 2 x 17
+                                  ["1024", "1024", "0.005", "0.0004"]])
 def test_diskdisk_matches_reference_driver(tmp_path, args):
     print("case diskdisk", " ".join(args), flush=True)
     a, b = tmp_path / "gpu", tmp_path / "orc"

@@ -17,6 +17,18 @@ LOOP_TOL = 1e-10      # loop tolerance, Sigma/vR/vphi after 10 steps
 OP_TOL = 1e-12        # single operator (one transform, one stage, one step)
 
 
+def state_errs(got, ref):
+    """Relative errors of (Sigma, vR, vphi): Sigma against its own maximum,
+    each velocity component against the speed scale max(|vR|, |vphi|) of the
+    reference state (vR starts at zero in these disks: after ten steps of a
+    fine grid its own maximum is ~1e-7 of the speed, and a per-component
+    scale would measure round-off of the speed against it)."""
+    vscale = max(np.max(np.abs(ref[1])), np.max(np.abs(ref[2])))
+    return [relerr(got[0], ref[0]),
+            float(np.max(np.abs(got[1] - ref[1])) / vscale),
+            float(np.max(np.abs(got[2] - ref[2])) / vscale)]
+
+
 @pytest.fixture(scope="module")
 def dfk():
     import diskfft_amd
@@ -144,6 +156,22 @@ def test_potential_register_columns(orc, dfk, Nu, Nphi, ok):
     assert relerr(p_reg, k.potential(Sig)) < 1e-13
 
 
+@pytest.mark.parametrize("Nu,Nphi", [(16, 34), (20, 38), (17, 46), (12, 1031), (1024, 1024),
+                                     (64, 4096), (2048, 64)])
+def test_potential_any_size(orc, dfk, Nu, Nphi):
+    """Transform lengths with prime factors above 13 (34 = 2 x 17, 38 = 2 x
+    19, 2Nu = 34, 46 = 2 x 23, 1031 prime: the direct-DFT stage) and the
+    largest grids (2Nu up to 4096, Nphi up to 4096, twiddles read through the
+    caches) against the oracle, whose FFT takes any length."""
+    print(f"case potential {Nu}x{Nphi} (any size)", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    rng = np.random.default_rng(Nu * 7 + Nphi)
+    Sig = rng.random((Nu, Nphi))
+    assert relerr(e.potential(Sig), k.potential(Sig)) < 1e-13
+    Kg = e.khat()
+    assert np.max(np.abs(Kg.imag)) / np.max(np.abs(Kg.real)) < 1e-13
+
+
 @pytest.mark.parametrize("Nu,Nphi", [(16, 16), (35, 40), (128, 128), (33, 26)])
 def test_forces_match_oracle(orc, dfk, Nu, Nphi):
     print(f"case forces {Nu}x{Nphi}", flush=True)
@@ -247,7 +275,8 @@ def _lka98(orc, Nu, Nphi, eps, amp=1e-3, selfgrav=1):
 @pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("Nu,Nphi,eps", [(64, 64, 0.02), (128, 128, 0.02), (96, 80, 0.01),
                                          (256, 256, 0.005), (48, 36, 0.02), (512, 1024, 0.01),
-                                         (3, 4, 0.02)])
+                                         (3, 4, 0.02), (64, 34, 0.02), (1024, 1024, 0.01),
+                                         (2048, 128, 0.01)])
 def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps, kind):
     """The loop gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative."""
     print(f"case 10-step loop {Nu}x{Nphi} eps={eps} update kernel {kind}", flush=True)
@@ -262,8 +291,8 @@ def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps, kind):
     st = e.status()
     assert st["done"] and st["steps"] == 10 and steps == 10
     assert abs(st["t"] - tf) / tf < 1e-13
-    errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
-    print("  rel errors Sigma, vR, vphi:", errs, flush=True)
+    errs = state_errs(e.get_state(), h.state())
+    print("  rel errors Sigma, vR, vphi (velocities vs the speed scale):", errs, flush=True)
     assert max(errs) <= LOOP_TOL
     sg = e.samples()
     assert sg.shape == so.shape
@@ -693,9 +722,10 @@ def test_error_paths(dfk):
     with pytest.raises(dfk.DiskfftError) as ei:
         dfk.Ensemble(dfk.disk_params(2, 16))
     assert ei.value.code == dfk.DFK_EINVAL
-    with pytest.raises(dfk.DiskfftError) as ei:
-        dfk.Ensemble(dfk.disk_params(16, 34))      # 34 = 2 * 17
-    assert ei.value.code == dfk.DFK_EUNSUPPORTED
+    for shape in ((4096, 16), (16, 8192)):          # beyond 2048 x 4096
+        with pytest.raises(dfk.DiskfftError) as ei:
+            dfk.Ensemble(dfk.disk_params(*shape))
+        assert ei.value.code == dfk.DFK_EINVAL
     e = dfk.Ensemble(dfk.disk_params(16, 16))
     with pytest.raises(dfk.DiskfftError) as ei:
         e.advance(1)
