@@ -49,8 +49,14 @@ def tlim_for(d):
 
 def run_shard(first, count, Nu=96, Nphi=96, seed=2026, total=4352, chunk=2000,
               max_wall=None, verbose=True, dtout=0.01, lo_factor=10.0, hi_factor=100.0,
-              sat_frac=0.5, peak_window=2.0, keep_first=600, keep_every=25, noise_seed=42):
-    disks = ensemble_cfg.draw(total, seed=seed, grids=[(Nu, Nphi)])[first:first + count]
+              sat_frac=0.5, peak_window=2.0, keep_first=600, keep_every=25, noise_seed=42,
+              tlim=None, indices=None):
+    """Runs first .. first+count-1 of the draw on one Nu x Nphi handle, each to
+    its 20 outer orbits (or to `tlim`: gamma, Omega_p and sat_peak are fixed
+    by t ~ 6, sat_late needs the whole run).  indices: the runs of the draw to
+    make instead of first .. first+count-1."""
+    alld = ensemble_cfg.draw(total, seed=seed, grids=[(Nu, Nphi)])
+    disks = [alld[i] for i in indices] if indices is not None else alld[first:first + count]
     if noise_seed is not None:
         for d in disks:
             d["seed"] = noise_seed
@@ -74,7 +80,7 @@ def run_shard(first, count, Nu=96, Nphi=96, seed=2026, total=4352, chunk=2000,
         Qgot.append(m.Qmin_got)
     autotune.apply(ens, regime="throughput")
     t_setup = time.perf_counter() - t0
-    tl = [tlim_for(d) for d in disks]
+    tl = [tlim_for(d) if tlim is None else tlim for d in disks]
     ens.set_graph_steps(200)
     ens.run_init(tl, dtout=dtout, cfl=0.4, mD=mDs, stop_on_nonfinite=1, sample_cap=512)
     series = [[] for _ in disks]
@@ -118,7 +124,8 @@ def run_shard(first, count, Nu=96, Nphi=96, seed=2026, total=4352, chunk=2000,
                 K=np.array(Ks),
                 Qgot=np.array(Qgot), steps=steps, flags=flags, tlim=np.array(tl),
                 series=ser, wall=wall, setup=t_setup, zone_updates=zones,
-                first=first, grid=np.array([Nu, Nphi]))
+                first=first, grid=np.array([Nu, Nphi]),
+                index=np.array(indices if indices is not None else range(first, first + count)))
 
 
 def main():
@@ -131,10 +138,18 @@ def main():
     ap.add_argument("--max-wall", type=float, default=None)
     ap.add_argument("--noise-seed", type=int, default=42,
                     help="noise realization of every run (negative: each run's own seed)")
+    ap.add_argument("--tlim", type=float, default=None,
+                    help="end every run at this time instead of after 20 outer orbits")
+    ap.add_argument("--indices", default=None,
+                    help="file with the draw indices of the runs to make (one per line)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
+    idx = None
+    if a.indices:
+        idx = [int(x) for x in open(a.indices).read().split()]
     r = run_shard(a.first, a.count, a.nu, a.nphi, total=a.total, max_wall=a.max_wall,
-                  noise_seed=None if a.noise_seed < 0 else a.noise_seed)
+                  noise_seed=None if a.noise_seed < 0 else a.noise_seed, tlim=a.tlim,
+                  indices=idx)
     np.savez_compressed(a.out, **r)
     print(f"shard {a.first}+{a.count}: {r['ok'].sum()} good runs, {r['zone_updates']:.3e} "
           f"zone-updates in {r['wall']:.1f}s ({r['zone_updates'] / r['wall']:.3e}/s), setup "

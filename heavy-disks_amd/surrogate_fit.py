@@ -77,7 +77,7 @@ def main():
     ap.add_argument("shards", nargs="+")
     ap.add_argument("--out", required=True)
     ap.add_argument("--n-train", type=int, default=4096)
-    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--epochs", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--members", type=int, default=5)
