@@ -134,13 +134,15 @@ def cpu_baseline(p, S, U, W, cfl, seconds, threads):
                        f"({dt:.1f} s), oracle built -O3 -march=x86-64-v3, OpenMP {threads} threads")
 
 
-def ensemble_legs(dfk, steps_batch, steps_seq, chains=0, cols_mode=None):
+def ensemble_legs(dfk, steps_batch, steps_seq, chains=0, cols_mode=None, rows_mode=None):
     import ensemble_cfg
     disks = ensemble_cfg.draw(64, seed=2026)
     ens, ics = ensemble_cfg.build(dfk, disks)
     ntuned = autotune.apply(ens, regime="throughput")
     if chains:
         ens.set_chains(chains)
+    if rows_mode is not None:   # row-transform mode of the handle, over the tuned one
+        ens.set_row_transform(rows_mode)
     if cols_mode is not None:   # column-convolution mode of every disk, over the tuned one
         for d in range(64):
             r, c, _ = ens.get_fft_options(d)
@@ -208,6 +210,8 @@ def main():
                     help="only the batched 64-disk leg (no single disk, no sequential leg)")
     ap.add_argument("--chains", type=int, default=0,
                     help="graph chains of the ensemble handle (0: the library default)")
+    ap.add_argument("--rows-mode", type=int, default=None,
+                    help="row-transform mode of the ensemble handle (dfk_set_row_transform)")
     ap.add_argument("--cols-mode", type=int, default=None,
                     help="column-convolution mode of every ensemble disk (dfk_set_fft_options)")
     args = ap.parse_args()
@@ -221,7 +225,7 @@ def main():
         sys.exit(1)
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_legs(dfk, args.ensemble_steps, 0, args.chains,
-                                                     args.cols_mode)}), flush=True)
+                                                     args.cols_mode, args.rows_mode)}), flush=True)
         return
     Nu, Nphi = args.nu, args.nphi
     r = single_disk(dfk, Nu, Nphi, args.eps, args.cfl, args.steps, args.warmup, args.chunk)

@@ -142,12 +142,13 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     # (1) or read through the caches (0).  A launch's LDS is the largest any
     # of its disks needs, so the knob only pays where the big transforms use
     # it; for the rows it lost in mixed ensembles and is not tuned.
-    # ... and the row transforms cooperative (0) or in place (1)
-    for rip in (0, 1):
+    # ... and the row transforms cooperative (0), in place (1) or by the
+    # two-pass register kernels (2, whose row pairs per workgroup are fixed)
+    for rip in (0, 1, 2):
         e.set_row_transform(rip)
-        if rip and not e.get_row_transform():
-            break                      # this Nphi has no in-place row transform
-        for pr in prs:
+        if e.get_row_transform() != rip:
+            continue                   # this Nphi has no such row transform
+        for pr in (prs if rip < 2 else [base[2]]):
             ms = prof(base[0], base[1], pr, base[3])
             res["pr"][f"{pr}:{rip}"] = float(ms[1] + ms[3])
     e.set_row_transform(0)

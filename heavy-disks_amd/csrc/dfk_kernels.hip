@@ -637,15 +637,32 @@ __device__ void control_disk(const DiskDev &D, DiskCtl &C, int d, const double *
                              const double *__restrict__ pcm, DfkSampleDev *__restrict__ samples,
                              int cap, int par) {
   __shared__ double red[5 * 16];
-  const int nt = D.ntile_i * D.ntile_j;
+  const int nt = D.ntile_i * D.ntile_j, B = blockDim.x;
   double mx = 1e-30, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-    double v = pcfl[D.part + t];
-    mx = (v > mx) ? v : mx;
-    s0 += pcm[4 * (D.part + t) + 0];
-    s1 += pcm[4 * (D.part + t) + 1];
-    s2 += pcm[4 * (D.part + t) + 2];
-    s3 += pcm[4 * (D.part + t) + 3];
+  // thread x takes tiles x, x + B, x + 2B, ... in that order; the loads of 8
+  // of them are issued at once (clamped indices, the values past the end
+  // dropped), so a workgroup of few threads (64 in the register row launch)
+  // does not pay one memory latency per tile
+  constexpr int KF = 8;
+  for (int t0 = threadIdx.x; t0 < nt; t0 += KF * B) {
+    double v[KF];
+    double4 c[KF];
+#pragma unroll
+    for (int q = 0; q < KF; ++q) {
+      const i64 t = D.part + min(t0 + q * B, nt - 1);
+      v[q] = pcfl[t];
+      c[q] = *reinterpret_cast<const double4 *>(pcm + 4 * t);
+    }
+#pragma unroll
+    for (int q = 0; q < KF; ++q) {
+      if (t0 + q * B < nt) {
+        mx = (v[q] > mx) ? v[q] : mx;
+        s0 += c[q].x;
+        s1 += c[q].y;
+        s2 += c[q].z;
+        s3 += c[q].w;
+      }
+    }
   }
   wg_reduce5(mx, s0, s1, s2, s3, red);
   if (threadIdx.x != 0) return;
@@ -1200,6 +1217,149 @@ __global__ void __launch_bounds__(WG) k_rows_c2r(C2R_PARAMS) { rows_c2r_body<0>(
 
 template <int TG, bool BATCH>
 __global__ void __launch_bounds__(TG) k_rows_c2r_group(C2R_PARAMS) { rows_c2r_body<TG, BATCH>(C2R_ARGS); }
+
+// Row transforms by the two-pass register transform of dfk_regfft.h (the
+// handles whose runtime set the rows to mode 2): n = Nphi = N1 N2 fixed at
+// compile time, one 64-lane wavefront per workgroup, N2 lanes per packed row
+// pair (rows 2p, 2p + 1 as real and imaginary parts; 64 / N2 pairs per
+// workgroup), every synchronisation a wavefront one.  Pass 1 reads (r2c) or
+// writes (c2r) the pair's rows with consecutive lanes on consecutive zones.
+// Pairs past the disk's last one recompute that pair and store nothing; the
+// odd row of a disk with odd Nu reads as zero.
+struct RegPair {
+  int ra, rb;        // the pair's rows (rb clamped to ra when it is past the disk)
+  bool live, hasb;   // the pair is the disk's; its second row is
+};
+template <int N2>
+__device__ __forceinline__ RegPair reg_pair(int grp, int Nu) {
+  constexpr int CPW = 64 / N2;
+  RegPair r;
+  const int p = grp * CPW + (int)threadIdx.x / N2;
+  r.live = 2 * p < Nu;
+  r.ra = r.live ? 2 * p : 2 * ((Nu - 1) >> 1);
+  r.hasb = r.ra + 1 < Nu;
+  r.rb = r.hasb ? r.ra + 1 : r.ra;
+  return r;
+}
+
+template <int N1, int N2>
+__device__ __forceinline__ void r2c_reg_body(const DiskDev &D, int grp, double2 *lds2,
+                                             const double *__restrict__ S, const double *__restrict__ R32,
+                                             const double2 *__restrict__ tw, double2 *__restrict__ out) {
+  constexpr int n = N1 * N2, LE = rfft::rows::lds_elems<N1, N2>();
+  const int Nu = D.Nu, t = threadIdx.x % N2;
+  const RegPair q = reg_pair<N2>(grp, Nu);
+  double2 *L = lds2 + (threadIdx.x / N2) * LE;
+  const double pa = R32[q.ra], pb = R32[q.rb];
+  const double *Sa = S + (i64)q.ra * n, *Sb = S + (i64)q.rb * n;
+  double2 v[N1];
+  rfft::rows::pass1<N1, N2, -1>(t, [&](int j) {
+    const double b = pb * Sb[j];
+    return make_double2(pa * Sa[j], q.hasb ? b : 0.0);
+  }, tw, v);
+  rfft::conv::fwd_write<N1, N2, 1>(t, 0, v, L);
+  wave_lds_sync();
+  double2 x[N2];
+  if (t < N1) rfft::conv::fwd_read_c<N1, N2, 1, 0>(t, L, x);
+  wave_lds_sync();
+  if (t < N1) {
+    rfft::dft<N2, -1>(x);
+    rfft::rows::spill<N1, N2>(t, x, L);
+  }
+  wave_lds_sync();
+  if (t < N1 && q.live)
+    rfft::rows::split<N1, N2>(t, x, L, [&](int k, double2 xa, double2 xb) {
+      out[(i64)k * Nu + q.ra] = xa;
+      if (q.hasb) out[(i64)k * Nu + q.ra + 1] = xb;
+    });
+}
+
+template <int N1, int N2>
+__device__ __forceinline__ void c2r_reg_body(const DiskDev &D, int grp, double2 *lds2,
+                                             const double2 *__restrict__ in, const double *__restrict__ sc,
+                                             const double2 *__restrict__ tw, double *__restrict__ P) {
+  constexpr int n = N1 * N2, LE = rfft::rows::lds_elems<N1, N2>();
+  const int Nu = D.Nu, t = threadIdx.x % N2;
+  const RegPair q = reg_pair<N2>(grp, Nu);
+  double2 *L = lds2 + (threadIdx.x / N2) * LE;
+  double2 v[N1];
+  rfft::rows::pass1<N1, N2, +1>(t, [&](int m) {
+    return rfft::rows::c2r_in<N1, N2>(m, [&](int k, double2 &xa, double2 &xb) {
+      xa = in[(i64)k * Nu + q.ra];
+      const double2 b = in[(i64)k * Nu + q.rb];
+      xb = q.hasb ? b : make_double2(0.0, 0.0);
+    });
+  }, tw, v);
+  rfft::conv::fwd_write<N1, N2, 1>(t, 0, v, L);
+  wave_lds_sync();
+  double2 x[N2];
+  if (t < N1) rfft::conv::fwd_read_c<N1, N2, 1, 0>(t, L, x);
+  if (t < N1 && q.live) {
+    rfft::dft<N2, +1>(x);
+    const double sa = sc[q.ra], sb = sc[q.rb];
+    double *Pa = P + (i64)q.ra * n, *Pb = P + (i64)q.rb * n;
+    rfft::sfor<0, N2>([&](auto k2_) {
+      constexpr int k2 = decltype(k2_)::value;
+      Pa[t + N1 * k2] = x[k2].x * sa;
+      if (q.hasb) Pb[t + N1 * k2] = x[k2].y * sb;
+    });
+  }
+}
+
+// One launch for every disk of the handle, whatever its Nphi (NMAX = 512:
+// the builds for rows up to 512, whose registers do not pay for the longer
+// transforms; 1024: all lengths).
+template <int NMAX>
+__global__ void __launch_bounds__(64) k_rows_r2c_reg(R2C_PARAMS) {
+  extern __shared__ double2 lds2[];
+  if ((int)blockIdx.x >= nrow_blk) {
+    trailing_control(disks, ctl, ctl0 + blockIdx.x - nrow_blk, pcfl, pcm, samples, cap, par);
+    return;
+  }
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const double *S = stp(st, ntot, par, 0, D), *R32 = gtab(geom, D, GT_R32);
+  const double2 *TW = tw + D.two_row;
+  double2 *out = spec + D.spo;
+  switch (D.Nphi) {
+    case 128: r2c_reg_body<8, 16>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 192: r2c_reg_body<12, 16>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 256: r2c_reg_body<16, 16>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 384: r2c_reg_body<12, 32>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 512: r2c_reg_body<16, 32>(D, wm.y, lds2, S, R32, TW, out); break;
+    default:
+      if constexpr (NMAX > 512) {
+        if (D.Nphi == 768) r2c_reg_body<24, 32>(D, wm.y, lds2, S, R32, TW, out);
+        else if (D.Nphi == 1024) r2c_reg_body<32, 32>(D, wm.y, lds2, S, R32, TW, out);
+      }
+      break;   // the runtime sets mode 2 only when every Nphi has a case
+  }
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(64) k_rows_c2r_reg(C2R_PARAMS) {
+  extern __shared__ double2 lds2[];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const double *sc = gtab(geom, D, GT_PHISC);
+  const double2 *TW = tw + D.two_row, *in = spec + D.spo;
+  double *P = phi + D.so;
+  switch (D.Nphi) {
+    case 128: c2r_reg_body<8, 16>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 192: c2r_reg_body<12, 16>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 256: c2r_reg_body<16, 16>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 384: c2r_reg_body<12, 32>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 512: c2r_reg_body<16, 32>(D, wm.y, lds2, in, sc, TW, P); break;
+    default:
+      if constexpr (NMAX > 512) {
+        if (D.Nphi == 768) c2r_reg_body<24, 32>(D, wm.y, lds2, in, sc, TW, P);
+        else if (D.Nphi == 1024) c2r_reg_body<32, 32>(D, wm.y, lds2, in, sc, TW, P);
+      }
+      break;
+  }
+}
 
 // ------------------------------------------------------------------ K-hat setup
 // Row r2c of the M x Nphi kernel table (kalnajs.hpp:105-114), all M rows, into
@@ -2024,6 +2184,12 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           int nctl, int ctl0, const double *pcfl, const double *pcm,
                           DfkSampleDev *samples, int cap, int inplace) {
   if (nblk + nctl <= 0) return hipSuccess;
+  if (inplace >= 3) {
+    auto k = inplace == 3 ? k_rows_r2c_reg<512> : k_rows_r2c_reg<1024>;
+    hipLaunchKernelGGL(k, dim3(nblk + nctl), dim3(64), lds, s, disks, map, base, ctl, st, ntot, geom,
+                       tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap);
+    return hipGetLastError();
+  }
   if (inplace) {
     const int batched = inplace == 2;
     auto k = PICK_GROUP(k_rows_r2c_group);
@@ -2097,6 +2263,22 @@ size_t dfk_conv_reg_lds(int M) {
   return (size_t)(64 / n2) * (a > b ? a : b) * sizeof(double2);
 }
 
+// Row transforms by the register kernels: the same (N1, N2) splits as the
+// column transforms of those lengths.
+int dfk_rows_reg_pairs(int n) {
+  int n1, n2, h;
+  conv_reg_split(n, &n1, &n2, &h);
+  return n2 ? 64 / n2 : 0;
+}
+
+size_t dfk_rows_reg_lds(int n) {
+  int n1, n2, h;
+  conv_reg_split(n, &n1, &n2, &h);
+  if (!n2) return 0;
+  const int a = n1 * (n2 + 1), b = n1 * n2;
+  return (size_t)(64 / n2) * (a > b ? a : b) * sizeof(double2);
+}
+
 hipError_t dfk_launch_conv_reg(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
                                const int2 *map, int base, DiskCtl *ctl, const double *khat,
                                const double2 *tw, double2 *spec, const DfkConvCtl &c) {
@@ -2132,6 +2314,11 @@ hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
                           const double *geom, const double2 *tw, double *phi, int inplace) {
   if (nblk <= 0) return hipSuccess;
+  if (inplace >= 3) {
+    auto k = inplace == 3 ? k_rows_c2r_reg<512> : k_rows_c2r_reg<1024>;
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(64), lds, s, disks, map, base, ctl, spec, geom, tw, phi);
+    return hipGetLastError();
+  }
   if (inplace) {
     const int batched = inplace == 2;
     auto k = PICK_GROUP(k_rows_c2r_group);

@@ -15,7 +15,9 @@ hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const 
                            const double *geom, const double *trig, double *pcfl, double *pcm);
 // nctl > 0 appends one control workgroup per disk ctl0 .. ctl0+nctl-1 to the launch.
 // inplace: 0 the cooperative row transforms, 1 the in-place ones (k_rows_*_group) with
-// one complex row per workgroup (pr = 1 for every disk), 2 in place with batches.
+// one complex row per workgroup (pr = 1 for every disk), 2 in place with batches,
+// 3 the register two-pass kernels (k_rows_*_reg; every Nphi <= 512, 64 threads,
+// dfk_rows_reg_pairs row pairs per workgroup), 4 the same with Nphi up to 1024.
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, DiskCtl *ctl, const double *st,
                           long long ntot, const double *geom, const double2 *tw, double2 *spec,
@@ -45,6 +47,10 @@ hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, 
 // need among the launch's disks).
 int dfk_conv_reg_cols(int M);
 size_t dfk_conv_reg_lds(int M);
+// Register row transforms: row pairs per 64-thread workgroup for a row of n
+// points (0: n unsupported) and the dynamic LDS bytes of such a disk.
+int dfk_rows_reg_pairs(int n);
+size_t dfk_rows_reg_lds(int n);
 hipError_t dfk_launch_conv_reg(hipStream_t s, int nblk, size_t lds, const DiskDev *disks,
                                const int2 *map, int base, DiskCtl *ctl, const double *khat,
                                const double2 *tw, double2 *spec, const DfkConvCtl &c);

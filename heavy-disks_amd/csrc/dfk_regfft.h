@@ -1,5 +1,6 @@
-// In-register DFTs for the column convolution's two-pass transform
-// (k_cols_conv_reg): sizes 2, 3, 4, 8, 12, 16, 24, 32, forward (SIGN < 0,
+// In-register DFTs for the two-pass transforms of the column convolution
+// (k_cols_conv_reg) and the row transforms (k_rows_r2c_reg,
+// k_rows_c2r_reg): sizes 2, 3, 4, 8, 12, 16, 24, 32, forward (SIGN < 0,
 // exp(-2 pi i nk/N)) or inverse (SIGN > 0), unnormalised, natural order in
 // and out, fully unrolled at compile time (indices are template constants,
 // so the arrays stay in registers and twiddles of the form +-1, +-i are
@@ -358,6 +359,77 @@ RF_HD void pass3(int b, double2 (&z)[N1], St st) {
 }
 
 }  // namespace conv
+
+// Row transforms of length n = N1 N2 by the same two passes, N1 <= N2: pass 1
+// on lanes t < N2 (inputs x[N2 n1 + t]), the transpose of conv::fwd_write /
+// fwd_read_c with H = 1, pass 2 on lanes k1 < N1, which then hold bins
+// k1 + N1 k2.
+namespace rows {
+
+// LDS elements per transform: the transpose, or the spectrum in natural order
+template <int N1, int N2>
+constexpr int lds_elems() { return conv::cmax(N1 * (N2 + 1), N1 * N2); }
+
+// pass 1, lane t < N2: ld(j) returns x[j]; v[k1] = Y[k1] W_n^(SIGN t k1)
+// with Y the N1-point DFT of x[N2 n1 + t] over n1.
+template <int N1, int N2, int SIGN, class Ld>
+RF_HD void pass1(int t, Ld ld, const double2 *tw, double2 (&v)[N1]) {
+  sfor<0, N1>([&](auto n1_) {
+    constexpr int n1 = decltype(n1_)::value;
+    v[n1] = ld(N2 * n1 + t);
+  });
+  dft<N1, SIGN>(v);
+  const conv::TwRow<N1, SIGN> w(tw, t);
+  sfor<1, N1>([&](auto k1_) {
+    constexpr int k1 = decltype(k1_)::value;
+    v[k1] = w.template mul<k1>(v[k1]);
+  });
+}
+
+// r2c, lane k1 < N1 holding Z[k1 + N1 k2] of a packed row pair (rows a, b as
+// real and imaginary parts): the spectrum in natural order into L[0, n).
+template <int N1, int N2>
+RF_HD void spill(int k1, const double2 (&x)[N2], double2 *L) {
+  sfor<0, N2>([&](auto k2_) {
+    constexpr int k2 = decltype(k2_)::value;
+    L[k1 + N1 * k2] = x[k2];
+  });
+}
+
+// r2c, lane k1 < N1 after spill: the two rows' bins k = k1 + N1 k2 <= n/2,
+// Xa = (Z[k] + conj Z[n-k]) / 2, Xb = (Z[k] - conj Z[n-k]) / 2i, to st(k, Xa, Xb).
+template <int N1, int N2, class St>
+RF_HD void split(int k1, const double2 (&x)[N2], const double2 *L, St st) {
+  constexpr int n = N1 * N2;
+  sfor<0, N2 / 2 + 1>([&](auto k2_) {
+    constexpr int k2 = decltype(k2_)::value;
+    if (k2 < N2 / 2 || k1 == 0) {   // k = n/2 (k2 = N2/2) only on lane 0
+      const int k = k1 + N1 * k2;
+      const double2 z = x[k2];
+      double2 y = L[k == 0 ? 0 : n - k];
+      y.y = -y.y;
+      st(k, make_double2(0.5 * (z.x + y.x), 0.5 * (z.y + y.y)),
+         make_double2(0.5 * (z.y - y.y), -0.5 * (z.x - y.x)));
+    }
+  });
+}
+
+// c2r input of lane t < N2, element m of the packed row pair from the two
+// rows' half spectra: bin(k) returns (Xa[k], Xb[k]) for k <= n/2; the
+// imaginary parts of the DC and Nyquist bins are dropped (FFTW's c2r
+// convention), bins above n/2 are the conjugates of bins n - m.
+template <int N1, int N2, class Bin>
+RF_HD double2 c2r_in(int m, Bin bin) {
+  constexpr int n = N1 * N2;
+  const bool lo = 2 * m <= n;
+  const int k = lo ? m : n - m;
+  double2 xa, xb;
+  bin(k, xa, xb);
+  if (k == 0 || 2 * k == n) { xa.y = 0.0; xb.y = 0.0; }
+  return lo ? make_double2(xa.x - xb.y, xa.y + xb.x) : make_double2(xa.x + xb.y, -xa.y + xb.x);
+}
+
+}  // namespace rows
 }  // namespace rfft
 
 #endif

@@ -103,6 +103,9 @@ const int kMaxNu = 2048, kMaxNphi = 4096;
 // Largest max|Im K-hat| / max|Re K-hat| accepted at create (round-off of the
 // transform of a real-even table is ~1e-16 x log2 of its size).
 const double kKhatImagTol = 1e-13;
+// Graph chains: up to 16 disk groups, on at most 4 streams (one per
+// hardware queue of the process).
+const int kMaxChainGroups = 16, kMaxChainStreams = 4;
 const int kStreamCols = 58;   // output columns per wave of the streaming update (kernel's STREAM_OUT)
 std::vector<int2> xcd_order(const std::vector<int2> &v) {
   const int L = (int)v.size();
@@ -157,9 +160,11 @@ struct dfk_handle {
   int thr_rows = 256, thr_cols = 256, thr_upd = 256;   // threads per workgroup
   int upd_kind = 0;              // 0: LDS-tile update kernel, 1: register-streaming
   int nchains = 3;               // disk groups stepped as parallel chains of a graph
-  int row_inplace = 0;           // requested: in-place row transforms (k_rows_*_group)
+  int row_inplace = 0;           // requested row transforms: 0 cooperative, 1 in place
+                                 // (k_rows_*_group), 2 register two-pass (k_rows_*_reg)
   int rows_ip = 0;               // in use (every self-gravitating disk's rows allow it)
   int rows_batched = 0;          // in place with pr > 1 for some disk
+  int rows_big = 0;              // register rows with some Nphi > 512
   int conv_batched = 0;          // some disk's in-place workgroup convolution has cb > 1
   int fused_req = 0;             // requested: fused row kernels (row c2r + update + row r2c)
                                  // with this many threads per workgroup (0: off)
@@ -213,10 +218,19 @@ int build_layout(dfk_handle *h) {
   i64 part = 0;
   size_t lu = 0, lr = 0, lc = 0, lcw = 0, lcg = 0, lcr = 0;
   const size_t kMaxLds = h->lds_max;
+  // the requested row transform where every self-gravitating disk allows it,
+  // else the next simpler one (register -> in place -> cooperative)
   h->rows_ip = h->row_inplace;
+  h->rows_big = 0;
   for (const DiskDev &D : h->hD)
-    if (D.selfgrav && (!inplace_radices(D.rad_row, D.nst_row, 1) || D.Nphi > ip_nmax(h->thr_rows)))
-      h->rows_ip = 0;
+    if (D.selfgrav) {
+      if (h->rows_ip == 2 && dfk_rows_reg_pairs(D.Nphi) == 0) h->rows_ip = 1;
+      h->rows_big |= D.Nphi > 512;
+    }
+  if (h->rows_ip == 1)
+    for (const DiskDev &D : h->hD)
+      if (D.selfgrav && (!inplace_radices(D.rad_row, D.nst_row, 1) || D.Nphi > ip_nmax(h->thr_rows)))
+        h->rows_ip = 0;
   // fused row kernels: tiles of ti x Nphi with ti even (whole row pairs)
   h->fused = h->fused_req > 0 && h->upd_kind == 0;
   for (const DiskDev &D : h->hD) {
@@ -265,21 +279,24 @@ int build_layout(dfk_handle *h) {
       // build with smaller LDS needs stays usable)
       // ping-pong buffers (+ the twiddle table where staged in LDS)
       auto lds_rows = [&](int pr) {
-        return (size_t)((h->rows_ip ? 1 : 2) * pr + (D.tw_lds & 1)) * D.Nphi * sizeof(double2);
+        return (size_t)((h->rows_ip == 1 ? 1 : 2) * pr + (D.tw_lds & 1)) * D.Nphi * sizeof(double2);
       };
       // in-place columns need one buffer per column, cooperative ones two;
       // the in-place workgroup kernels take at most ip_nmax points at once
       auto lds_cols = [&](int cb) {
         return (size_t)((D.conv_wave ? 1 : 2) * cb + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
       };
-      while (D.pr > 1 && (lds_rows(D.pr) > kMaxLds ||
-                          (h->rows_ip && D.pr * D.Nphi > ip_nmax(h->thr_rows))))
+      while (h->rows_ip != 2 && D.pr > 1 &&
+             (lds_rows(D.pr) > kMaxLds || (h->rows_ip == 1 && D.pr * D.Nphi > ip_nmax(h->thr_rows))))
         --D.pr;
       while (D.cb > 1 && (lds_cols(D.cb) > kMaxLds ||
                           (D.conv_wave == 2 && D.cb * D.M > ip_nmax(h->thr_cols))))
         --D.cb;
-      for (int g = 0; g < ceil_div(D.Nu, 2 * D.pr); ++g) mr.push_back(make_int2(d, g));
-      lr = std::max(lr, lds_rows(D.pr));
+      // register rows: row pairs per workgroup fixed by Nphi, twiddles read
+      // through the caches (the disk's own pr is kept for the other modes)
+      const int prb = h->rows_ip == 2 ? dfk_rows_reg_pairs(D.Nphi) : D.pr;
+      for (int g = 0; g < ceil_div(D.Nu, 2 * prb); ++g) mr.push_back(make_int2(d, g));
+      lr = std::max(lr, h->rows_ip == 2 ? dfk_rows_reg_lds(D.Nphi) : lds_rows(D.pr));
       if (D.conv_wave == 3) {   // columns per workgroup fixed by the transform length
         for (int g = 0; g < ceil_div(D.Nc, dfk_conv_reg_cols(D.M)); ++g) mreg.push_back(make_int2(d, g));
         lcr = std::max(lcr, dfk_conv_reg_lds(D.M));
@@ -293,7 +310,7 @@ int build_layout(dfk_handle *h) {
   }
   h->rows_batched = h->conv_batched = 0;
   for (const DiskDev &D : h->hD) {
-    if (D.selfgrav && D.pr > 1) h->rows_batched = 1;
+    if (D.selfgrav && D.pr > 1 && h->rows_ip == 1) h->rows_batched = 1;
     if (D.selfgrav && D.conv_wave == 2 && D.cb > 1) h->conv_batched = 1;
   }
   h->upd0[h->nd] = (int)mu.size();
@@ -391,6 +408,12 @@ double *field(dfk_handle *h, int buf, int f, int d) {
   return h->st + (i64)(buf * 3 + f) * h->ntot + h->hD[d].so;
 }
 
+// Launch variant of the row kernels (dfk_launch_r2c / dfk_launch_c2r).
+int row_kernel(const dfk_handle *h) {
+  if (h->rows_ip == 2) return h->rows_big ? 4 : 3;
+  return h->rows_ip ? 1 + h->rows_batched : 0;
+}
+
 // Row r2c of map range [base, base + nblk), with the loop control of disks
 // ctl0 .. ctl0 + nctl - 1 in trailing workgroups of the same launch (nctl = 0:
 // none), on stream s.
@@ -398,7 +421,7 @@ hipError_t launch_r2c(dfk_handle *h, hipStream_t s, int nblk, int base, int par,
                       int ctl0 = 0, double2 *spec = nullptr) {
   return dfk_launch_r2c(s, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl,
                         h->st, h->ntot, h->geom, h->tw, spec ? spec : h->spec, par, nctl, ctl0,
-                        h->pcfl, h->pcm, h->samples, h->cap, h->rows_ip ? 1 + h->rows_batched : 0);
+                        h->pcfl, h->pcm, h->samples, h->cap, row_kernel(h));
 }
 
 // The half spectrum that holds S-hat of the states in buffer par: with fused
@@ -409,7 +432,7 @@ double2 *spec_of(dfk_handle *h, int par) { return (h->fused && par) ? h->spec2 :
 // Row c2r of map range [base, base + nblk) on stream s.
 hipError_t launch_c2r(dfk_handle *h, hipStream_t s, int nblk, int base) {
   return dfk_launch_c2r(s, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl, h->spec,
-                        h->geom, h->tw, h->phi, h->rows_ip ? 1 + h->rows_batched : 0);
+                        h->geom, h->tw, h->phi, row_kernel(h));
 }
 
 // Column convolution of disks [a, b) on stream s, in place on `spec` (null:
@@ -538,8 +561,10 @@ std::vector<int> chain_bounds(dfk_handle *h) {
   for (const DiskDev &D : h->hD) tot += (i64)D.Nu * D.Nphi;
   for (int d = 0; d < h->nd && (int)cut.size() < k; ++d) {
     acc += (i64)h->hD[d].Nu * h->hD[d].Nphi;
-    // close the group once it holds its share; keep a disk for every later group
-    if (acc * k >= tot * (i64)cut.size() && h->nd - (d + 1) >= k - (int)cut.size()) cut.push_back(d + 1);
+    // close the group once it holds its share, keeping a disk for every
+    // later group, or when only one disk per later group is left
+    const int left = h->nd - (d + 1), later = k - (int)cut.size();
+    if (left == later || (left > later && acc * k >= tot * (i64)cut.size())) cut.push_back(d + 1);
   }
   cut.push_back(h->nd);
   return cut;
@@ -551,7 +576,8 @@ int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
   auto it = h->graphs.find(key);
   if (it != h->graphs.end()) { *out = it->second; return DFK_OK; }
   const std::vector<int> cb = chain_bounds(h);
-  const int nc = (int)cb.size() - 1;
+  const int ng = (int)cb.size() - 1;              // disk groups
+  const int nc = std::min(ng, kMaxChainStreams);  // streams; group g runs on stream g % nc
   HIPCHK(hipSetDevice(h->device));
   while ((int)h->side.size() < nc - 1) {
     hipStream_t s;
@@ -568,9 +594,11 @@ int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
   if (nc == 1) {
     for (int s = 0; s < nsteps && rc == DFK_OK; ++s) rc = launch_step_all(h, (par0 + s) & 1);
   } else {
-    // chain c steps disks [cb[c], cb[c+1]) on its own stream: the chains
-    // share nothing but the launch, so one chain's kernels run while
-    // another's drain
+    // stream c steps the disk groups c, c + nc, ... (group g holds disks
+    // [cb[g], cb[g+1])), each step group after group: the streams share
+    // nothing but the launch, so one stream's kernels run while another's
+    // drain, and a group's four launches follow each other while its
+    // spectrum and potential are still in the caches
     auto chk = [&](hipError_t x) {
       if (x != hipSuccess && rc == DFK_OK) {
         set_err("graph chain: %s", hipGetErrorString(x));
@@ -582,7 +610,8 @@ int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
     for (int c = 0; c < nc && rc == DFK_OK; ++c) {
       hipStream_t st = c == 0 ? h->stream : h->side[c - 1];
       for (int s = 0; s < nsteps && rc == DFK_OK; ++s)
-        rc = launch_step_range(h, st, cb[c], cb[c + 1], (par0 + s) & 1);
+        for (int g = c; g < ng && rc == DFK_OK; g += nc)
+          rc = launch_step_range(h, st, cb[g], cb[g + 1], (par0 + s) & 1);
     }
     for (int c = 1; c < nc; ++c) {
       chk(hipEventRecord(h->join[c - 1], h->side[c - 1]));
@@ -1094,7 +1123,7 @@ int dfk_potential_noalloc(dfk_handle *h, int d, const double *Sigma, double *Phi
   double2 *spb = (double2 *)shift(spec, D.spo * (i64)sizeof(double2));
   DiskCtl *cb = (DiskCtl *)shift(ctl, (i64)d * (i64)sizeof(DiskCtl));
   const int nr = h->r2c0[d + 1] - h->r2c0[d];
-  const int ip = h->rows_ip ? 1 + h->rows_batched : 0;
+  const int ip = row_kernel(h);
   HIPCHK(dfk_launch_r2c(s, nr, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, h->r2c0[d], cb, st, 0,
                         h->geom, h->tw, spb, 0, 0, 0, nullptr, nullptr, nullptr, 0, ip));
   HIPCHK(launch_conv_range(h, s, d, d + 1, spb, -1, cb));
@@ -1449,7 +1478,10 @@ int dfk_set_threads(dfk_handle *h, int rows, int cols, int update) {
 }
 
 int dfk_set_row_transform(dfk_handle *h, int inplace) {
-  if (!h || inplace < 0 || inplace > 1) { set_err("row transform must be 0 (cooperative) or 1 (in place)"); return DFK_EINVAL; }
+  if (!h || inplace < 0 || inplace > 2) {
+    set_err("row transform must be 0 (cooperative), 1 (in place) or 2 (register)");
+    return DFK_EINVAL;
+  }
   if (inplace != h->row_inplace) {
     h->row_inplace = inplace;
     h->layout_ok = false;
@@ -1507,7 +1539,10 @@ int dfk_get_update_kernel(dfk_handle *h, int *kind) {
 }
 
 int dfk_set_chains(dfk_handle *h, int k) {
-  if (!h || k < 1 || k > 4) { set_err("chains must be 1 .. 4"); return DFK_EINVAL; }
+  if (!h || k < 1 || k > kMaxChainGroups) {
+    set_err("chains must be 1 .. %d", kMaxChainGroups);
+    return DFK_EINVAL;
+  }
   if (k != h->nchains) {
     h->nchains = k;
     drop_graphs(h);

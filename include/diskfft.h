@@ -200,11 +200,14 @@ int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update);
 /* Row transforms of all disks of the handle (an autotuner knob; no
  * counterpart in the reference): 0 = the workgroup cooperates on its pr
  * complex rows with two LDS buffers (default); 1 = in place in one LDS buffer
- * (half the LDS per row; pr x Nphi is then kept <= max(8 x threads, 1024)).
- * In place needs every self-gravitating disk's Nphi to factor into 2, 3, 4
- * and 8; otherwise the handle keeps the cooperative transforms, and
- * dfk_get_row_transform reports the variant in use.  Both agree to
- * round-off. */
+ * (half the LDS per row; pr x Nphi is then kept <= max(8 x threads, 1024));
+ * 2 = two-pass register transforms (one 64-thread wavefront per 4 row pairs
+ * for Nphi <= 256, per 2 above; pr and the row threads do not apply).  In
+ * place needs every self-gravitating disk's Nphi to factor into 2, 3, 4 and
+ * 8, register every such Nphi to be 128, 192, 256, 384, 512, 768 or 1024;
+ * otherwise the handle takes the next of register -> in place ->
+ * cooperative that all its disks allow, and dfk_get_row_transform reports
+ * the variant in use.  All agree to round-off. */
 int dfk_set_row_transform(dfk_handle *h, int inplace);
 int dfk_get_row_transform(dfk_handle *h, int *inplace);
 
@@ -231,12 +234,14 @@ int dfk_get_update_kernel(dfk_handle *h, int *kind);
 int dfk_set_fused_rows(dfk_handle *h, int threads);
 int dfk_get_fused_rows(dfk_handle *h, int *threads);
 
-/* Graph chains (1 .. 4; no counterpart in the reference, which steps one
+/* Graph chains (1 .. 16; no counterpart in the reference, which steps one
  * disk): the disks of the handle are split into that many contiguous groups
  * of about equal zone counts, and each group is stepped by its own chain of
- * launches in the loop's graphs, on its own stream, so that the kernels of
- * one group fill the GPU while those of another drain.  Results are the same
- * bits as with one chain.  Default 3 (at most one chain per disk); only
+ * launches in the loop's graphs; group g runs on stream g mod min(k, 4), the
+ * groups of one stream one after the other within each step, so that the
+ * kernels of one group fill the GPU while those of another drain.  Results
+ * are the same bits as with one chain.  Default 3 (at most one group per
+ * disk); only
  * handles whose disks all have self-gravity use more than one chain;
  * dfk_get_chains reports the number in use. */
 int dfk_set_chains(dfk_handle *h, int k);

@@ -156,6 +156,36 @@ def test_potential_register_columns(orc, dfk, Nu, Nphi, ok):
     assert relerr(p_reg, k.potential(Sig)) < 1e-13
 
 
+@pytest.mark.parametrize("Nu,Nphi,mode", [(64, 128, 2), (37, 192, 2), (256, 256, 2), (33, 384, 2),
+                                          (512, 512, 2), (96, 768, 2), (512, 1024, 2), (3, 128, 2),
+                                          (5, 256, 2), (40, 96, 1), (35, 40, 0)])
+def test_potential_register_rows(orc, dfk, Nu, Nphi, mode):
+    """The two-pass register row transforms (dfk_set_row_transform 2) at every
+    row length they are built for (Nphi = 128 .. 1024), with odd Nu (the last
+    pair's second row past the disk) and disks of fewer rows than a
+    workgroup's pairs: round-off agreement with the cooperative transforms and
+    the oracle, alone and with the register column convolution; other Nphi
+    fall back to the in-place rows, or to the cooperative ones where those do
+    not apply (same bits as asking for them)."""
+    print(f"case potential {Nu}x{Nphi} register rows", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    rng = np.random.default_rng(Nu + 5 * Nphi)
+    Sig = rng.random((Nu, Nphi))
+    p_ref = e.potential(Sig)
+    e.set_row_transform(mode)
+    p_fb = e.potential(Sig)
+    e.set_row_transform(2)
+    assert e.get_row_transform() == mode
+    p_reg = e.potential(Sig)
+    assert np.array_equal(p_reg, p_fb)
+    if mode == 2:
+        assert relerr(p_reg, p_ref) < 1e-14
+    assert relerr(p_reg, k.potential(Sig)) < 1e-13
+    e.set_fft_options(1, 0, 3)
+    p_both = e.potential(Sig)
+    assert relerr(p_both, k.potential(Sig)) < 1e-13
+
+
 @pytest.mark.parametrize("Nu,Nphi", [(16, 34), (20, 38), (17, 46), (12, 1031), (1024, 1024),
                                      (64, 4096), (2048, 64)])
 def test_potential_any_size(orc, dfk, Nu, Nphi):
@@ -530,17 +560,20 @@ def test_potential_row_transform(orc, dfk, Nu, Nphi, ok, threads, pr, cb):
     assert relerr(p_opt, k.potential(Sig)) < 1e-13
 
 
-@pytest.mark.parametrize("cols", [2, 3])
-@pytest.mark.parametrize("Nu,Nphi,eps", [(64, 96, 0.02), (256, 256, 0.01)])
-def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps, cols):
-    """The 10-step loop gate with in-place row transforms and the in-place
+@pytest.mark.parametrize("rows,cols", [(1, 2), (1, 3), (2, 3), (2, 0)])
+@pytest.mark.parametrize("Nu,Nphi,eps", [(64, 96, 0.02), (256, 256, 0.01), (61, 384, 0.01)])
+def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps, rows, cols):
+    """The 10-step loop gate with in-place (1) or register (2; Nphi = 96 has
+    none and keeps in place) row transforms and the cooperative (0), in-place
     workgroup (2) or two-pass register (3) column convolution."""
-    print(f"case loop {Nu}x{Nphi} in-place transforms, columns mode {cols}", flush=True)
+    print(f"case loop {Nu}x{Nphi} rows mode {rows}, columns mode {cols}", flush=True)
     h, k = _lka98(orc, Nu, Nphi, eps)
     e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
-    e.set_row_transform(1)
+    e.set_row_transform(rows)
+    assert e.get_row_transform() == (1 if Nphi == 96 else rows)
     e.set_fft_options(1, 0, cols)
-    assert e.get_fft_options(0)[2] == cols
+    reg_lengths = (128, 192, 256, 384, 512, 768, 1024)   # both in-place modes cover these
+    assert e.get_fft_options(0)[2] == (cols if cols == 0 or 2 * Nu in reg_lengths else 0)
     e.set_threads(128, 128, 256)
     e.set_state(*h.state())
     mD = h.disk_mass()
@@ -550,6 +583,58 @@ def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps, cols):
     errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
     print("  rel errors Sigma, vR, vphi:", errs, flush=True)
     assert e.status()["steps"] == 10 and max(errs) <= LOOP_TOL
+
+
+@pytest.mark.parametrize("big", [False, True])
+def test_register_rows_ensemble(orc, dfk, big):
+    """A handle of disks of every register row length up to 512 (odd and even
+    Nu, register and cooperative columns), rows in register mode: each disk
+    the same bits as alone with the same options, one of them within the loop
+    gate of the oracle.  With a disk of Nphi = 768 in the handle (big), the
+    rows run in the build for lengths up to 1024: round-off agreement."""
+    print(f"case register rows ensemble, big {big}", flush=True)
+    shapes = [(64, 128, 0.02), (37, 192, 0.01), (48, 256, 0.02), (33, 384, 0.01), (20, 512, 0.02)]
+    cols = [3, 0, 3, 0, 3]
+    if big:
+        shapes.append((24, 768, 0.02))
+        cols.append(3)
+    params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
+    ics, mDs, hs = [], [], []
+    for a, b, c in shapes:
+        h, k = _lka98(orc, a, b, c)
+        ics.append(h.state())
+        mDs.append(h.disk_mass())
+        hs.append((h, k))
+
+    def setup(e, ds):
+        e.set_update_kernel(1)
+        e.set_row_transform(2)
+        for q, d in enumerate(ds):
+            e.set_fft_options(1, 0, cols[d], d=q)
+            e.set_state(*ics[d], d=q)
+
+    ens = dfk.Ensemble(params)
+    setup(ens, range(len(shapes)))
+    assert ens.get_row_transform() == 2
+    ens.run_init(1e9, cfl=0.3, mD=mDs, max_steps=10)
+    ens.advance(12)
+    for d, p in enumerate(params):
+        one = dfk.Ensemble(p)
+        setup(one, [d])
+        one.run_init(1e9, cfl=0.3, mD=mDs[d], max_steps=10)
+        one.advance(12)
+        for a, b in zip(ens.get_state(d), one.get_state()):
+            if big:
+                assert relerr(a, b) < 1e-13
+            else:
+                assert np.array_equal(a, b)
+        one.close()
+    h, k = hs[1]
+    orc.run(h, k, 1, 1e9, dtout=1e9, cfl=0.3, max_steps=10, mD=mDs[1])
+    errs = [relerr(a, b) for a, b in zip(ens.get_state(1), h.state())]
+    print("  rel errors Sigma, vR, vphi (37x192 vs oracle):", errs, flush=True)
+    assert max(errs) <= LOOP_TOL
+    ens.close()
 
 
 @pytest.mark.parametrize("rows_ip", [0, 1])
@@ -598,7 +683,7 @@ def test_mixed_transform_modes_ensemble(orc, dfk, rows_ip):
     ens.close()
 
 
-@pytest.mark.parametrize("chains", [2, 3])
+@pytest.mark.parametrize("chains", [2, 3, 5])
 def test_graph_chains_same_bits(orc, dfk, chains):
     """Disks stepped as parallel graph chains (dfk_set_chains) give the same
     states, clocks and samples as one chain, with disks that stop at
