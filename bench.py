@@ -246,11 +246,12 @@ def main():
             pre = "k_update_stream<false" if r["shapes"].get("kind") == 1 else "k_update<false"
             names = [k for k in tk if k.startswith(pre)]
         elif dom == "cols_conv":  # the column kernel of the tuned mode
-            pre = {0: "k_cols_conv", 1: "k_cols_conv_wave", 2: "k_cols_conv_group<"}[
-                r["shapes"].get("fft", [0, 0, 0])[2]]
+            pre = {0: "k_cols_conv", 1: "k_cols_conv_wave", 2: "k_cols_conv_group<",
+                   3: "k_cols_conv_reg"}[r["shapes"].get("fft", [0, 0, 0])[2]]
             names = [k for k in tk if k == pre or (pre.endswith("<") and k.startswith(pre))]
-        elif r["shapes"].get("rows_inplace"):   # the in-place row kernels
-            names = [k for k in tk if k.startswith("k_" + dom + "_group<")]
+        elif r["shapes"].get("rows_inplace"):   # the in-place (1) or register (2) row kernels
+            suf = "_group<" if r["shapes"]["rows_inplace"] == 1 else "_reg<"
+            names = [k for k in tk if k.startswith("k_" + dom + suf)]
         else:
             names = [k for k in tk if k == "k_" + dom]
         traffic = tk[names[0]]["traffic_bytes"] if names else None

@@ -11,8 +11,8 @@ at R = 1) in one batched handle with every other run of the shard.  The
 extractor (dfk_extract_growth) turns each run's (t, |C0|, |C2|, arg C2)
 series into the targets (gamma, Omega_p, saturated |C2|/|C0|).
 
-All runs use one grid (default 96 x 96) so a shard is a uniform batch; the
-grid is a numerical choice, not a surrogate input.  In this model the m = 2
+All runs use one grid (default 256 x 256, the committed table's) so a shard
+is a uniform batch; the grid is a numerical choice, not a surrogate input.  In this model the m = 2
 amplitude first grows from the 1e-3 noise by a fast mode of the sharp inner
 edge (gamma ~ 10, Omega_p ~ 14, within t < 2) and then wanders nonlinearly.
 The targets are therefore: gamma and Omega_p of that first linear growth,
@@ -24,7 +24,9 @@ the second half of the run (sat_late) is stored too; it follows the
 turbulent late state and scatters by ~20% between neighbouring parameter
 points.  The stored series keep the first 600 samples and every 25th after.
 
-    python heavy-disks_amd/surrogate_data.py --first 0 --count 1024 --out shard0.npz
+    python heavy-disks_amd/surrogate_data.py --first 0 --count 544 --tlim 8 --out shard0.npz
+
+(surrogate_fit.py's docstring gives the recipe of the committed table.)
 """
 import argparse
 import os
@@ -47,7 +49,7 @@ def tlim_for(d):
     return ORBITS * 2.0 * np.pi / np.sqrt(1.0 + d["mass_ratio"])
 
 
-def run_shard(first, count, Nu=96, Nphi=96, seed=2026, total=4352, chunk=2000,
+def run_shard(first, count, Nu=256, Nphi=256, seed=2026, total=4352, chunk=2000,
               max_wall=None, verbose=True, dtout=0.01, lo_factor=10.0, hi_factor=100.0,
               sat_frac=0.5, peak_window=2.0, keep_first=600, keep_every=25, noise_seed=42,
               tlim=None, indices=None):
@@ -133,8 +135,8 @@ def main():
     ap.add_argument("--first", type=int, default=0)
     ap.add_argument("--count", type=int, default=1024)
     ap.add_argument("--total", type=int, default=4352)
-    ap.add_argument("--nu", type=int, default=96)
-    ap.add_argument("--nphi", type=int, default=96)
+    ap.add_argument("--nu", type=int, default=256)
+    ap.add_argument("--nphi", type=int, default=256)
     ap.add_argument("--max-wall", type=float, default=None)
     ap.add_argument("--noise-seed", type=int, default=42,
                     help="noise realization of every run (negative: each run's own seed)")

@@ -1,8 +1,15 @@
 """Train the surrogate ensemble on hydro shards, judge it on held-out hydro runs,
 scan 10^5 parameter points, and compare with surrogate-free baselines.
 
-    python heavy-disks_amd/surrogate_fit.py shard*.npz --out surrogate_eval.json \
-        [--export tests/golden/surrogate_runs.npz]
+    python heavy-disks_amd/surrogate_fit.py shard*.npz [--extend rerun.npz ...] \
+        --out surrogate_eval.json [--export tests/golden/surrogate_runs.npz]
+
+The committed table: eight shards of 544 runs at 256 x 256 to t = 8
+(surrogate_data.py --first F --count 544 --tlim 8, F = 0, 544, ..., 3808),
+and the 73 runs listed in tests/golden/surrogate_extended_runs.txt, which
+had not grown and saturated by t = 8, re-run to 20 outer orbits
+(surrogate_data.py --indices tests/golden/surrogate_extended_runs.txt) and
+merged with --extend.
 
 Runs 0 .. n_train-1 of the shards train the ensemble, runs n_train .. end are
 held out.  Error of a prediction p of a hydro value h for output k:
@@ -40,11 +47,27 @@ def load(paths):
     return np.concatenate(Xs), np.concatenate(Ys), np.concatenate(oks)
 
 
-def export(paths, out):
+def table(paths, extend=()):
+    """(X, Y, ok, tlim_extended, grid) of the shards, the rows of the runs in
+    the `extend` files (made with surrogate_data.py --indices) replaced by
+    those runs and marked."""
+    X, Y, ok = load(paths)
+    ext = np.zeros(len(X), np.int8)
+    d0 = np.load(sorted(paths)[0])
+    grid = d0["grid"] if "grid" in d0.files else np.array([0, 0])
+    for p in extend:
+        d = np.load(p)
+        i = d["index"]
+        X[i], Y[i], ok[i] = d["X"], d["Y"], d["ok"]
+        ext[i] = 1
+    return X, Y, ok, ext, grid
+
+
+def export(paths, out, extend=()):
     """The runs' parameters and targets only (a few hundred KB): the table the
     surrogate tests train on without regenerating the hydro runs."""
-    X, Y, ok = load(paths)
-    np.savez_compressed(out, X=X, Y=Y, ok=ok, first=0)
+    X, Y, ok, ext, grid = table(paths, extend)
+    np.savez_compressed(out, X=X, Y=Y, ok=ok, first=0, grid=grid, tlim_extended=ext)
 
 
 def errors(pred, true, scale):
@@ -82,11 +105,13 @@ def main():
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--scan", type=int, default=100000)
+    ap.add_argument("--extend", nargs="*", default=[],
+                    help="re-run files whose rows replace those of the same runs")
     ap.add_argument("--export", default=None, help="also write the (X, Y, ok) table here")
     a = ap.parse_args()
     if a.export:
-        export(a.shards, a.export)
-    X, Y, ok = load(a.shards)
+        export(a.shards, a.export, a.extend)
+    X, Y, ok, _, _ = table(a.shards, a.extend)
     tr = np.arange(len(X)) < a.n_train
     te = ~tr
     tr &= ok & np.all(np.isfinite(Y), axis=1)

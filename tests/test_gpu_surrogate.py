@@ -3,9 +3,10 @@ with hydro fallback, and what it gains over the method without it.
 
 Data: the table of 4352 batched hydro runs in tests/golden/surrogate_runs.npz
 (parameters and extracted targets only): 256 x 256 power-law disks drawn from
-the ensemble's parameter box (heavy-disks_amd/surrogate_data.py --nu 256
---nphi 256 --tlim 8; the 73 runs whose m = 2 growth had not completed by
-t = 8 re-run to 20 outer orbits), exported by surrogate_fit.py --export.
+the ensemble's parameter box (heavy-disks_amd/surrogate_data.py --tlim 8; the
+73 runs of tests/golden/surrogate_extended_runs.txt, whose m = 2 growth had
+not completed by t = 8, re-run to 20 outer orbits), exported by
+surrogate_fit.py --export (its docstring gives the recipe).
 The ensemble (5 members, BF16 weights, FP32 accumulation) is trained on the
 GPU by the library's own kernels on runs 0..4095, with the configuration of
 tests/longrun/surrogate_eval.json (6000 epochs, batch 256, lr 2e-3; the

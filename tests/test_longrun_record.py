@@ -44,6 +44,7 @@ def test_mass_conserved(rec):
 REC_PL = os.path.join(HERE, "longrun", "results_pl256.json")
 REC_FU = os.path.join(HERE, "longrun", "results_fused.json")
 REC_SG = os.path.join(HERE, "longrun", "surrogate_eval.json")
+ROOT = os.path.dirname(HERE)
 
 
 def test_config2_long_run_record():
@@ -96,3 +97,41 @@ def test_surrogate_record_beats_baselines_and_meets_three_percent():
         for base in ("nearest_neighbour", "linear_interp"):
             assert sg["median_rel_err"] < r[base][n]["median_rel_err"]
 
+
+
+def test_surrogate_table_marks_the_extended_runs():
+    """The surrogate table's runs re-made to 20 outer orbits are exactly those
+    listed in tests/golden/surrogate_extended_runs.txt, and the table merge
+    (surrogate_fit.table) puts a re-run's rows at its draw indices."""
+    import sys
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "heavy-disks_amd"))
+    import surrogate_fit
+    gold = os.path.join(ROOT, "tests", "golden")
+    tab = np.load(os.path.join(gold, "surrogate_runs.npz"))
+    listed = np.loadtxt(os.path.join(gold, "surrogate_extended_runs.txt"), dtype=int)
+    assert np.array_equal(np.sort(listed), np.flatnonzero(tab["tlim_extended"]))
+    assert tab["grid"].tolist() == [256, 256] and len(tab["X"]) == 4352
+
+
+def test_surrogate_table_merge(tmp_path):
+    import sys
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "heavy-disks_amd"))
+    import surrogate_fit
+    rng = np.random.default_rng(0)
+    shards = []
+    for first in (0, 4):
+        p = os.path.join(tmp_path, f"s{first}.npz")
+        np.savez(p, X=rng.random((4, 4)), Y=rng.random((4, 3)), ok=np.ones(4, bool), first=first,
+                 grid=np.array([8, 8]))
+        shards.append(p)
+    rer = os.path.join(tmp_path, "re.npz")
+    np.savez(rer, X=np.full((2, 4), 7.0), Y=np.full((2, 3), 9.0), ok=np.array([True, False]),
+             index=np.array([1, 6]), first=0, grid=np.array([8, 8]))
+    X, Y, ok, ext, grid = surrogate_fit.table(shards[::-1], [rer])
+    base = np.concatenate([np.load(p)["Y"] for p in shards])
+    assert ext.tolist() == [0, 1, 0, 0, 0, 0, 1, 0] and grid.tolist() == [8, 8]
+    assert np.all(Y[[1, 6]] == 9.0) and np.all(X[[1, 6]] == 7.0) and ok.tolist()[6] is False
+    keep = [0, 2, 3, 4, 5, 7]
+    assert np.array_equal(Y[keep], base[keep])
