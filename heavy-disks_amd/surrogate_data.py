@@ -12,17 +12,20 @@ extractor (dfk_extract_growth) turns each run's (t, |C0|, |C2|, arg C2)
 series into the targets (gamma, Omega_p, saturated |C2|/|C0|).
 
 All runs use one grid (default 256 x 256, the committed table's) so a shard
-is a uniform batch; the grid is a numerical choice, not a surrogate input.  In this model the m = 2
-amplitude first grows from the 1e-3 noise by a fast mode of the sharp inner
-edge (gamma ~ 10, Omega_p ~ 14, within t < 2) and then wanders nonlinearly.
+is a uniform batch; the grid is a numerical choice, not a surrogate input.
+In this model the m = 2 amplitude first grows from the 1e-3 noise by a fast
+mode of the sharp inner edge (gamma ~ 10, Omega_p ~ 14, within t < 2) and
+then wanders nonlinearly.
 The targets are therefore: gamma and Omega_p of that first linear growth,
 fitted while |C2|/|C0| rises from 10x to 100x its initial value
 (dfk_extract_growth_ex, hi_factor = 100), from samples every dtout = 0.01;
 sat = the amplitude at which that growth saturates, the largest |C2|/|C0|
-within 2 time units of the window start (sat_peak).  The mean |C2|/|C0| over
-the second half of the run (sat_late) is stored too; it follows the
-turbulent late state and scatters by ~20% between neighbouring parameter
-points.  The stored series keep the first 600 samples and every 25th after.
+within 2 time units of the window start (sat_peak).  The saturation of the
+extractor's default definition, the mean |C2|/|C0| over the final quarter of
+the run (sat_late), is stored too; it follows the turbulent late state and
+scatters by ~20% between neighbouring parameter points, so it is not a
+surrogate target.  The stored series keep the first 600 samples and every
+25th after.
 
     python heavy-disks_amd/surrogate_data.py --first 0 --count 544 --tlim 8 --out shard0.npz
 
@@ -51,7 +54,7 @@ def tlim_for(d):
 
 def run_shard(first, count, Nu=256, Nphi=256, seed=2026, total=4352, chunk=2000,
               max_wall=None, verbose=True, dtout=0.01, lo_factor=10.0, hi_factor=100.0,
-              sat_frac=0.5, peak_window=2.0, keep_first=600, keep_every=25, noise_seed=42,
+              sat_frac=0.25, peak_window=2.0, keep_first=600, keep_every=25, noise_seed=42,
               tlim=None, indices=None):
     """Runs first .. first+count-1 of the draw on one Nu x Nphi handle, each to
     its 20 outer orbits (or to `tlim`: gamma, Omega_p and sat_peak are fixed
