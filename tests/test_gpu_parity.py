@@ -187,7 +187,7 @@ def test_potential_register_rows(orc, dfk, Nu, Nphi, mode):
 
 
 @pytest.mark.parametrize("Nu,Nphi", [(16, 34), (20, 38), (17, 46), (12, 1031), (1024, 1024),
-                                     (64, 4096), (2048, 64)])
+                                     (64, 4096), (2048, 64), (2048, 4096)])
 def test_potential_any_size(orc, dfk, Nu, Nphi):
     """Transform lengths with prime factors above 13 (34 = 2 x 17, 38 = 2 x
     19, 2Nu = 34, 46 = 2 x 23, 1031 prime: the direct-DFT stage) and the
@@ -306,9 +306,10 @@ def _lka98(orc, Nu, Nphi, eps, amp=1e-3, selfgrav=1):
 @pytest.mark.parametrize("Nu,Nphi,eps", [(64, 64, 0.02), (128, 128, 0.02), (96, 80, 0.01),
                                          (256, 256, 0.005), (48, 36, 0.02), (512, 1024, 0.01),
                                          (3, 4, 0.02), (64, 34, 0.02), (1024, 1024, 0.01),
-                                         (2048, 128, 0.01)])
+                                         (2048, 128, 0.01), (2048, 4096, 0.01)])
 def test_ten_step_loop_parity(orc, dfk, Nu, Nphi, eps, kind):
-    """The loop gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative."""
+    """The loop gate: Sigma, vR, vphi after 10 steps <= 1e-10 relative, from
+    the smallest grid to the largest the library takes (2048 x 4096)."""
     print(f"case 10-step loop {Nu}x{Nphi} eps={eps} update kernel {kind}", flush=True)
     h, k = _lka98(orc, Nu, Nphi, eps)
     e = dfk.Ensemble(dfk.disk_params(Nu, Nphi, eps=eps))
