@@ -118,8 +118,9 @@ def test_surrogate_scan_1e5_points(trained):
 
 def test_training_table_reproduces():
     """Four held-out runs of the table re-made now by the batched hydro (256 x
-    256, t = 8, their own handle): the same targets as the table holds, to the
-    bit (the kernels per disk are those of the table's 544-disk shards)."""
+    256, t = 8, their own handle): the same parameters, and the targets the
+    table holds to round-off (1e-9 relative: the tuned transform variants
+    change with the build, their results only in the last bits)."""
     import diskfft_amd
     if diskfft_amd.device_count() < 1:
         pytest.fail("no HIP device visible")
@@ -129,4 +130,6 @@ def test_training_table_reproduces():
     r = surrogate_data.run_shard(4096, 4, 256, 256, tlim=8.0, verbose=False)
     assert r["ok"].all() and np.array_equal(r["X"], tab["X"][4096:4100])
     assert not tab["tlim_extended"][4096:4100].any()
-    assert np.array_equal(r["Y"], tab["Y"][4096:4100]), (r["Y"], tab["Y"][4096:4100])
+    rel = np.abs(r["Y"] - tab["Y"][4096:4100]) / np.abs(tab["Y"][4096:4100])
+    print(f"  max relative difference from the table: {rel.max():.2e}", flush=True)
+    assert rel.max() < 1e-9, (r["Y"], tab["Y"][4096:4100])
