@@ -67,10 +67,19 @@ class DevBuf:
         self.hip.hipFree(self.p)
 
 
-def test_potential_noalloc_matches_potential(dfk, hip, orc):
-    shapes = [(256, 256, 0.01), (64, 96, 0.02)]           # the second call: a smaller disk
-    print("case potential_noalloc", shapes, flush=True)
+@pytest.mark.parametrize("modes", [None, (2, 3)])
+def test_potential_noalloc_matches_potential(dfk, hip, orc, modes):
+    """modes: the handle's default transforms, or the register row (2) and
+    column (3) kernels."""
+    shapes = [(256, 256, 0.01), (64, 96 if modes is None else 128, 0.02)]   # then a smaller disk
+    print("case potential_noalloc", shapes, "modes", modes, flush=True)
     e = dfk.Ensemble([dfk.disk_params(a, b, eps=c, Rin=0.1) for a, b, c in shapes])
+    if modes is not None:
+        e.set_row_transform(modes[0])
+        for d in range(len(shapes)):
+            e.set_fft_options(1, 0, modes[1], d=d)
+        assert e.get_row_transform() == modes[0]
+        assert all(e.get_fft_options(d)[2] == modes[1] for d in range(len(shapes)))
     s = ctypes.c_void_p()
     assert hip.hipStreamCreate(ctypes.byref(s)) == 0
     rng = np.random.default_rng(3)
