@@ -623,7 +623,7 @@ void orc_ic_lka98(orc_hydro *h, orc_kalnajs *grav, int selfgrav, double amp) {
   double *gRdisk = (double *)calloc(Nu, sizeof(double));
   if (selfgrav && grav) {
     size_t N = (size_t)Nu * Nphi;
-    double *Sax = (double *)malloc(sizeof(double) * N);
+    double *Sax = (double *)calloc(N, sizeof(double));   /* every element written below */
     double *Phi = (double *)malloc(sizeof(double) * N);
     double *gR = (double *)malloc(sizeof(double) * N), *gp = (double *)malloc(sizeof(double) * N);
     for (int i = 0; i < Nu; ++i)
