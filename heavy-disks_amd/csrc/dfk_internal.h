@@ -11,7 +11,8 @@
 //                           and V-hat after the column convolution (in place)
 //   khat                    real K-hat, per disk [Nc][Nu+1] (rows p=0..Nu of the
 //                           real-even M x Nc transform; row M-p equals row p)
-//   geom                    per disk GT_COUNT tables of Nu+1 doubles
+//   geom                    per disk GT_COUNT tables of Nu+1 doubles, then Nu+1
+//                           row records of RR_STRIDE doubles
 //   tw                      per transform length n: exp(-2 pi i t/n), t<n
 #ifndef DFK_INTERNAL_H
 #define DFK_INTERNAL_H
@@ -38,8 +39,27 @@ enum {
   GT_GRC,        // 1/(2 du R_i) interior, 1/(du R_i) at i=0, Nu-1 (self-gravity g_R)
   GT_GPC,        // 1/(2 dphi R_i)                                  (self-gravity g_phi)
   GT_DL,         // R_i dphi  (CFL azimuthal length, formed as hydro.hpp:56 forms it)
+  GT_AFZ,        // Rf_i dphi at the interior faces 1..Nu-1, 0 at faces 0 and Nu (no flux)
+  GT_HUP,        // 0.5 for the interior cells 1..Nu-2, 0 at cells 0 and Nu-1 (no slope):
+  GT_HDN,        //   MUSCL factor of an upwind cell left (HUP) / right (HDN, -0.5) of its face
   GT_COUNT
 };
+
+// After a disk's GT_COUNT tables, its row records: Nu+1 records of
+// RR_STRIDE doubles, record i holding the values of row (or face) i that the
+// streaming update reads, so that one scalar pointer per row reaches all of
+// them (values of the GT tables of the same name; record Nu holds only AFZ).
+enum {
+  RR_INVDRC = 0, RR_INVR, RR_GSTAR, RR_GRC, RR_GPC,   // source step of row i
+  RR_INVDR, RR_HUP, RR_HDN, RR_AFZ,                   // radial face i and its cells
+  RR_INVVOL, RR_INVDL, RR_AFACE, RR_DA, RR_DL,        // gather, azimuthal sweep, CFL, C_m of row i
+  RR_STRIDE = 16
+};
+#define DFK_GEOM_DOUBLES(Nu) ((long long)(GT_COUNT + RR_STRIDE) * ((Nu) + 1))
+
+// Byte range of the streaming update's buffer descriptors: a state buffer's
+// three fields of every disk (3 ntot doubles) must lie within it.
+#define DFK_STREAM_BYTES 0x7ffffff0
 
 struct DiskDev {
   int Nu, Nphi, Nc, M;

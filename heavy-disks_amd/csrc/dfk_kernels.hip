@@ -35,19 +35,28 @@ typedef long long i64;
 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
-// 1/x to about 1 ulp: the hardware reciprocal estimate refined by two Newton
-// steps (x finite, nonzero and normal where it is used).
+// 1/x to about 1 ulp (x finite, nonzero and normal where it is used): the
+// hardware estimate r (relative error below 2^-24 on gfx950, measured over
+// 16.7M operands) times 1 + e + e^2 with e = 1 - x r, the cubically
+// convergent step (error ~e^3 < 2^-72 before the final rounding; it gave the
+// correctly rounded 1/x for every measured operand, as two Newton steps did,
+// in three FMAs instead of four).
 __device__ __forceinline__ double frcp(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
+  const double r = __builtin_amdgcn_rcp(x);
+  const double e = fma(-x, r, 1.0);
+  return fma(r, fma(e, e, e), r);
 }
 
 // hydro.hpp:16-18.  a*b > 0 keeps a+b away from zero and from denormals.
 __device__ __forceinline__ double vanleer(double a, double b) {
   return (a * b > 0.0) ? (2.0 * a * b) * frcp(a + b) : 0.0;
+}
+// The same value with the quotient formed on every lane and then selected,
+// so that no branch surrounds the reciprocal (where a*b <= 0 the quotient
+// may be inf or NaN; the select drops it).
+__device__ __forceinline__ double vanleer_sel(double a, double b) {
+  const double q = (2.0 * a * b) * frcp(a + b);
+  return (a * b > 0.0) ? q : 0.0;
 }
 
 // Barotropic EOS (hydro.hpp:44-45).  G2: every disk of the launch has
@@ -612,6 +621,30 @@ __device__ __forceinline__ void cfl_max(const DiskDev &D, double &mx, double s, 
     const double r = au / dR + aw / dl;
     mx = (r > mx) ? r : mx;
   }
+}
+
+// a / b correctly rounded, given ib = 1/b correctly rounded (the host's IEEE
+// 1.0 / b in the tables): q0 = a ib lies within an ulp of a / b, the
+// remainder a - q0 b is exact in one FMA, and q0 + r ib rounded is a / b
+// rounded (Markstein's theorem for binary64 with a fused multiply-add;
+// checked against IEEE division on the GPU for 16.7M random operands, and
+// the dt tests check it end to end).
+__device__ __forceinline__ double div_rn(double a, double b, double ib) {
+  const double q0 = a * ib;
+  return fma(fma(-q0, b, a), ib, q0);
+}
+
+// The CFL rate (|vR| + cs)/dR + (|vphi| + cs)/(R dphi) of one zone formed
+// exactly as hydro.hpp:52-57 forms it (the same sqrt, two correctly rounded
+// quotients, one sum), with no branch.  A zone with an infinite speed gives a
+// NaN remainder and rate, which the running maximum ignores (the reference's
+// quotient is inf there and its dt 0); such a zone raises the non-finite
+// flag.
+template <bool G2 = false>
+__device__ __forceinline__ double cfl_rate(const DiskDev &D, double s, double u, double w,
+                                           double invdR, double invdl, double dR, double dl) {
+  const double cs = sqrt(cs2f<G2>(D, smax(s, 1e-12)));
+  return div_rn(fabs(u) + cs, dR, invdR) + div_rn(fabs(w) + cs, dl, invdl);
 }
 
 __device__ __forceinline__ double nonfinite(double s, double u, double w) {
@@ -1773,10 +1806,47 @@ k_rows_fused(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, in
 // face) stays in registers, azimuthal neighbours -- of the source step's
 // pressure and potential as well as of the azimuthal sweep -- come from
 // cross-lane shifts (a 3-lane halo: the source values of lanes 1..62 are
-// exact, those of the edge lanes reach no output), and row tables are
-// wave-uniform.  Same arithmetic as k_update.
+// exact, those of the edge lanes reach no output), and the row tables are
+// wave-uniform scalar loads from per-row records.  The formulas of k_update,
+// arranged without branches (the results agree to round-off; dt is the
+// reference's exactly in both).
 #define STREAM_OUT 58
 #define STREAM_HALO 3
+
+// The streaming update reads and writes the state through buffer
+// descriptors.  One descriptor covers a disk's three fields of one state
+// buffer (they lie ntot doubles apart): the field and row go into the scalar
+// offset, the lane's column into the vector offset, so a load or store costs
+// no vector address arithmetic.  A lane with nothing to store (halo lanes,
+// columns past the row's end, rows outside the strip) passes kDropOffset as
+// its vector offset, and the hardware's range check discards the write: the
+// descriptors span DFK_STREAM_BYTES, every real element lies inside (3 ntot
+// doubles <= DFK_STREAM_BYTES, checked by dfk_set_update_kernel) and the drop
+// offset outside, whether or not the range check adds the scalar offset.  So
+// every store is issued by the whole wavefront outside any branch, and the
+// compiler can count the stores that follow a row's loads: the wait for the
+// next rows' loads no longer includes the completion of earlier stores.
+static constexpr int kDropOffset = 0x7fffffff;
+typedef unsigned int dfk_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t stream_rsrc(const double *p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(p), 0, DFK_STREAM_BYTES, 0x00020000);   // raw
+}
+__device__ __forceinline__ void store_or_drop(__amdgpu_buffer_rsrc_t r, int voff, int soff, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(dfk_u32x2, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ double load_at(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+// Geometry tables read through the constant address space: they are not
+// written while a loop kernel runs, and a load from that space stays a
+// scalar load even after the buffer stores of the same loop, which the
+// compiler cannot prove are elsewhere (a global-space load after them becomes
+// a vector load with a wait for it).
+typedef const __attribute__((address_space(4))) double ctab_t;
+__device__ __forceinline__ ctab_t *ctab(const double *geom, const DiskDev &D, int t) {
+  return (ctab_t *)gtab(geom, D, t);
+}
 
 // Whole-wavefront lane shifts by one through DPP (wave_shr:1 / wave_shl:1 on
 // each 32-bit half): lane L receives lane L-1 (up) or L+1 (down); the lane
@@ -1793,7 +1863,7 @@ __device__ __forceinline__ double dn1(double v) {
 }
 
 template <bool GARR, bool G2>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 4)   // at most 128 VGPRs: four waves per SIMD
 k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
                 const DiskCtl *__restrict__ ctl, double *__restrict__ st, i64 ntot,
                 const double *__restrict__ phi, const double *__restrict__ gRa,
@@ -1813,24 +1883,31 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   const int lane = threadIdx.x;
   int c = (j0 - STREAM_HALO + lane) % Nphi;
   if (c < 0) c += Nphi;
+  const unsigned cb = (unsigned)c * sizeof(double);   // the lane's byte offset in a row
   const bool out_ok = lane >= STREAM_HALO && lane < STREAM_HALO + STREAM_OUT &&
                       (j0 + lane - STREAM_HALO) < Nphi;
 
-  const double *S = stp(st, ntot, par, 0, D), *U = stp(st, ntot, par, 1, D),
-               *W = stp(st, ntot, par, 2, D);
-  double *So = stpw(st, ntot, 1 - par, 0, D), *Uo = stpw(st, ntot, 1 - par, 1, D),
-         *Wo = stpw(st, ntot, 1 - par, 2, D);
-  const double *Ph = phi + D.so;
-  const double *tInvdRc = gtab(geom, D, GT_INVDRC), *tInvR = gtab(geom, D, GT_INVR),
-               *tGstar = gtab(geom, D, GT_GSTAR), *tGrc = gtab(geom, D, GT_GRC),
-               *tGpc = gtab(geom, D, GT_GPC), *tInvdR = gtab(geom, D, GT_INVDR),
-               *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL),
-               *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE),
-               *tdA = gtab(geom, D, GT_DA), *tDl = gtab(geom, D, GT_DL);
+  // the state buffer read (fields Sigma, v_R, v_phi at byte offsets 0, fb,
+  // 2 fb), the one written, and the potential
+  const __amdgpu_buffer_rsrc_t rIn = stream_rsrc(stp(st, ntot, par, 0, D)),
+                               rOut = stream_rsrc(stpw(st, ntot, 1 - par, 0, D)),
+                               rP = stream_rsrc(phi + D.so);
+  const int fb = (int)(ntot * (i64)sizeof(double));
+  // row records: rec(i)[RR_*] (a scalar pointer per row, immediate offsets)
+  ctab_t *rec0 = ctab(geom, D, GT_COUNT);
+  auto rec = [&](int i) { return rec0 + i * RR_STRIDE; };
   const double c2j = trig[D.cos_off + c], s2j = trig[D.cos_off + Nphi + c];
+  // self-gravity terms are formed on every disk and kept by a select (Phi
+  // of a disk without self-gravity may hold anything; the select drops it),
+  // so that one build serves every handle and a disk's bits do not depend on
+  // the other disks of its handle
   const bool sg = !GARR && D.selfgrav != 0;
 
   auto clampr = [&](int r) { return r < 0 ? 0 : (r >= Nu ? Nu - 1 : r); };
+  // row r's element of this lane (r clamped into the disk)
+  const int rowb = Nphi * (int)sizeof(double);
+  auto at = [&](int field, int r) { return load_at(rIn, cb, field * fb + clampr(r) * rowb); };
+  auto atP = [&](int r) { return load_at(rP, cb, clampr(r) * rowb); };
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 
   // Even strips walk down (rows i0-2 .. i1+1), odd strips walk up (i1+1 ..
@@ -1838,73 +1915,74 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   // time -- both at their start or both at their end -- and the second read
   // of those rows hits L2.  The arithmetic of every face and cell is the
   // same either way; only the order of the partial sums differs.
+  //
+  // Per source row s: the source step (forces, pressure, geometry) on the
+  // row; then the radial face f whose four cells are now post-source (f =
+  // s-1 walking down, s+2 walking up); then the gather, azimuthal sweep and
+  // output of row g, whose two radial faces are now known (g = f-1 down, f
+  // up).  The first four rows of a strip only fill the stencil (sources,
+  // then the first face); after them every row does all three.  That steady
+  // part has no branches: rows past the disk's edge are read clamped and
+  // give finite values that no output uses (the edge faces have zero area,
+  // the edge cells a zero MUSCL factor, from the tables GT_AFZ, GT_HUP,
+  // GT_HDN), and a row or lane without output stores to kDropOffset and adds
+  // nothing to the CFL maximum and the sums.
   auto strip = [&](auto down_tag) {
     constexpr bool DOWN = decltype(down_tag)::value;
     constexpr int STEP = DOWN ? 1 : -1;
-    const int sfirst = DOWN ? i0 - 2 : i1 + 1;   // the last row is i1 + 1 (down) or i0 - 2 (up)
-    // the row beyond the first one on the far side, and the first row
-    double Sb = S[(i64)clampr(sfirst - STEP) * Nphi + c], S0 = S[(i64)clampr(sfirst) * Nphi + c];
-    double Pb = 0.0, P0 = 0.0;
-    if (sg) {
-      Pb = Ph[(i64)clampr(sfirst - STEP) * Nphi + c];
-      P0 = Ph[(i64)clampr(sfirst) * Nphi + c];
-    }
+    const int sfirst = DOWN ? i0 - 2 : i1 + 1;
+    // Sigma and Phi windows: the row behind (b), this row (0); pressures
+    // P(Sigma) are formed once per row as it arrives and shifted across lanes
+    // (the same values the reference forms at each neighbour)
+    double S0 = at(0, sfirst);
+    double Prb = pres<G2>(D, at(0, sfirst - STEP)), Pr0 = pres<G2>(D, S0);
+    double Pb = atP(sfirst - STEP), P0 = atP(sfirst);
     // post-source ring over rows f-2 .. f+1 of the face f in hand: slot k
     // holds row f-2+k (rows enter at slot 3 walking down, at slot 0 walking up)
     double qs0 = 0, qs1 = 0, qs2 = 0, qs3 = 0, qu0 = 0, qu1 = 0, qu2 = 0, qu3 = 0;
     double qw0 = 0, qw1 = 0, qw2 = 0, qw3 = 0;
     double pfm = 0, pfu = 0, pfw = 0;     // the face met before this one
 
-    // loads of the next row are issued one iteration ahead (rows clamped
-    // into the disk; values of rows outside it are never used)
-    double nU, nW, nSa, nPa = 0.0;
+    // fetch(r): v_R, v_phi of row r and Sigma, Phi of row r + STEP, issued
+    // one row ahead (two rows ahead measured no faster and costs 8 VGPRs)
+    double nU, nW, nSa, nPa;
     auto fetch = [&](int r) {
-      const i64 ro = (i64)clampr(r) * Nphi, ra = (i64)clampr(r + STEP) * Nphi;
-      nU = U[ro + c];
-      nW = W[ro + c];
-      nSa = S[ra + c];
-      if (sg) nPa = Ph[ra + c];
+      nU = at(1, r);
+      nW = at(2, r);
+      nSa = at(0, r + STEP);
+      nPa = atP(r + STEP);     // the potential pool covers every disk: loaded whether used or not
     };
-    // rows in groups of four, so that the unrolled body renames the ring
-    // instead of moving it; the up to three rows past the last one compute
-    // nothing that is kept (their faces fall outside [i0, i1])
-    const int ngrp = (i1 - i0 + 4 + 3) >> 2;
     fetch(sfirst);
     int s = sfirst;
-    for (int grp = 0; grp < ngrp; ++grp)
-#pragma unroll
-    for (int k = 0; k < 4; ++k, s += STEP) {
-      // row s + STEP (ahead) and row s - STEP (behind)
+
+    // ---- source at row s (hydro.hpp:73-96, forces kalnajs.hpp:65-80), pushed into the ring
+    auto source_row = [&]() {
       const double Sa = nSa, Pa = nPa, u = nU, w = nW;
       fetch(s + STEP);
-      // azimuthal neighbours of row s from the neighbouring lanes (the edge
-      // lanes 0 and 63 get zeros; their results reach no output lane)
-      const double Se = dn1(S0), Sw = up1(S0);
-      const double Pe = sg ? dn1(P0) : 0.0, Pw = sg ? up1(P0) : 0.0;
-      const double Sp = DOWN ? Sa : Sb, Sm = DOWN ? Sb : Sa;
-      const double Pp = DOWN ? Pa : Pb, Pm = DOWN ? Pb : Pa;
-      // ---- source at row s (hydro.hpp:73-96, forces kalnajs.hpp:65-80)
-      double ns_s = 0.0, ns_u = 0.0, ns_w = 0.0;
-      if (s >= 0 && s < Nu) {
-        const i64 ro = (i64)s * Nphi;
-        const double dPdR = (pres<G2>(D, Sp) - pres<G2>(D, Sm)) * tInvdRc[s];
-        const double dPdphi = (pres<G2>(D, Se) - pres<G2>(D, Sw)) * D.inv2dphi;
-        double gR = tGstar[s], gp = 0.0;
-        if (GARR) {
-          if (has_gR) gR += gRa[D.so + ro + c];
-          if (has_gp) gp += gpa[D.so + ro + c];
-        } else if (sg) {
-          gR += -(Pp - Pm) * tGrc[s];
-          gp += -(Pe - Pw) * tGpc[s];
-        }
-        const double rs = frcp(S0);
-        const double invR = tInvR[s];
-        const double aR = -dPdR * rs + gR + w * w * invR;
-        const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
-        ns_s = S0;
-        ns_u = u + dt * aR;
-        ns_w = w + dt * ap;
+      const double Pra = pres<G2>(D, Sa);
+      ctab_t *rs_ = rec(clampr(s));
+      const double Prp = DOWN ? Pra : Prb, Prm = DOWN ? Prb : Pra;
+      const double dPdR = (Prp - Prm) * rs_[RR_INVDRC];
+      // azimuthal neighbours from the neighbouring lanes (the edge lanes 0
+      // and 63 get zeros; their results reach no output lane)
+      const double dPdphi = (dn1(Pr0) - up1(Pr0)) * D.inv2dphi;
+      double gR = rs_[RR_GSTAR], gp = 0.0;
+      if (GARR) {
+        const i64 ro = (i64)clampr(s) * Nphi + c;
+        if (has_gR) gR += gRa[D.so + ro];
+        if (has_gp) gp += gpa[D.so + ro];
+      } else {
+        const double Pp = DOWN ? Pa : Pb, Pm = DOWN ? Pb : Pa;
+        const double gRs = gR + -(Pp - Pm) * rs_[RR_GRC];
+        const double gps = gp + -(dn1(P0) - up1(P0)) * rs_[RR_GPC];
+        gR = sg ? gRs : gR;
+        gp = sg ? gps : gp;
       }
+      const double rs = frcp(S0);
+      const double invR = rs_[RR_INVR];
+      const double aR = -dPdR * rs + gR + w * w * invR;
+      const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
+      const double ns_s = S0, ns_u = u + dt * aR, ns_w = w + dt * ap;
       if (DOWN) {
         qs0 = qs1; qs1 = qs2; qs2 = qs3; qs3 = ns_s;
         qu0 = qu1; qu1 = qu2; qu2 = qu3; qu3 = ns_u;
@@ -1914,84 +1992,109 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
         qu3 = qu2; qu2 = qu1; qu1 = qu0; qu0 = ns_u;
         qw3 = qw2; qw2 = qw1; qw1 = qw0; qw0 = ns_w;
       }
-      Sb = S0; S0 = Sa;
+      Prb = Pr0; Pr0 = Pra; S0 = Sa;
       Pb = P0; P0 = Pa;
-      // ---- radial face f between cells f-1 (slot 1) and f (slot 2) (hydro.hpp:111-130)
-      const int f = DOWN ? s - 1 : s + 2;
-      double fm = 0.0, fu = 0.0, fw = 0.0;
-      if (f >= i0 && f <= i1) {
-        if (f >= 1 && f <= Nu - 1) {
-          const double vf = 0.5 * (qu1 + qu2);
-          const bool up = vf > 0;
-          const int cell = up ? f - 1 : f;
-          const double bs = up ? qs1 : qs2, bu = up ? qu1 : qu2, bw = up ? qw1 : qw2;
-          const double ls = up ? qs0 : qs1, lu = up ? qu0 : qu1, lw = up ? qw0 : qw1;
-          const double hs = up ? qs2 : qs3, hu = up ? qu2 : qu3, hw = up ? qw2 : qw3;
-          const double fac = up ? 0.5 * (1.0 - vf * dt * tInvdR[f - 1])
-                                : -0.5 * (1.0 + vf * dt * tInvdR[f]);
-          double ss = bs, su = bu, sw = bw;
-          if (cell > 0 && cell < Nu - 1) {
-            ss += vanleer(hs - bs, bs - ls) * fac;
-            su += vanleer(hu - bu, bu - lu) * fac;
-            sw += vanleer(hw - bw, bw - lw) * fac;
-          }
-          fm = vf * ss * tAf[f];
-          fu = fm * su;
-          fw = fm * sw;
-        }
-        // ---- radial gather of the row g whose two faces are now known
-        // (faces g and g+1; walking down g = f-1, walking up g = f), then the
-        // azimuthal sweep
-        const int g = DOWN ? f - 1 : f;
-        if (g >= i0 && g < i1) {
-          const double s_ = DOWN ? qs1 : qs2, u_ = DOWN ? qu1 : qu2, w_ = DOWN ? qw1 : qw2;
-          const double flo = DOWN ? pfm : fm, fhi = DOWN ? fm : pfm;
-          const double ulo = DOWN ? pfu : fu, uhi = DOWN ? fu : pfu;
-          const double wlo = DOWN ? pfw : fw, whi = DOWN ? fw : pfw;
-          const double iv = dt * tInvVol[g];
-          const double ns = s_ + (flo - fhi) * iv;
-          const double nmR = s_ * u_ + (ulo - uhi) * iv;
-          const double nmP = s_ * w_ + (wlo - whi) * iv;
-          const double rn = frcp(smax(ns, 1e-10));
-          const double Xs = ns, Xu = nmR * rn, Xw = nmP * rn;
-          // azimuthal face at this lane's left edge (hydro.hpp:153-168)
-          const double sm1 = up1(Xs), um1 = up1(Xu), wm1 = up1(Xw);
-          const double sm2 = up1(sm1), um2 = up1(um1), wm2 = up1(wm1);
-          const double sp1 = dn1(Xs), vp1 = dn1(Xu), wp1 = dn1(Xw);
-          const double vf = 0.5 * (wm1 + Xw);
-          const bool upw = vf > 0;
-          const double bs = upw ? sm1 : Xs, bu = upw ? um1 : Xu, bw = upw ? wm1 : Xw;
-          const double ls = upw ? sm2 : sm1, lu = upw ? um2 : um1, lw = upw ? wm2 : wm1;
-          const double hs = upw ? Xs : sp1, hu = upw ? Xu : vp1, hw = upw ? Xw : wp1;
-          const double fac = upw ? 0.5 * (1.0 - vf * dt * tInvdl[g])
-                                 : -0.5 * (1.0 + vf * dt * tInvdl[g]);
-          const double ss = bs + vanleer(hs - bs, bs - ls) * fac;
-          const double su = bu + vanleer(hu - bu, bu - lu) * fac;
-          const double sw = bw + vanleer(hw - bw, bw - lw) * fac;
-          const double Fm = vf * ss * tAface[g];
-          const double FU = Fm * su, FW = Fm * sw;
-          const double Fm1 = dn1(Fm), FU1 = dn1(FU), FW1 = dn1(FW);
-          // azimuthal gather + floor (hydro.hpp:170-178, 68-69)
-          const double ns2 = Xs + (Fm - Fm1) * iv;
-          const double nmR2 = Xs * Xu + (FU - FU1) * iv;
-          const double nmP2 = Xs * Xw + (FW - FW1) * iv;
-          const double rn2 = frcp(smax(ns2, 1e-10));
-          const double so = smax(ns2, 1e-10), uo = nmR2 * rn2, wo = nmP2 * rn2;
-          if (out_ok) {
-            const i64 o = (i64)g * Nphi + c;
-            So[o] = so;
-            Uo[o] = uo;
-            Wo[o] = wo;
-            cfl_max<G2>(D, mx, so, uo, wo, tInvdR[g], tInvdl[g], tAface[g], tDl[g]);
-            a0 += so * tdA[g];
-            a1 += so * c2j * tdA[g];
-            a2 -= so * s2j * tdA[g];
-            a3 += nonfinite(so, uo, wo);
-          }
-        }
+    };
+    // ---- radial face f between cells f-1 (slot 1) and f (slot 2) (hydro.hpp:111-130)
+    auto radial_face = [&](int f, double &fm, double &fu, double &fw) {
+      const int fc = min(max(f, 0), Nu);     // faces 0 .. Nu (past them only in masked rows)
+      const int fl = max(fc - 1, 0), fr = min(fc, Nu - 1);
+      const double vf = 0.5 * (qu1 + qu2);
+      const bool up = vf > 0;
+      const double bs = up ? qs1 : qs2, bu = up ? qu1 : qu2, bw = up ? qw1 : qw2;
+      const double ls = up ? qs0 : qs1, lu = up ? qu0 : qu1, lw = up ? qw0 : qw1;
+      const double hs = up ? qs2 : qs3, hu = up ? qu2 : qu3, hw = up ? qw2 : qw3;
+      // 0.5 (1 - vf dt/dR_{f-1}) upwind left, -0.5 (1 + vf dt/dR_f) upwind
+      // right, 0 for a cell at the disk's edge (no slope there)
+      // both sides' table values are loaded before the choice (scalar
+      // loads; a choice between two loads would become one vector load
+      // from a per-lane address, a dependent memory wait in every row)
+      ctab_t *rl = rec(fl), *rr = rec(fr);
+      const double iDl = rl[RR_INVDR], iDr = rr[RR_INVDR], hl = rl[RR_HUP], hr = rr[RR_HDN];
+      const double iD = up ? iDl : iDr;
+      const double hh = up ? hl : hr;
+      const double fac = hh * fma(-(fabs(vf) * dt), iD, 1.0);
+      const double ss = bs + vanleer_sel(hs - bs, bs - ls) * fac;
+      const double su = bu + vanleer_sel(hu - bu, bu - lu) * fac;
+      const double sw = bw + vanleer_sel(hw - bw, bw - lw) * fac;
+      fm = vf * ss * rec(fc)[RR_AFZ];
+      fu = fm * su;
+      fw = fm * sw;
+    };
+    // ---- radial gather of row g (faces g and g+1), azimuthal sweep, floor, output
+    auto gather_row = [&](int g, double flo, double fhi, double ulo, double uhi, double wlo,
+                          double whi) {
+      ctab_t *rg = rec(clampr(g));
+      const double s_ = DOWN ? qs1 : qs2, u_ = DOWN ? qu1 : qu2, w_ = DOWN ? qw1 : qw2;
+      const double iv = dt * rg[RR_INVVOL];
+      const double ns = s_ + (flo - fhi) * iv;
+      const double nmR = s_ * u_ + (ulo - uhi) * iv;
+      const double nmP = s_ * w_ + (wlo - whi) * iv;
+      const double rn = frcp(smax(ns, 1e-10));
+      const double Xs = ns, Xu = nmR * rn, Xw = nmP * rn;
+      // azimuthal face at this lane's left edge (hydro.hpp:153-168)
+      const double sm1 = up1(Xs), um1 = up1(Xu), wm1 = up1(Xw);
+      const double sm2 = up1(sm1), um2 = up1(um1), wm2 = up1(wm1);
+      const double sp1 = dn1(Xs), vp1 = dn1(Xu), wp1 = dn1(Xw);
+      const double vf = 0.5 * (wm1 + Xw);
+      const bool upw = vf > 0;
+      const double bs = upw ? sm1 : Xs, bu = upw ? um1 : Xu, bw = upw ? wm1 : Xw;
+      const double ls = upw ? sm2 : sm1, lu = upw ? um2 : um1, lw = upw ? wm2 : wm1;
+      const double hs = upw ? Xs : sp1, hu = upw ? Xu : vp1, hw = upw ? Xw : wp1;
+      const double fac = (upw ? 0.5 : -0.5) * fma(-(fabs(vf) * dt), rg[RR_INVDL], 1.0);
+      const double ss = bs + vanleer_sel(hs - bs, bs - ls) * fac;
+      const double su = bu + vanleer_sel(hu - bu, bu - lu) * fac;
+      const double sw = bw + vanleer_sel(hw - bw, bw - lw) * fac;
+      const double Fm = vf * ss * rg[RR_AFACE];
+      const double FU = Fm * su, FW = Fm * sw;
+      const double Fm1 = dn1(Fm), FU1 = dn1(FU), FW1 = dn1(FW);
+      // azimuthal gather + floor (hydro.hpp:170-178, 68-69)
+      const double ns2 = Xs + (Fm - Fm1) * iv;
+      const double nmR2 = Xs * Xu + (FU - FU1) * iv;
+      const double nmP2 = Xs * Xw + (FW - FW1) * iv;
+      const double rn2 = frcp(smax(ns2, 1e-10));
+      const double so = smax(ns2, 1e-10), uo = nmR2 * rn2, wo = nmP2 * rn2;
+      // (bitwise, not short-circuit: no branch)
+      const bool ok = out_ok & ((g >= i0) & (g < i1));
+      const int voff = ok ? (int)cb : kDropOffset, roff = clampr(g) * rowb;
+      store_or_drop(rOut, voff, roff, so);
+      store_or_drop(rOut, voff, fb + roff, uo);
+      store_or_drop(rOut, voff, 2 * fb + roff, wo);
+      // the new state's CFL rate (exact, hydro.hpp:52-57) and C_0, C_2 sums
+      const double rate = cfl_rate<G2>(D, so, uo, wo, rg[RR_INVDR], rg[RR_INVDL], rg[RR_AFACE], rg[RR_DL]);
+      mx = (ok & (rate > mx)) ? rate : mx;
+      const double m = ok ? so : 0.0;
+      const double dA = rg[RR_DA];
+      a0 += m * dA;
+      a1 += m * c2j * dA;
+      a2 -= m * s2j * dA;
+      a3 += (ok & (nonfinite(so, uo, wo) != 0.0)) ? 1.0 : 0.0;
+    };
+
+    // stencil fill: sources of the first three rows, then the fourth row's
+    // source and the strip's first face (f = i0 down, i1 up)
+#pragma unroll
+    for (int k = 0; k < 3; ++k, s += STEP) source_row();
+    source_row();
+    radial_face(DOWN ? s - 1 : s + 2, pfm, pfu, pfw);
+    s += STEP;
+    // steady rows in groups of four, so that the unrolled body renames the
+    // ring instead of moving it; the rows past a short strip's end compute
+    // nothing that is kept
+    const int ngrp = (i1 - i0 + 3) >> 2;
+    for (int grp = 0; grp < ngrp; ++grp)
+#pragma unroll
+      for (int k = 0; k < 4; ++k, s += STEP) {
+        source_row();
+        const int f = DOWN ? s - 1 : s + 2;
+        double fm, fu, fw;
+        radial_face(f, fm, fu, fw);
+        if (DOWN)
+          gather_row(f - 1, pfm, fm, pfu, fu, pfw, fw);
+        else
+          gather_row(f, fm, pfm, fu, pfu, fw, pfw);
         pfm = fm; pfu = fu; pfw = fw;
       }
-    }
   };
   if (ks & 1)
     strip(std::integral_constant<bool, false>{});

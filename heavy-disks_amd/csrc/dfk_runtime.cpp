@@ -703,7 +703,7 @@ int diag_disk(dfk_handle *h, int d, double *mx, double sums[4]) {
 void host_tables(const DiskDev &D, const dfk_disk_params &p, std::vector<double> &geom,
                  std::vector<double> &trig) {
   const int Nu = D.Nu, Nphi = D.Nphi, L = Nu + 1;
-  geom.assign((size_t)GT_COUNT * L, 0.0);
+  geom.assign((size_t)DFK_GEOM_DOUBLES(Nu), 0.0);
   auto T = [&](int t, int i) -> double & { return geom[(size_t)t * L + i]; };
   const double umin = std::log(p.Rin);
   const double du = D.du, dphi = D.dphi;
@@ -714,6 +714,10 @@ void host_tables(const DiskDev &D, const dfk_disk_params &p, std::vector<double>
   for (int i = 0; i <= Nu; ++i) {
     T(GT_RF, i) = Rf[i];
     T(GT_AF, i) = Rf[i] * dphi;                                                // hydro.hpp:113
+    T(GT_AFZ, i) = (i >= 1 && i <= Nu - 1) ? T(GT_AF, i) : 0.0;               // hydro.hpp:111
+    const bool inner = i >= 1 && i <= Nu - 2;                                  // hydro.hpp:118-123
+    T(GT_HUP, i) = inner ? 0.5 : 0.0;
+    T(GT_HDN, i) = inner ? -0.5 : 0.0;
   }
   for (int i = 0; i < Nu; ++i) {
     const int im = std::max(i - 1, 0), ip = std::min(i + 1, Nu - 1);
@@ -733,6 +737,17 @@ void host_tables(const DiskDev &D, const dfk_disk_params &p, std::vector<double>
     T(GT_GRC, i) = (i == 0 || i == Nu - 1) ? 1.0 / (du * R[i]) : 1.0 / (2.0 * du * R[i]);  // kalnajs.hpp:74-77
     T(GT_GPC, i) = 1.0 / (2.0 * dphi * R[i]);                                   // kalnajs.hpp:72,78
     T(GT_DL, i) = R[i] * dphi;                                                  // hydro.hpp:56
+  }
+  // row records (the same values, gathered per row)
+  for (int i = 0; i <= Nu; ++i) {
+    double *r = &geom[(size_t)GT_COUNT * L + (size_t)i * RR_STRIDE];
+    r[RR_AFZ] = T(GT_AFZ, i);
+    if (i == Nu) continue;
+    r[RR_INVDRC] = T(GT_INVDRC, i); r[RR_INVR] = T(GT_INVR, i); r[RR_GSTAR] = T(GT_GSTAR, i);
+    r[RR_GRC] = T(GT_GRC, i); r[RR_GPC] = T(GT_GPC, i);
+    r[RR_INVDR] = T(GT_INVDR, i); r[RR_HUP] = T(GT_HUP, i); r[RR_HDN] = T(GT_HDN, i);
+    r[RR_INVVOL] = T(GT_INVVOL, i); r[RR_INVDL] = T(GT_INVDL, i); r[RR_AFACE] = T(GT_AFACE, i);
+    r[RR_DA] = T(GT_DA, i); r[RR_DL] = T(GT_DL, i);
   }
   trig.assign((size_t)2 * Nphi, 0.0);
   for (int j = 0; j < Nphi; ++j) {
@@ -899,7 +914,7 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
     D.so = so; so += (i64)p.Nu * p.Nphi;
     D.spo = spo; spo += (i64)p.Nu * D.Nc;
     D.kho = kho; kho += (i64)(p.Nu + 1) * D.Nc;
-    D.go = go; go += (i64)GT_COUNT * (p.Nu + 1);
+    D.go = go; go += DFK_GEOM_DOUBLES(p.Nu);
     D.cos_off = (int)co; co += 2 * (i64)p.Nphi;
     h->maxN = std::max(h->maxN, (i64)p.Nu * p.Nphi);
     h->maxF = std::max(h->maxF, (i64)(p.Nu + 1) * p.Nphi);
@@ -1520,6 +1535,12 @@ int dfk_get_fused_rows(dfk_handle *h, int *threads) {
 
 int dfk_set_update_kernel(dfk_handle *h, int kind) {
   if (!h || kind < 0 || kind > 1) { set_err("update kernel must be 0 (LDS tile) or 1 (streaming)"); return DFK_EINVAL; }
+  if (kind == 1 && 3 * h->ntot * (i64)sizeof(double) > DFK_STREAM_BYTES) {
+    set_err("streaming update: the state of this handle (%lld zones) exceeds its 31-bit buffer offsets "
+            "(at most %lld zones); use the LDS-tile update", (long long)h->ntot,
+            (long long)(DFK_STREAM_BYTES / (3 * sizeof(double))));
+    return DFK_EUNSUPPORTED;
+  }
   if (kind != h->upd_kind) {
     h->upd_kind = kind;
     for (auto &D : h->hD) {
