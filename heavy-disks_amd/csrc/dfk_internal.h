@@ -40,8 +40,8 @@ enum {
   GT_GPC,        // 1/(2 dphi R_i)                                  (self-gravity g_phi)
   GT_DL,         // R_i dphi  (CFL azimuthal length, formed as hydro.hpp:56 forms it)
   GT_AFZ,        // Rf_i dphi at the interior faces 1..Nu-1, 0 at faces 0 and Nu (no flux)
-  GT_HUP,        // 0.5 for the interior cells 1..Nu-2, 0 at cells 0 and Nu-1 (no slope):
-  GT_HDN,        //   MUSCL factor of an upwind cell left (HUP) / right (HDN, -0.5) of its face
+  GT_HUP,        // 1 for the interior cells 1..Nu-2, 0 at cells 0 and Nu-1 (no slope): sign
+  GT_HDN,        //   of the MUSCL factor of an upwind cell left (HUP) / right (HDN, -1) of its face
   GT_COUNT
 };
 

@@ -51,12 +51,16 @@ __device__ __forceinline__ double frcp(double x) {
 __device__ __forceinline__ double vanleer(double a, double b) {
   return (a * b > 0.0) ? (2.0 * a * b) * frcp(a + b) : 0.0;
 }
-// The same value with the quotient formed on every lane and then selected,
-// so that no branch surrounds the reciprocal (where a*b <= 0 the quotient
-// may be inf or NaN; the select drops it).
-__device__ __forceinline__ double vanleer_sel(double a, double b) {
-  const double q = (2.0 * a * b) * frcp(a + b);
-  return (a * b > 0.0) ? q : 0.0;
+// Half the van Leer slope of a cell, vanleer(hi - mid, mid - lo) / 2, with
+// the quotient formed on every lane and then selected, so that no branch
+// surrounds the reciprocal (where a*b <= 0 the quotient may be inf or NaN;
+// the select drops it).  It is the reference's value halved exactly:
+// RN(2 RN(ab) r) = 2 RN(RN(ab) r) with r = frcp(a + b).
+__device__ __forceinline__ double half_slope(double lo, double mid, double hi) {
+  const double a = hi - mid, b = mid - lo;
+  const double p = a * b;
+  const double q = p * frcp(a + b);
+  return (p > 0.0) ? q : 0.0;
 }
 
 // Barotropic EOS (hydro.hpp:44-45).  G2: every disk of the launch has
@@ -1927,6 +1931,14 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   // the edge cells a zero MUSCL factor, from the tables GT_AFZ, GT_HUP,
   // GT_HDN), and a row or lane without output stores to kDropOffset and adds
   // nothing to the CFL maximum and the sums.
+  //
+  // MUSCL face values (hydro.hpp:107-124, 149-163): a face takes q + d F from
+  // its upwind cell, d the cell's half van Leer slope (half_slope) and F =
+  // +-(1 -+ vf dt/dx) twice the reference's factor +-0.5 (1 -+ vf dt/dx), or 0
+  // at a cell on the disk's edge: the same real product as the reference's
+  // vanleer * factor, rounded once in the FMA.  Azimuthally every lane forms
+  // its own cell's slope and takes its left neighbour's by a lane shift;
+  // radially the face forms the slope of its upwind cell from the ring.
   auto strip = [&](auto down_tag) {
     constexpr bool DOWN = decltype(down_tag)::value;
     constexpr int STEP = DOWN ? 1 : -1;
@@ -2004,19 +2016,19 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
       const double bs = up ? qs1 : qs2, bu = up ? qu1 : qu2, bw = up ? qw1 : qw2;
       const double ls = up ? qs0 : qs1, lu = up ? qu0 : qu1, lw = up ? qw0 : qw1;
       const double hs = up ? qs2 : qs3, hu = up ? qu2 : qu3, hw = up ? qw2 : qw3;
-      // 0.5 (1 - vf dt/dR_{f-1}) upwind left, -0.5 (1 + vf dt/dR_f) upwind
-      // right, 0 for a cell at the disk's edge (no slope there)
-      // both sides' table values are loaded before the choice (scalar
-      // loads; a choice between two loads would become one vector load
-      // from a per-lane address, a dependent memory wait in every row)
+      // F = 1 - vf dt/dR_{f-1} upwind left, -(1 + vf dt/dR_f) upwind right,
+      // 0 for a cell at the disk's edge (no slope there); both sides' table
+      // values are loaded before the choice (scalar loads; a choice between
+      // two loads would become one vector load from a per-lane address, a
+      // dependent memory wait in every row)
       ctab_t *rl = rec(fl), *rr = rec(fr);
       const double iDl = rl[RR_INVDR], iDr = rr[RR_INVDR], hl = rl[RR_HUP], hr = rr[RR_HDN];
       const double iD = up ? iDl : iDr;
       const double hh = up ? hl : hr;
-      const double fac = hh * fma(-(fabs(vf) * dt), iD, 1.0);
-      const double ss = bs + vanleer_sel(hs - bs, bs - ls) * fac;
-      const double su = bu + vanleer_sel(hu - bu, bu - lu) * fac;
-      const double sw = bw + vanleer_sel(hw - bw, bw - lw) * fac;
+      const double F = hh * fma(-(fabs(vf) * dt), iD, 1.0);
+      const double ss = fma(half_slope(ls, bs, hs), F, bs);
+      const double su = fma(half_slope(lu, bu, hu), F, bu);
+      const double sw = fma(half_slope(lw, bw, hw), F, bw);
       fm = vf * ss * rec(fc)[RR_AFZ];
       fu = fm * su;
       fw = fm * sw;
@@ -2032,19 +2044,17 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
       const double nmP = s_ * w_ + (wlo - whi) * iv;
       const double rn = frcp(smax(ns, 1e-10));
       const double Xs = ns, Xu = nmR * rn, Xw = nmP * rn;
-      // azimuthal face at this lane's left edge (hydro.hpp:153-168)
+      // azimuthal face at this lane's left edge (hydro.hpp:153-168): this
+      // lane's cell slope, the left neighbour's value and slope by shifts
       const double sm1 = up1(Xs), um1 = up1(Xu), wm1 = up1(Xw);
-      const double sm2 = up1(sm1), um2 = up1(um1), wm2 = up1(wm1);
-      const double sp1 = dn1(Xs), vp1 = dn1(Xu), wp1 = dn1(Xw);
+      const double ds = half_slope(sm1, Xs, dn1(Xs)), du = half_slope(um1, Xu, dn1(Xu)),
+                   dw = half_slope(wm1, Xw, dn1(Xw));
       const double vf = 0.5 * (wm1 + Xw);
       const bool upw = vf > 0;
-      const double bs = upw ? sm1 : Xs, bu = upw ? um1 : Xu, bw = upw ? wm1 : Xw;
-      const double ls = upw ? sm2 : sm1, lu = upw ? um2 : um1, lw = upw ? wm2 : wm1;
-      const double hs = upw ? Xs : sp1, hu = upw ? Xu : vp1, hw = upw ? Xw : wp1;
-      const double fac = (upw ? 0.5 : -0.5) * fma(-(fabs(vf) * dt), rg[RR_INVDL], 1.0);
-      const double ss = bs + vanleer_sel(hs - bs, bs - ls) * fac;
-      const double su = bu + vanleer_sel(hu - bu, bu - lu) * fac;
-      const double sw = bw + vanleer_sel(hw - bw, bw - lw) * fac;
+      const double F = (upw ? 1.0 : -1.0) * fma(-(fabs(vf) * dt), rg[RR_INVDL], 1.0);
+      const double ss = fma(upw ? up1(ds) : ds, F, upw ? sm1 : Xs);
+      const double su = fma(upw ? up1(du) : du, F, upw ? um1 : Xu);
+      const double sw = fma(upw ? up1(dw) : dw, F, upw ? wm1 : Xw);
       const double Fm = vf * ss * rg[RR_AFACE];
       const double FU = Fm * su, FW = Fm * sw;
       const double Fm1 = dn1(Fm), FU1 = dn1(FU), FW1 = dn1(FW);

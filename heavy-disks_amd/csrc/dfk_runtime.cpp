@@ -716,8 +716,8 @@ void host_tables(const DiskDev &D, const dfk_disk_params &p, std::vector<double>
     T(GT_AF, i) = Rf[i] * dphi;                                                // hydro.hpp:113
     T(GT_AFZ, i) = (i >= 1 && i <= Nu - 1) ? T(GT_AF, i) : 0.0;               // hydro.hpp:111
     const bool inner = i >= 1 && i <= Nu - 2;                                  // hydro.hpp:118-123
-    T(GT_HUP, i) = inner ? 0.5 : 0.0;
-    T(GT_HDN, i) = inner ? -0.5 : 0.0;
+    T(GT_HUP, i) = inner ? 1.0 : 0.0;
+    T(GT_HDN, i) = inner ? -1.0 : 0.0;
   }
   for (int i = 0; i < Nu; ++i) {
     const int im = std::max(i - 1, 0), ip = std::min(i + 1, Nu - 1);
