@@ -1949,20 +1949,27 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
     double S0 = at(0, sfirst);
     double Prb = pres<G2>(D, at(0, sfirst - STEP)), Pr0 = pres<G2>(D, S0);
     double Pb = atP(sfirst - STEP), P0 = atP(sfirst);
-    // post-source ring over rows f-2 .. f+1 of the face f in hand: slot k
-    // holds row f-2+k (rows enter at slot 3 walking down, at slot 0 walking up)
-    double qs0 = 0, qs1 = 0, qs2 = 0, qs3 = 0, qu0 = 0, qu1 = 0, qu2 = 0, qu3 = 0;
-    double qw0 = 0, qw1 = 0, qw2 = 0, qw3 = 0;
+    // post-source ring of three rows in increasing row order: after the
+    // source of row s, slot k holds row s-2+k walking down (rows enter at
+    // slot 2), row s+k walking up (rows enter at slot 0)
+    double qs0 = 0, qs1 = 0, qs2 = 0, qu0 = 0, qu1 = 0, qu2 = 0, qw0 = 0, qw1 = 0, qw2 = 0;
+    // half van Leer slopes: *n of the middle slot's cell (formed after this
+    // row's source), *c of the cell formed one row before (slot 0 walking
+    // down, slot 2 walking up)
+    double dsn = 0, dun = 0, dwn = 0, dsc = 0, duc = 0, dwc = 0;
     double pfm = 0, pfu = 0, pfw = 0;     // the face met before this one
 
     // fetch(r): v_R, v_phi of row r and Sigma, Phi of row r + STEP, issued
-    // one row ahead (two rows ahead measured no faster and costs 8 VGPRs)
+    // one row ahead (two rows ahead measured no faster and costs 8 VGPRs);
+    // past the strip's last source row (i1+1 down, i0-2 up) the fetch repeats
+    // that row's loads (cache hits), so no row beyond the stencil is read
     double nU, nW, nSa, nPa;
     auto fetch = [&](int r) {
-      nU = at(1, r);
-      nW = at(2, r);
-      nSa = at(0, r + STEP);
-      nPa = atP(r + STEP);     // the potential pool covers every disk: loaded whether used or not
+      const int rl = DOWN ? min(r, i1 + 1) : max(r, i0 - 2);
+      nU = at(1, rl);
+      nW = at(2, rl);
+      nSa = at(0, rl + STEP);
+      nPa = atP(rl + STEP);    // the potential pool covers every disk: loaded whether used or not
     };
     fetch(sfirst);
     int s = sfirst;
@@ -1996,26 +2003,35 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
       const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
       const double ns_s = S0, ns_u = u + dt * aR, ns_w = w + dt * ap;
       if (DOWN) {
-        qs0 = qs1; qs1 = qs2; qs2 = qs3; qs3 = ns_s;
-        qu0 = qu1; qu1 = qu2; qu2 = qu3; qu3 = ns_u;
-        qw0 = qw1; qw1 = qw2; qw2 = qw3; qw3 = ns_w;
+        qs0 = qs1; qs1 = qs2; qs2 = ns_s;
+        qu0 = qu1; qu1 = qu2; qu2 = ns_u;
+        qw0 = qw1; qw1 = qw2; qw2 = ns_w;
       } else {
-        qs3 = qs2; qs2 = qs1; qs1 = qs0; qs0 = ns_s;
-        qu3 = qu2; qu2 = qu1; qu1 = qu0; qu0 = ns_u;
-        qw3 = qw2; qw2 = qw1; qw1 = qw0; qw0 = ns_w;
+        qs2 = qs1; qs1 = qs0; qs0 = ns_s;
+        qu2 = qu1; qu1 = qu0; qu0 = ns_u;
+        qw2 = qw1; qw1 = qw0; qw0 = ns_w;
       }
       Prb = Pr0; Pr0 = Pra; S0 = Sa;
       Pb = P0; P0 = Pa;
     };
-    // ---- radial face f between cells f-1 (slot 1) and f (slot 2) (hydro.hpp:111-130)
+    // ---- the radial half slope of the middle slot's cell (hydro.hpp:107-110)
+    auto slopes = [&]() {
+      dsc = dsn; duc = dun; dwc = dwn;
+      dsn = half_slope(qs0, qs1, qs2);
+      dun = half_slope(qu0, qu1, qu2);
+      dwn = half_slope(qw0, qw1, qw2);
+    };
+    // ---- radial face f between cells f-1 and f (hydro.hpp:111-130): slots
+    // 0 and 1 walking down (slopes c and n), 1 and 2 walking up (n and c)
     auto radial_face = [&](int f, double &fm, double &fu, double &fw) {
       const int fc = min(max(f, 0), Nu);     // faces 0 .. Nu (past them only in masked rows)
       const int fl = max(fc - 1, 0), fr = min(fc, Nu - 1);
-      const double vf = 0.5 * (qu1 + qu2);
+      const double sL = DOWN ? qs0 : qs1, uL = DOWN ? qu0 : qu1, wL = DOWN ? qw0 : qw1;
+      const double sR = DOWN ? qs1 : qs2, uR = DOWN ? qu1 : qu2, wR = DOWN ? qw1 : qw2;
+      const double dsL = DOWN ? dsc : dsn, duL = DOWN ? duc : dun, dwL = DOWN ? dwc : dwn;
+      const double dsR = DOWN ? dsn : dsc, duR = DOWN ? dun : duc, dwR = DOWN ? dwn : dwc;
+      const double vf = 0.5 * (uL + uR);
       const bool up = vf > 0;
-      const double bs = up ? qs1 : qs2, bu = up ? qu1 : qu2, bw = up ? qw1 : qw2;
-      const double ls = up ? qs0 : qs1, lu = up ? qu0 : qu1, lw = up ? qw0 : qw1;
-      const double hs = up ? qs2 : qs3, hu = up ? qu2 : qu3, hw = up ? qw2 : qw3;
       // F = 1 - vf dt/dR_{f-1} upwind left, -(1 + vf dt/dR_f) upwind right,
       // 0 for a cell at the disk's edge (no slope there); both sides' table
       // values are loaded before the choice (scalar loads; a choice between
@@ -2026,9 +2042,9 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
       const double iD = up ? iDl : iDr;
       const double hh = up ? hl : hr;
       const double F = hh * fma(-(fabs(vf) * dt), iD, 1.0);
-      const double ss = fma(half_slope(ls, bs, hs), F, bs);
-      const double su = fma(half_slope(lu, bu, hu), F, bu);
-      const double sw = fma(half_slope(lw, bw, hw), F, bw);
+      const double ss = fma(up ? dsL : dsR, F, up ? sL : sR);
+      const double su = fma(up ? duL : duR, F, up ? uL : uR);
+      const double sw = fma(up ? dwL : dwR, F, up ? wL : wR);
       fm = vf * ss * rec(fc)[RR_AFZ];
       fu = fm * su;
       fw = fm * sw;
@@ -2037,7 +2053,7 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
     auto gather_row = [&](int g, double flo, double fhi, double ulo, double uhi, double wlo,
                           double whi) {
       ctab_t *rg = rec(clampr(g));
-      const double s_ = DOWN ? qs1 : qs2, u_ = DOWN ? qu1 : qu2, w_ = DOWN ? qw1 : qw2;
+      const double s_ = DOWN ? qs0 : qs2, u_ = DOWN ? qu0 : qu2, w_ = DOWN ? qw0 : qw2;
       const double iv = dt * rg[RR_INVVOL];
       const double ns = s_ + (flo - fhi) * iv;
       const double nmR = s_ * u_ + (ulo - uhi) * iv;
@@ -2081,11 +2097,14 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
       a3 += (ok & (nonfinite(so, uo, wo) != 0.0)) ? 1.0 : 0.0;
     };
 
-    // stencil fill: sources of the first three rows, then the fourth row's
-    // source and the strip's first face (f = i0 down, i1 up)
+    // stencil fill: sources of the first three rows and the slope of the
+    // middle one, then the fourth row's source and slope and the strip's
+    // first face (f = i0 down, i1 up)
 #pragma unroll
     for (int k = 0; k < 3; ++k, s += STEP) source_row();
+    slopes();
     source_row();
+    slopes();
     radial_face(DOWN ? s - 1 : s + 2, pfm, pfu, pfw);
     s += STEP;
     // steady rows in groups of four, so that the unrolled body renames the
@@ -2096,6 +2115,7 @@ k_update_stream(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
 #pragma unroll
       for (int k = 0; k < 4; ++k, s += STEP) {
         source_row();
+        slopes();
         const int f = DOWN ? s - 1 : s + 2;
         double fm, fu, fw;
         radial_face(f, fm, fu, fw);

@@ -142,6 +142,12 @@ struct dfk_handle {
   i64 ntiles = 0;
   int2 *map_upd = nullptr, *map_r2c = nullptr, *map_conv = nullptr, *map_convw = nullptr,
        *map_convg = nullptr, *map_convr = nullptr;
+  // the update's work list for the loop's launches, reordered per chain
+  // group (loop_cb: the group bounds it was built for; empty: not built),
+  // and the list in layout order it is built from
+  int2 *map_loop = nullptr;
+  std::vector<int> loop_cb;
+  std::vector<int2> mu_host;
   std::vector<int> upd0, r2c0, conv0, convw0, convg0, convr0;   // per-disk first block (size nd+1)
   size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0, lds_convw = 0, lds_convg = 0, lds_convr = 0;
   size_t lds_max = 64 * 1024;   // per workgroup, from the device at create (gfx950: 160 KiB)
@@ -333,13 +339,17 @@ int build_layout(dfk_handle *h) {
   h->lds_convg = lcg;
   h->lds_convr = lcr;
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv);
-  hipFree(h->map_convw); hipFree(h->map_convg); hipFree(h->map_convr);
+  hipFree(h->map_convw); hipFree(h->map_convg); hipFree(h->map_convr); hipFree(h->map_loop);
   h->pcfl = h->pcm = nullptr;
   h->map_upd = h->map_r2c = h->map_conv = h->map_convw = h->map_convg = h->map_convr = nullptr;
+  h->map_loop = nullptr;
+  h->loop_cb.clear();
+  h->mu_host = mu;
   h->ntiles = part;
   TRY(dalloc(&h->pcfl, part));
   TRY(dalloc(&h->pcm, 4 * part));
   TRY(dalloc(&h->map_upd, (i64)mu.size()));
+  TRY(dalloc(&h->map_loop, (i64)mu.size()));
   TRY(dalloc(&h->map_r2c, (i64)mr.size()));
   TRY(dalloc(&h->map_conv, (i64)mc.size()));
   TRY(dalloc(&h->map_convw, (i64)mcw.size()));
@@ -370,6 +380,8 @@ int build_layout(dfk_handle *h) {
     per_disk(mreg, h->convr0);
   }
   HIPCHK(hipMemcpy(h->map_upd, mu.data(), sizeof(int2) * mu.size(), hipMemcpyHostToDevice));
+  // (a valid list for any disk ranges until ensure_loop_map orders it)
+  HIPCHK(hipMemcpy(h->map_loop, mu.data(), sizeof(int2) * mu.size(), hipMemcpyHostToDevice));
   if (!mr.empty()) HIPCHK(hipMemcpy(h->map_r2c, mr.data(), sizeof(int2) * mr.size(), hipMemcpyHostToDevice));
   if (!mc.empty()) HIPCHK(hipMemcpy(h->map_conv, mc.data(), sizeof(int2) * mc.size(), hipMemcpyHostToDevice));
   if (!mcw.empty())
@@ -493,15 +505,16 @@ int launch_gravity_disk(dfk_handle *h, int d, int par) {
 // Update of map range [base, base + nblk) from state buffer par; the
 // gamma = 2 kernels when every disk of the handle has gamma = 2.
 hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR, int has_gp, int par,
-                         hipStream_t s = nullptr) {
+                         hipStream_t s = nullptr, const int2 *map = nullptr) {
   if (!s) s = h->stream;
+  if (!map) map = h->map_upd;
   int g2 = 1;
   for (const DiskDev &D : h->hD) g2 &= (D.gamma2 != 0);
   if (h->upd_kind == 1)
-    return dfk_launch_update_stream(s, nblk, garr, g2, h->dD, h->map_upd, base, h->ctl, h->st,
+    return dfk_launch_update_stream(s, nblk, garr, g2, h->dD, map, base, h->ctl, h->st,
                                     h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
                                     h->trig, h->pcfl, h->pcm, par);
-  return dfk_launch_update(s, nblk, h->thr_upd, h->lds_upd, garr, g2, h->dD, h->map_upd, base,
+  return dfk_launch_update(s, nblk, h->thr_upd, h->lds_upd, garr, g2, h->dD, map, base,
                            h->ctl, h->st, h->ntot, h->phi, h->gR, h->gp, has_gR, has_gp, h->geom,
                            h->trig, h->pcfl, h->pcm, par);
 }
@@ -533,7 +546,7 @@ int launch_step_all(dfk_handle *h, int par) {
   } else {
     HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
   }
-  HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par));
+  HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par, h->stream, h->map_loop));
   return DFK_OK;
 }
 
@@ -545,7 +558,7 @@ int launch_step_range(dfk_handle *h, hipStream_t s, int a, int b, int par) {
   HIPCHK(launch_r2c(h, s, nr, h->r2c0[a], par, b - a, a));
   HIPCHK(launch_conv_range(h, s, a, b));
   HIPCHK(launch_c2r(h, s, nr, h->r2c0[a]));
-  HIPCHK(launch_update(h, nu, h->upd0[a], 0, 0, 0, par, s));
+  HIPCHK(launch_update(h, nu, h->upd0[a], 0, 0, 0, par, s, h->map_loop));
   return DFK_OK;
 }
 
@@ -570,12 +583,37 @@ std::vector<int> chain_bounds(dfk_handle *h) {
   return cut;
 }
 
+// The loop's update work list for the chain groups cb: each group's part of
+// the layout-order list reordered on its own (one launch covers a group), so
+// that XCD x runs one contiguous chunk of the group -- whole strips of
+// several disks -- and neighbouring strips, which read each other's edge
+// rows, mostly share an L2.  (map_upd is reordered per disk, for the calls
+// on one disk; with the several disks of a loop launch that put most strip
+// neighbours of a 128^2 .. 512^2 disk on different XCDs.)  A rebuild drops
+// the cached graphs, which launch the old order.
+int ensure_loop_map(dfk_handle *h, const std::vector<int> &cb) {
+  if (h->loop_cb == cb) return DFK_OK;
+  drop_graphs(h);
+  std::vector<int2> m(h->mu_host.size());
+  for (size_t g = 0; g + 1 < cb.size(); ++g) {
+    const int a = h->upd0[cb[g]], b = h->upd0[cb[g + 1]];
+    std::vector<int2> part(h->mu_host.begin() + a, h->mu_host.begin() + b);
+    part = xcd_order(part);
+    std::copy(part.begin(), part.end(), m.begin() + a);
+  }
+  if (!m.empty())
+    HIPCHK(hipMemcpy(h->map_loop, m.data(), sizeof(int2) * m.size(), hipMemcpyHostToDevice));
+  h->loop_cb = cb;
+  return DFK_OK;
+}
+
 // Graph of `nsteps` iterations starting with state buffer par0 (cached).
 int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
   const int key = 2 * nsteps + par0;
   auto it = h->graphs.find(key);
   if (it != h->graphs.end()) { *out = it->second; return DFK_OK; }
   const std::vector<int> cb = chain_bounds(h);
+  TRY(ensure_loop_map(h, cb));
   const int ng = (int)cb.size() - 1;              // disk groups
   const int nc = std::min(ng, kMaxChainStreams);  // streams; group g runs on stream g % nc
   HIPCHK(hipSetDevice(h->device));
@@ -999,7 +1037,7 @@ void dfk_destroy(dfk_handle *h) {
   hipFree(h->dD); hipFree(h->st); hipFree(h->phi); hipFree(h->gR); hipFree(h->gp); hipFree(h->F);
   hipFree(h->spec); hipFree(h->spec2); hipFree(h->khat); hipFree(h->geom); hipFree(h->trig); hipFree(h->tw);
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv); hipFree(h->map_convw);
-  hipFree(h->map_convg); hipFree(h->map_convr);
+  hipFree(h->map_convg); hipFree(h->map_convr); hipFree(h->map_loop);
   hipFree(h->ctl); hipFree(h->samples);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
