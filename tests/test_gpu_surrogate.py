@@ -119,8 +119,12 @@ def test_surrogate_scan_1e5_points(trained):
 def test_training_table_reproduces():
     """Four held-out runs of the table re-made now by the batched hydro (256 x
     256, t = 8, their own handle): the same parameters, and the targets the
-    table holds to round-off (1e-9 relative: the tuned transform variants
-    change with the build, their results only in the last bits)."""
+    table holds.  gamma and Omega_p come from the linear growth and agree to
+    round-off (1e-9 relative; builds whose arithmetic differs in the last
+    bits, as the tuned transform variants and the update kernels' forms do,
+    give ~1e-12).  The saturation peak comes after the growth turns
+    nonlinear, which amplifies last-bit differences of the state (~1e-4
+    between such builds at t = 8): 1e-3 relative."""
     import diskfft_amd
     if diskfft_amd.device_count() < 1:
         pytest.fail("no HIP device visible")
@@ -131,5 +135,7 @@ def test_training_table_reproduces():
     assert r["ok"].all() and np.array_equal(r["X"], tab["X"][4096:4100])
     assert not tab["tlim_extended"][4096:4100].any()
     rel = np.abs(r["Y"] - tab["Y"][4096:4100]) / np.abs(tab["Y"][4096:4100])
-    print(f"  max relative difference from the table: {rel.max():.2e}", flush=True)
-    assert rel.max() < 1e-9, (r["Y"], tab["Y"][4096:4100])
+    print(f"  max relative difference from the table (gamma, Omega_p, sat): "
+          f"{rel.max(axis=0)}", flush=True)
+    assert rel[:, :2].max() < 1e-9, (r["Y"], tab["Y"][4096:4100])
+    assert rel[:, 2].max() < 1e-3, (r["Y"], tab["Y"][4096:4100])
