@@ -214,7 +214,10 @@ int dfk_get_row_transform(dfk_handle *h, int *inplace);
 /* Update-kernel variant for all disks of the handle: 0 = LDS tile (ti x tj
  * zones per workgroup; suits small, latency-bound runs), 1 = register
  * streaming (one wavefront per 58 columns x ti rows; suits large or batched,
- * bandwidth-bound runs).  Both compute the same step. */
+ * bandwidth-bound runs).  Both compute the same step (to round-off; dt to the
+ * bit in both).  Kind 1 addresses a state buffer with 31-bit offsets: a
+ * handle of more than 89 478 485 zones (3 fields x 8 bytes each within
+ * 2^31 - 16 bytes) gets DFK_EUNSUPPORTED and keeps kind 0. */
 int dfk_set_update_kernel(dfk_handle *h, int kind);
 int dfk_get_update_kernel(dfk_handle *h, int *kind);
 
