@@ -816,3 +816,23 @@ def test_error_paths(dfk):
     with pytest.raises(dfk.DiskfftError) as ei:
         e.advance(1)
     assert ei.value.code == dfk.DFK_ESTATE
+
+
+def test_streaming_update_size_limit(dfk):
+    """The streaming update addresses a state buffer with 31-bit offsets: a
+    handle of more than 89 478 485 zones (eleven 2048 x 4096 disks, 92 M
+    zones) is refused with DFK_EUNSUPPORTED and keeps the LDS tile update;
+    ten such disks (84 M zones) take it.  Disks without self-gravity: no
+    spectra or K-hat to build."""
+    print("case streaming update size limit (92 M and 84 M zones)", flush=True)
+    big = [dfk.disk_params(2048, 4096, selfgrav=0) for _ in range(11)]
+    e = dfk.Ensemble(big)
+    with pytest.raises(dfk.DiskfftError) as ei:
+        e.set_update_kernel(1)
+    assert ei.value.code == dfk.DFK_EUNSUPPORTED
+    assert e.get_update_kernel() == 0
+    e.close()
+    e = dfk.Ensemble(big[:10])
+    e.set_update_kernel(1)
+    assert e.get_update_kernel() == 1
+    e.close()
