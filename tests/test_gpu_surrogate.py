@@ -20,13 +20,13 @@ whose |h| exceeds 5% of the training targets' 5-95% range, as
 surrogate_fit.errors reports it):
   - with the hydro fallback where the members' spread exceeds 5% (those
     points take the hydro value, the north star's system): mean <= 3% for
-    each output (recorded 0.20%, 0.07%, 2.1%);
-  - surrogate alone: median <= 3% for each output (recorded 0.34%, 0.10%,
-    2.8%);
+    each output (recorded 0.24%, 0.06%, 2.1%);
+  - surrogate alone: median <= 3% for each output (recorded 0.30%, 0.09%,
+    2.7%);
   - the surrogate beats nearest-neighbour and linear interpolation of the
     same training runs (the screening one would do without a network).
 The 10^5-point scan over the parameter box reports the fraction of points
-that would fall back to the hydro (recorded 37%) and its throughput.
+that would fall back to the hydro (recorded 38%) and its throughput.
 """
 import os
 import time
@@ -119,12 +119,13 @@ def test_surrogate_scan_1e5_points(trained):
 def test_training_table_reproduces():
     """Four held-out runs of the table re-made now by the batched hydro (256 x
     256, t = 8, their own handle): the same parameters, and the targets the
-    table holds.  gamma and Omega_p come from the linear growth and agree to
-    round-off (1e-9 relative; builds whose arithmetic differs in the last
-    bits, as the tuned transform variants and the update kernels' forms do,
-    give ~1e-12).  The saturation peak comes after the growth turns
-    nonlinear, which amplifies last-bit differences of the state (~1e-4
-    between such builds at t = 8): 1e-3 relative."""
+    table holds, to round-off (1e-9 relative).  The table was made by this
+    build's kernels: a build whose arithmetic differs in the last bits (a
+    tuned transform variant, another form of the update) reproduces gamma
+    and Omega_p, fitted in the linear growth, to ~1e-12, but the saturation
+    peak comes after the growth turns nonlinear, which amplifies last-bit
+    differences of the state (up to a few per cent in the runs that saturate
+    chaotically); such a change regenerates the table (surrogate_fit.py)."""
     import diskfft_amd
     if diskfft_amd.device_count() < 1:
         pytest.fail("no HIP device visible")
@@ -137,5 +138,4 @@ def test_training_table_reproduces():
     rel = np.abs(r["Y"] - tab["Y"][4096:4100]) / np.abs(tab["Y"][4096:4100])
     print(f"  max relative difference from the table (gamma, Omega_p, sat): "
           f"{rel.max(axis=0)}", flush=True)
-    assert rel[:, :2].max() < 1e-9, (r["Y"], tab["Y"][4096:4100])
-    assert rel[:, 2].max() < 1e-3, (r["Y"], tab["Y"][4096:4100])
+    assert rel.max() < 1e-9, (r["Y"], tab["Y"][4096:4100])
