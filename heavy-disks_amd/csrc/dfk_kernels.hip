@@ -740,12 +740,23 @@ __device__ void control_disk(const DiskDev &D, DiskCtl &C, int d, const double *
 // (row r2c, or column convolution with fused row kernels): that launch's
 // other workgroups need none of its results.  Workgroups of a disk that it
 // stops read `done` racily; they then only transform a spectrum that no
-// later kernel of the disk reads.
+// later kernel of the disk reads.  mirror (null: none): a host-visible copy
+// of the control blocks (coherent pinned host memory) that the control of
+// a captured run's last iteration also writes, so that the host reads the
+// loop's state after the run without a device-to-host copy.  Nothing later
+// in that iteration writes the control block.
 __device__ __forceinline__ void trailing_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
                                                  int d, const double *__restrict__ pcfl,
                                                  const double *__restrict__ pcm,
-                                                 DfkSampleDev *__restrict__ samples, int cap, int par) {
+                                                 DfkSampleDev *__restrict__ samples, int cap, int par,
+                                                 DiskCtl *__restrict__ mirror) {
   if (!ctl[d].done) control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
+  // thread 0 made every change to the block (the others left control_disk
+  // after the reduction)
+  if (mirror && threadIdx.x == 0) {
+    mirror[d] = ctl[d];
+    __threadfence_system();
+  }
 }
 
 // Control of every disk as a launch of its own (one workgroup per disk).  In
@@ -754,10 +765,8 @@ __device__ __forceinline__ void trailing_control(const DiskDev *__restrict__ dis
 __global__ void __launch_bounds__(WG)
 k_control(const DiskDev *__restrict__ disks, DiskCtl *__restrict__ ctl,
           const double *__restrict__ pcfl, const double *__restrict__ pcm,
-          DfkSampleDev *__restrict__ samples, int cap, int par) {
-  const int d = blockIdx.x;
-  if (ctl[d].done) return;
-  control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
+          DfkSampleDev *__restrict__ samples, int cap, int par, DiskCtl *__restrict__ mirror) {
+  trailing_control(disks, ctl, blockIdx.x, pcfl, pcm, samples, cap, par, mirror);
 }
 
 // ------------------------------------------------------------------ diag
@@ -890,10 +899,11 @@ __device__ __forceinline__ void rows_r2c_body(const DiskDev *__restrict__ disks,
                                               const double *__restrict__ geom, const double2 *__restrict__ tw,
                                               double2 *__restrict__ spec, int par, int nrow_blk, int ctl0,
                                               const double *__restrict__ pcfl, const double *__restrict__ pcm,
-                                              DfkSampleDev *__restrict__ samples, int cap) {
+                                              DfkSampleDev *__restrict__ samples, int cap,
+                                              DiskCtl *__restrict__ mirror) {
   extern __shared__ double2 lds2[];
   if ((int)blockIdx.x >= nrow_blk) {   // the loop control of disk ctl0 + blockIdx.x - nrow_blk
-    trailing_control(disks, ctl, ctl0 + blockIdx.x - nrow_blk, pcfl, pcm, samples, cap, par);
+    trailing_control(disks, ctl, ctl0 + blockIdx.x - nrow_blk, pcfl, pcm, samples, cap, par, mirror);
     return;
   }
   const int2 wm = map[map_base + blockIdx.x];
@@ -951,8 +961,9 @@ __device__ __forceinline__ void rows_r2c_body(const DiskDev *__restrict__ disks,
       const double *__restrict__ geom, const double2 *__restrict__ tw,                          \
       double2 *__restrict__ spec, int par, int nrow_blk, int ctl0,                              \
       const double *__restrict__ pcfl, const double *__restrict__ pcm,                          \
-      DfkSampleDev *__restrict__ samples, int cap
-#define R2C_ARGS disks, map, map_base, ctl, st, ntot, geom, tw, spec, par, nrow_blk, ctl0, pcfl, pcm, samples, cap
+      DfkSampleDev *__restrict__ samples, int cap, DiskCtl *__restrict__ mirror
+#define R2C_ARGS \
+  disks, map, map_base, ctl, st, ntot, geom, tw, spec, par, nrow_blk, ctl0, pcfl, pcm, samples, cap, mirror
 
 __global__ void __launch_bounds__(WG) k_rows_r2c(R2C_PARAMS) { rows_r2c_body<0>(R2C_ARGS); }
 
@@ -966,10 +977,11 @@ __global__ void __launch_bounds__(TG) k_rows_r2c_group(R2C_PARAMS) { rows_r2c_bo
 // disks ctl0, ctl0 + 1, ...
 #define CONV_CTL_PARAMS                                                                         \
   int nconv, int ctl0, const double *__restrict__ pcfl, const double *__restrict__ pcm,         \
-      DfkSampleDev *__restrict__ samples, int cap, int par
+      DfkSampleDev *__restrict__ samples, int cap, int par, DiskCtl *__restrict__ mirror
 #define CONV_CONTROL                                                                            \
   if ((int)blockIdx.x >= nconv) {                                                               \
-    trailing_control(disks, ctl, ctl0 + blockIdx.x - nconv, pcfl, pcm, samples, cap, par);     \
+    trailing_control(disks, ctl, ctl0 + blockIdx.x - nconv, pcfl, pcm, samples, cap, par,      \
+                     mirror);                                                                   \
     return;                                                                                     \
   }
 
@@ -1350,7 +1362,7 @@ template <int NMAX>
 __global__ void __launch_bounds__(64) k_rows_r2c_reg(R2C_PARAMS) {
   extern __shared__ double2 lds2[];
   if ((int)blockIdx.x >= nrow_blk) {
-    trailing_control(disks, ctl, ctl0 + blockIdx.x - nrow_blk, pcfl, pcm, samples, cap, par);
+    trailing_control(disks, ctl, ctl0 + blockIdx.x - nrow_blk, pcfl, pcm, samples, cap, par, mirror);
     return;
   }
   const int2 wm = map[map_base + blockIdx.x];
@@ -2305,10 +2317,10 @@ K pick_tg(int nthr, int batched, K a0, K a1, K b0, K b1, K c0, K c1, K d0, K d1)
 
 hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
                               const double *pcfl, const double *pcm, DfkSampleDev *samples, int cap,
-                              int par) {
+                              int par, DiskCtl *mirror) {
   if (ndisk <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_control, dim3(ndisk), dim3(WG), 0, s, disks, ctl, pcfl, pcm, samples, cap,
-                     par);
+                     par, mirror);
   return hipGetLastError();
 }
 
@@ -2325,12 +2337,12 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           const int2 *map, int base, DiskCtl *ctl, const double *st, i64 ntot,
                           const double *geom, const double2 *tw, double2 *spec, int par,
                           int nctl, int ctl0, const double *pcfl, const double *pcm,
-                          DfkSampleDev *samples, int cap, int inplace) {
+                          DfkSampleDev *samples, int cap, int inplace, DiskCtl *mirror) {
   if (nblk + nctl <= 0) return hipSuccess;
   if (inplace >= 3) {
     auto k = inplace == 3 ? k_rows_r2c_reg<512> : k_rows_r2c_reg<1024>;
     hipLaunchKernelGGL(k, dim3(nblk + nctl), dim3(64), lds, s, disks, map, base, ctl, st, ntot, geom,
-                       tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap);
+                       tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap, mirror);
     return hipGetLastError();
   }
   if (inplace) {
@@ -2338,15 +2350,15 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
     auto k = PICK_GROUP(k_rows_r2c_group);
     if (nthr != 64 && nthr != 128 && nthr != 192 && nthr != 256) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k, dim3(nblk + nctl), dim3(nthr), lds, s, disks, map, base, ctl, st,
-                       ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap);
+                       ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap, mirror);
     return hipGetLastError();
   }
   hipLaunchKernelGGL(k_rows_r2c, dim3(nblk + nctl), dim3(nthr), lds, s, disks, map, base, ctl, st,
-                     ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap);
+                     ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap, mirror);
   return hipGetLastError();
 }
 
-#define CONV_CTL_ARGS nblk, c.ctl0, c.pcfl, c.pcm, c.samples, c.cap, c.par
+#define CONV_CTL_ARGS nblk, c.ctl0, c.pcfl, c.pcm, c.samples, c.cap, c.par, c.mirror
 
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                            const int2 *map, int base, DiskCtl *ctl, const double *khat,

@@ -9,11 +9,14 @@
 
 hipError_t dfk_launch_control(hipStream_t s, int ndisk, const DiskDev *disks, DiskCtl *ctl,
                               const double *pcfl, const double *pcm, DfkSampleDev *samples,
-                              int cap, int par);
+                              int cap, int par, DiskCtl *mirror = nullptr);
 hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const int2 *map,
                            int base, const DiskCtl *ctl, const double *st, long long ntot,
                            const double *geom, const double *trig, double *pcfl, double *pcm);
 // nctl > 0 appends one control workgroup per disk ctl0 .. ctl0+nctl-1 to the launch.
+// mirror (here and in the control launches): null, or a host-visible copy of the
+// control blocks that the control workgroups also write (the last iteration of a
+// captured run).
 // inplace: 0 the cooperative row transforms, 1 the in-place ones (k_rows_*_group) with
 // one complex row per workgroup (pr = 1 for every disk), 2 in place with batches,
 // 3 the register two-pass kernels (k_rows_*_reg; every Nphi <= 512, 64 threads,
@@ -22,7 +25,7 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           const int2 *map, int base, DiskCtl *ctl, const double *st,
                           long long ntot, const double *geom, const double2 *tw, double2 *spec,
                           int par, int nctl, int ctl0, const double *pcfl, const double *pcm,
-                          DfkSampleDev *samples, int cap, int inplace);
+                          DfkSampleDev *samples, int cap, int inplace, DiskCtl *mirror = nullptr);
 // Loop control riding in a column-convolution launch: nctl trailing
 // workgroups run the control of disks ctl0 .. ctl0 + nctl - 1 (nctl = 0: none).
 struct DfkConvCtl {
@@ -30,6 +33,7 @@ struct DfkConvCtl {
   const double *pcfl = nullptr, *pcm = nullptr;
   DfkSampleDev *samples = nullptr;
   int cap = 0, par = 0;
+  DiskCtl *mirror = nullptr;
 };
 hipError_t dfk_launch_conv(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                            const int2 *map, int base, DiskCtl *ctl, const double *khat,

@@ -152,6 +152,11 @@ struct dfk_handle {
   size_t lds_upd = 0, lds_r2c = 0, lds_conv = 0, lds_convw = 0, lds_convg = 0, lds_convr = 0;
   size_t lds_max = 64 * 1024;   // per workgroup, from the device at create (gfx950: 160 KiB)
   DiskCtl *ctl = nullptr;
+  // host-visible copy of ctl (coherent pinned host memory; ctl_dev: its
+  // device address) that the control of each captured run's last iteration
+  // writes: dfk_advance reads the loop's state there after the run instead
+  // of copying the control blocks back
+  DiskCtl *ctl_host = nullptr, *ctl_dev = nullptr;
   // run
   bool run_ready = false;
   int cap = 0;
@@ -430,10 +435,10 @@ int row_kernel(const dfk_handle *h) {
 // ctl0 .. ctl0 + nctl - 1 in trailing workgroups of the same launch (nctl = 0:
 // none), on stream s.
 hipError_t launch_r2c(dfk_handle *h, hipStream_t s, int nblk, int base, int par, int nctl,
-                      int ctl0 = 0, double2 *spec = nullptr) {
+                      int ctl0 = 0, double2 *spec = nullptr, DiskCtl *mirror = nullptr) {
   return dfk_launch_r2c(s, nblk, h->thr_rows, h->lds_r2c, h->dD, h->map_r2c, base, h->ctl,
                         h->st, h->ntot, h->geom, h->tw, spec ? spec : h->spec, par, nctl, ctl0,
-                        h->pcfl, h->pcm, h->samples, h->cap, row_kernel(h));
+                        h->pcfl, h->pcm, h->samples, h->cap, row_kernel(h), mirror);
 }
 
 // The half spectrum that holds S-hat of the states in buffer par: with fused
@@ -452,9 +457,10 @@ hipError_t launch_c2r(dfk_handle *h, hipStream_t s, int nblk, int base) {
 // (cooperative, wave-local, workgroup in place) for the disks that use it.
 // ctl_par >= 0: the last of these launches also runs the loop control of
 // disks a .. b-1 (state buffer ctl_par) in trailing workgroups.
-// ctl: the control blocks the kernels read (null: the handle's).
+// ctl: the control blocks the kernels read (null: the handle's); mirror: as
+// for the row r2c's control.
 hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b, double2 *spec = nullptr,
-                             int ctl_par = -1, DiskCtl *ctl = nullptr) {
+                             int ctl_par = -1, DiskCtl *ctl = nullptr, DiskCtl *mirror = nullptr) {
   const int n = h->conv0[b] - h->conv0[a], nw = h->convw0[b] - h->convw0[a];
   const int ng = h->convg0[b] - h->convg0[a];
   if (!spec) spec = h->spec;
@@ -468,6 +474,7 @@ hipError_t launch_conv_range(dfk_handle *h, hipStream_t s, int a, int b, double2
     c.samples = h->samples;
     c.cap = h->cap;
     c.par = ctl_par;
+    c.mirror = mirror;
   }
   // the control rides in the last launch that has work (the cooperative one,
   // empty but for the control, when none has)
@@ -522,10 +529,10 @@ hipError_t launch_update(dfk_handle *h, int nblk, int base, int garr, int has_gR
 // Fused row kernels: two launches, the column convolution of S-hat in
 // spec_of(par) (with the loop control in trailing workgroups) and the row
 // kernel, which leaves S-hat of the new state in spec_of(1 - par).
-int launch_step_fused(dfk_handle *h, hipStream_t s, int a, int b, int par) {
+int launch_step_fused(dfk_handle *h, hipStream_t s, int a, int b, int par, DiskCtl *mirror) {
   int g2 = 1;
   for (const DiskDev &D : h->hD) g2 &= (D.gamma2 != 0);
-  HIPCHK(launch_conv_range(h, s, a, b, spec_of(h, par), par));
+  HIPCHK(launch_conv_range(h, s, a, b, spec_of(h, par), par, nullptr, mirror));
   HIPCHK(dfk_launch_rows_fused(s, h->upd0[b] - h->upd0[a], h->fused_req, h->lds_fused, g2, h->dD,
                                h->map_upd, h->upd0[a], h->ctl, h->st, h->ntot, spec_of(h, par),
                                spec_of(h, 1 - par), h->geom, h->tw, h->trig, h->pcfl, h->pcm, par));
@@ -536,15 +543,18 @@ int launch_step_fused(dfk_handle *h, hipStream_t s, int a, int b, int par) {
 // launches (two with fused row kernels).  Every running disk's state is in
 // buffer `par`; the update writes buffer 1 - par.  The loop control rides in
 // the r2c launch (a launch of its own when no disk has self-gravity).
-int launch_step_all(dfk_handle *h, int par) {
+// mirror: null, or the host-visible control copy that this iteration's
+// control also writes (the last iteration of a captured run).
+int launch_step_all(dfk_handle *h, int par, DiskCtl *mirror) {
   const int nu = h->upd0[h->nd], nr = h->r2c0[h->nd];
-  if (h->fused) return launch_step_fused(h, h->stream, 0, h->nd, par);
+  if (h->fused) return launch_step_fused(h, h->stream, 0, h->nd, par, mirror);
   if (nr > 0) {
-    HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd));
+    HIPCHK(launch_r2c(h, h->stream, nr, 0, par, h->nd, 0, nullptr, mirror));
     HIPCHK(launch_conv(h, -1));
     HIPCHK(launch_c2r(h, h->stream, nr, 0));
   } else {
-    HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par));
+    HIPCHK(dfk_launch_control(h->stream, h->nd, h->dD, h->ctl, h->pcfl, h->pcm, h->samples, h->cap, par,
+                              mirror));
   }
   HIPCHK(launch_update(h, nu, 0, 0, 0, 0, par, h->stream, h->map_loop));
   return DFK_OK;
@@ -552,10 +562,10 @@ int launch_step_all(dfk_handle *h, int par) {
 
 // The same for the disks [a, b) on stream s (every one of them
 // self-gravitating, so the control rides in their r2c launch).
-int launch_step_range(dfk_handle *h, hipStream_t s, int a, int b, int par) {
-  if (h->fused) return launch_step_fused(h, s, a, b, par);
+int launch_step_range(dfk_handle *h, hipStream_t s, int a, int b, int par, DiskCtl *mirror) {
+  if (h->fused) return launch_step_fused(h, s, a, b, par, mirror);
   const int nr = h->r2c0[b] - h->r2c0[a], nu = h->upd0[b] - h->upd0[a];
-  HIPCHK(launch_r2c(h, s, nr, h->r2c0[a], par, b - a, a));
+  HIPCHK(launch_r2c(h, s, nr, h->r2c0[a], par, b - a, a, nullptr, mirror));
   HIPCHK(launch_conv_range(h, s, a, b));
   HIPCHK(launch_c2r(h, s, nr, h->r2c0[a]));
   HIPCHK(launch_update(h, nu, h->upd0[a], 0, 0, 0, par, s, h->map_loop));
@@ -630,7 +640,8 @@ int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
   HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
   int rc = DFK_OK;
   if (nc == 1) {
-    for (int s = 0; s < nsteps && rc == DFK_OK; ++s) rc = launch_step_all(h, (par0 + s) & 1);
+    for (int s = 0; s < nsteps && rc == DFK_OK; ++s)
+      rc = launch_step_all(h, (par0 + s) & 1, s == nsteps - 1 ? h->ctl_dev : nullptr);
   } else {
     // stream c steps the disk groups c, c + nc, ... (group g holds disks
     // [cb[g], cb[g+1])), each step group after group: the streams share
@@ -649,7 +660,8 @@ int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
       hipStream_t st = c == 0 ? h->stream : h->side[c - 1];
       for (int s = 0; s < nsteps && rc == DFK_OK; ++s)
         for (int g = c; g < ng && rc == DFK_OK; g += nc)
-          rc = launch_step_range(h, st, cb[g], cb[g + 1], (par0 + s) & 1);
+          rc = launch_step_range(h, st, cb[g], cb[g + 1], (par0 + s) & 1,
+                                 s == nsteps - 1 ? h->ctl_dev : nullptr);
     }
     for (int c = 1; c < nc; ++c) {
       chk(hipEventRecord(h->join[c - 1], h->side[c - 1]));
@@ -671,10 +683,9 @@ int get_graph(dfk_handle *h, int nsteps, int par0, hipGraphExec_t *out) {
   return DFK_OK;
 }
 
-// Copies the samples produced since the last drain into the host buffers.
-int drain(dfk_handle *h) {
-  std::vector<DiskCtl> c(h->nd);
-  HIPCHK(hipMemcpy(c.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
+// Copies the samples produced since the last drain into the host buffers;
+// c: the disks' control blocks as the run left them.
+int drain(dfk_handle *h, const DiskCtl *c) {
   bool any = false;
   for (int d = 0; d < h->nd; ++d) any |= c[d].nsamples > h->drained[d];
   std::vector<DfkSampleDev> ring;
@@ -986,6 +997,11 @@ int dfk_create(const dfk_disk_params *params, int ndisks, int device, dfk_handle
       (rc = dalloc(&h->geom, h->ngeom)) || (rc = dalloc(&h->trig, h->ntrig)) ||
       (rc = dalloc(&h->tw, h->ntw)) || (rc = dalloc(&h->ctl, ndisks)))
     return fail(rc);
+  if (hipHostMalloc((void **)&h->ctl_host, sizeof(DiskCtl) * ndisks, hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&h->ctl_dev, h->ctl_host, 0) != hipSuccess) {
+    set_err("pinned host control copy: hipHostMalloc / hipHostGetDevicePointer failed");
+    return fail(DFK_ENOMEM);
+  }
   if (hipMemset(h->st, 0, sizeof(double) * 6 * h->ntot) != hipSuccess ||
       hipMemset(h->phi, 0, sizeof(double) * h->ntot) != hipSuccess ||
       hipMemset(h->ctl, 0, sizeof(DiskCtl) * ndisks) != hipSuccess) {
@@ -1039,6 +1055,7 @@ void dfk_destroy(dfk_handle *h) {
   hipFree(h->pcfl); hipFree(h->pcm); hipFree(h->map_upd); hipFree(h->map_r2c); hipFree(h->map_conv); hipFree(h->map_convw);
   hipFree(h->map_convg); hipFree(h->map_convr); hipFree(h->map_loop);
   hipFree(h->ctl); hipFree(h->samples);
+  if (h->ctl_host) hipHostFree(h->ctl_host);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
   for (hipStream_t s : h->side) hipStreamDestroy(s);
@@ -1317,15 +1334,22 @@ int dfk_advance(dfk_handle *h, int nsteps, int *all_done) {
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     total_ms += ms;
     h->iter += chunk;
-    TRY(drain(h));
+    // the run's last iteration wrote every disk's control block to ctl_host
+    // before the event completed
+    TRY(drain(h, h->ctl_host));
     left -= chunk;
   }
   h->last_ms = total_ms;
   if (all_done) {
-    std::vector<DiskCtl> c(h->nd);
-    HIPCHK(hipMemcpy(c.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
+    std::vector<DiskCtl> c;
+    const DiskCtl *cc = h->ctl_host;
+    if (nsteps == 0) {   // no run: the control blocks themselves
+      c.resize(h->nd);
+      HIPCHK(hipMemcpy(c.data(), h->ctl, sizeof(DiskCtl) * h->nd, hipMemcpyDeviceToHost));
+      cc = c.data();
+    }
     int ad = 1;
-    for (int d = 0; d < h->nd; ++d) ad &= (c[d].done != 0);
+    for (int d = 0; d < h->nd; ++d) ad &= (cc[d].done != 0);
     *all_done = ad;
   }
   return DFK_OK;

@@ -787,6 +787,62 @@ def test_chains_and_graph_steps_change_mid_run(orc, dfk):
         assert len(pa) > 2 and np.array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("mode", ["rows", "rows_3chains", "fused", "no_selfgrav"])
+def test_advance_chunks_report_samples_and_done(orc, dfk, mode):
+    """dfk_advance reads the loop's state after a run (the done flags it
+    returns, the sample counts it drains) from the host copy that the
+    control of each captured run's last iteration writes.  Runs split into
+    advances of 0, 1, 2, 3 and 5 steps (graphs of 1 to 4 steps) report the
+    same all_done, status, samples and state, step by step, as the run made
+    of one advance; the control rides in the row r2c (one and three chains),
+    in the column convolution (fused row kernels) or in a launch of its own
+    (disks without self-gravity)."""
+    print(f"case advance chunks / host control copy: {mode}", flush=True)
+    shapes = [(32, 48, 0.02), (40, 32, 0.01), (24, 36, 0.03), (36, 64, 0.02)]
+    sg = 0 if mode == "no_selfgrav" else 1
+    params = [dfk.disk_params(a, b, eps=c, selfgrav=sg) for a, b, c in shapes]
+    ics = [_lka98(orc, a, b, c, selfgrav=sg)[0] for a, b, c in shapes]
+    caps = [17, 11, 17, 23]   # per-disk step caps: the disks finish at different advances
+
+    def start():
+        e = dfk.Ensemble(params)
+        for d, h in enumerate(ics):
+            e.set_state(*h.state(), d=d)
+        if mode == "fused":
+            e.set_fused_rows(256)
+        e.set_chains(3 if mode == "rows_3chains" else 1)
+        e.set_graph_steps(4)
+        e.run_init(1e9, dtout=0.003, mD=[h.disk_mass() for h in ics], max_steps=caps,
+                   sample_cap=64)
+        return e
+
+    one = start()
+    assert not one.advance(0)
+    assert one.advance(30)
+    ref = [(one.get_state(d), one.status(d), one.samples(d)) for d in range(len(shapes))]
+    one.close()
+    e = start()
+    if mode == "fused":
+        assert e.get_fused_rows() > 0
+    if mode == "rows_3chains":
+        assert e.get_chains() > 1
+    total, chunks = 0, [0, 1, 2, 3, 5, 0, 1, 3, 2, 5, 3, 5]
+    for n in chunks:
+        done = e.advance(n)
+        total += n
+        steps = [e.status(d)["steps"] for d in range(len(shapes))]
+        assert steps == [min(total, c) for c in caps], (total, steps)
+        # a disk is done at the control after its last step
+        assert done == (total > max(caps)), (total, done)
+    assert total > max(caps)
+    for d, (s, t, p) in enumerate(ref):
+        assert e.status(d) == t
+        assert np.array_equal(e.samples(d), p) and len(p) > 2
+        for a, b in zip(e.get_state(d), s):
+            assert np.array_equal(a, b)
+    e.close()
+
+
 def test_nonfinite_flag_stops_disk(dfk):
     Nu, Nphi = 32, 32
     print("case nonfinite flag", flush=True)
