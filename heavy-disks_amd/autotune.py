@@ -154,20 +154,20 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     e.set_row_transform(0)
     # column batches: twiddles in LDS or through the caches; workgroup-
     # cooperative columns (ping-pong buffers), one wave per column in place,
-    # the workgroup in place on one column after another, or the two-pass
-    # register transform (its batch and twiddle reads are fixed)
+    # the workgroup in place on one column after another, or the two- or
+    # three-pass register transforms (their batch and twiddle reads are fixed)
     thr0 = e.get_threads()
     M, Nc = 2 * Nu, Nphi // 2 + 1
     cbs_wave = [c for c in (1, 2, 4, 8, 16) if (c + 1) * M * 16 <= LDS_MAX and c <= Nc]
     cbs_group = [c for c in (1, 2, 4) if c <= Nc]
-    for wave in (0, 1, 2, 3):
+    for wave in (0, 1, 2, 3, 4):
         for twl in (1, 0):
             e.set_fft_options(1, twl, wave)
             if wave and e.get_fft_options(0)[2] == 0:
                 break                  # this length has no in-place / register transform
-            if wave == 3 and twl == 0:
-                break                  # the register transform has no twiddle or batch knob
-            for cb in {0: cbs, 1: cbs_wave, 2: cbs_group, 3: [1]}[wave]:
+            if wave >= 3 and twl == 0:
+                break                  # the register transforms have no twiddle or batch knob
+            for cb in {0: cbs, 1: cbs_wave, 2: cbs_group, 3: [1], 4: [1]}[wave]:
                 if wave == 1:
                     e.set_threads(thr0[0], min(256, 64 * cb), thr0[2])
                 elif wave == 2:

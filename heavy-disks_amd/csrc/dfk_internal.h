@@ -87,7 +87,8 @@ struct DiskDev {
                        // twiddles in LDS (else read them through the caches)
   int conv_wave;       // column convolution: 0 the workgroup cooperates on its columns
                        // (ping-pong buffers), 1 one wave per column in place, 2 the
-                       // workgroup in place on one column after another
+                       // workgroup in place on one column after another, 3 the
+                       // two-pass register transform, 4 the three-pass one
   int pad2_;
 };
 

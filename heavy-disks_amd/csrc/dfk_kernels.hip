@@ -1174,9 +1174,52 @@ __device__ __forceinline__ void conv_reg_body(const DiskDev &D, int grp, double2
   });
 }
 
+// The three-pass form (rfft::conv3: M = 64 P, P = 4, 6, 8), one column per
+// 64-thread workgroup: the latency regime's register convolution.  Every
+// table and K-hat read is issued before the first transpose.
+template <int P>
+__device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L,
+                                           const double *__restrict__ khat,
+                                           const double2 *__restrict__ tw,
+                                           double2 *__restrict__ spec) {
+  constexpr int Nu = 32 * P;
+  const int l = threadIdx.x, r = l & 7, k1 = rfft::conv3::lane_k1<P>(l);
+  const bool live = col < D.Nc;
+  const int cc = live ? col : D.Nc - 1;   // a column past the spectrum reads the last one, stores nothing
+  const double2 *TW = tw + D.two_col;
+  double2 *io = spec + D.spo + (i64)cc * Nu;
+  const double *kh = khat + D.kho + (i64)cc * (Nu + 1);
+  double kv[8];
+  rfft::conv3::khat_vals<P>(k1, r, [&](int p) { return kh[p]; }, kv);
+  double2 v[P], x[8];
+  rfft::conv3::passA<P>(l, [&](int row) { return io[row]; }, TW, v);
+  rfft::conv3::t1_write<P>(l, v, L);
+  wave_lds_sync();
+  rfft::conv3::t1_read(k1, r, L, x);
+  wave_lds_sync();
+  rfft::conv3::passB<P>(r, TW, x);
+  rfft::conv3::t2_write(k1, r, x, L);
+  wave_lds_sync();
+  rfft::conv3::t2_read(8 * k1 + r, L, x);
+  wave_lds_sync();
+  rfft::conv3::passC<P>(r, kv, TW, x);
+  rfft::conv3::t2_write(k1, r, x, L);
+  wave_lds_sync();
+  rfft::conv3::t2_read(8 * k1 + r, L, x);
+  wave_lds_sync();
+  rfft::conv3::passB_inv(k1, r, TW, x);
+  rfft::conv3::t4_write<P>(k1, r, x, L);
+  wave_lds_sync();
+  rfft::conv3::t4_read<P>(l, L, v);
+  rfft::conv3::passA_inv<P>(l, v, [&](int row, double2 q) {
+    if (live) io[row] = q;
+  });
+}
+
 // One launch for every disk of the mode, whatever its transform length: the
 // workgroup branches once on its disk's M (workgroups are dispatched in map
-// order, disk by disk, so a CU mostly runs one instance at a time).
+// order, disk by disk, so a CU mostly runs one instance at a time); disks in
+// column mode 4 take the three-pass form, one column per workgroup.
 __global__ void __launch_bounds__(64)
 k_cols_conv_reg(const DiskDev *__restrict__ disks, const int2 *__restrict__ map, int map_base,
                 DiskCtl *__restrict__ ctl, const double *__restrict__ khat,
@@ -1186,6 +1229,15 @@ k_cols_conv_reg(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
   const int2 wm = map[map_base + blockIdx.x];
   const DiskDev &D = disks[wm.x];
   if (ctl[wm.x].done) return;
+  if (D.conv_wave == 4) {
+    switch (D.M) {
+      case 256: conv3_body<4>(D, wm.y, lds2, khat, tw, spec); break;
+      case 384: conv3_body<6>(D, wm.y, lds2, khat, tw, spec); break;
+      case 512: conv3_body<8>(D, wm.y, lds2, khat, tw, spec); break;
+      default: break;   // the runtime marks no other length mode 4
+    }
+    return;
+  }
   switch (D.M) {
     case 128: conv_reg_body<8, 16, 1>(D, wm.y, lds2, khat, tw, spec); break;
     case 192: conv_reg_body<12, 16, 1>(D, wm.y, lds2, khat, tw, spec); break;
@@ -2409,6 +2461,10 @@ int dfk_conv_reg_cols(int M) {
   conv_reg_split(M, &n1, &n2, &h);
   return n2 ? 64 / n2 : 0;
 }
+
+int dfk_conv_reg3_ok(int M) { return M == 256 || M == 384 || M == 512; }
+
+size_t dfk_conv_reg3_lds() { return (size_t)rfft::conv3::lds_elems() * sizeof(double2); }
 
 size_t dfk_conv_reg_lds(int M) {
   int n1, n2, h;

@@ -51,6 +51,11 @@ hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, 
 // need among the launch's disks).
 int dfk_conv_reg_cols(int M);
 size_t dfk_conv_reg_lds(int M);
+// Three-pass register column convolution (column mode 4, one column per
+// 64-thread workgroup in the same launch): whether M (= 256, 384, 512) has
+// it, and its dynamic LDS bytes.
+int dfk_conv_reg3_ok(int M);
+size_t dfk_conv_reg3_lds();
 // Register row transforms: row pairs per 64-thread workgroup for a row of n
 // points (0: n unsupported) and the dynamic LDS bytes of such a disk.
 int dfk_rows_reg_pairs(int n);

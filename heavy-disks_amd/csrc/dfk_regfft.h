@@ -360,6 +360,163 @@ RF_HD void pass3(int b, double2 (&z)[N1], St st) {
 
 }  // namespace conv
 
+// ------------------------------------------------------------------ column convolution, three passes
+// The same zero-padded convolution for M = 64 P (P = 4, 6, 8: Nu = 32 P) by
+// one column per wavefront and three passes of in-register DFTs per
+// direction (P, 8 and 8 points), two LDS transposes per direction, so that
+// no lane holds more than one 8-point DFT's values (latency: the two-pass
+// form gives half of each wavefront an N2-point DFT per direction).
+//   forward, n = 64 n1 + 8 n2 + n3, k = k1 + P k2 + 8 P k3:
+//     pass A (lane m = 8 n2 + n3):  A[k1] = W_M^(m k1) DFT_P(x[64 n1 + m])[k1]       -> T1 [k1][m]
+//     pass B (lane 8 k1 + n3):      B[k2] = W_64^(n3 k2) DFT_8(A[k1][8 n2 + n3])[k2] -> T2 [8 k1 + k2][n3]
+//     pass C (lane 8 k1 + k2):      X[k1 + P k2 + 8 P k3] = DFT_8(B[8 k1 + k2][.])[k3]
+//   inverse, n = c + 8 b + 64 a:
+//     pass C (lane 8 k1 + k2):      C[c] = W_64^(-c k2) IDFT_8(K X)[c]                -> T2 [8 k1 + c][k2]
+//     pass B' (lane 8 k1 + c):      E[b] = W_M^(-(8 b + c) k1) IDFT_8(C[.])[b]        -> T1 [k1][8 b + c]
+//     pass A' (lane m = 8 b + c):   y[m + 64 a] = IDFT_P(E[.][m])[a], a < P/2
+// (the exponents of W_M that the index splits leave are multiples of M).
+// Passes B to B' use lanes 8 k1 + r, k1 < P; for P < 8 the lanes above 8 P
+// take k1 = P - 1 and repeat that lane's work (same reads, same values
+// written), so every access stays inside the tables and the LDS area.
+namespace conv3 {
+
+constexpr int S1 = 65, S2 = 9;   // row strides of T1 ([P][64]) and T2 ([64][8]) in double2
+constexpr int lds_elems() { return 64 * S2; }   // >= 8 * S1
+
+// lane l's k1 in passes B .. B'
+template <int P>
+RF_HD int lane_k1(int l) { return (l >> 3) < P ? (l >> 3) : P - 1; }
+
+// pass A, lane m: ld(row) returns row `row` (< Nu) of the column
+template <int P, class Ld>
+RF_HD void passA(int m, Ld ld, const double2 *tw, double2 (&v)[P]) {
+  sfor<0, P / 2>([&](auto n1_) {
+    constexpr int n1 = decltype(n1_)::value;
+    v[n1] = ld(64 * n1 + m);
+  });
+  const conv::TwRow<P, -1> w(tw, m);
+  dft_half_in<P, -1>(v);
+  sfor<1, P>([&](auto k1_) {
+    constexpr int k1 = decltype(k1_)::value;
+    v[k1] = w.template mul<k1>(v[k1]);
+  });
+}
+
+// T1: lane m of pass A writes its A[k1]; lane (k1, n3) reads A[k1][8 n2 + n3], n2 < 8
+template <int P>
+RF_HD void t1_write(int m, const double2 (&v)[P], double2 *L) {
+  sfor<0, P>([&](auto k1_) {
+    constexpr int k1 = decltype(k1_)::value;
+    L[k1 * S1 + m] = v[k1];
+  });
+}
+RF_HD void t1_read(int k1, int n3, const double2 *L, double2 (&x)[8]) {
+  sfor<0, 8>([&](auto n2_) {
+    constexpr int n2 = decltype(n2_)::value;
+    x[n2] = L[k1 * S1 + 8 * n2 + n3];
+  });
+}
+
+// pass B, lane (k1, n3)
+template <int P>
+RF_HD void passB(int n3, const double2 *tw, double2 (&x)[8]) {
+  const conv::TwRow<8, -1> w(tw, P * n3);
+  dft<8, -1>(x);
+  sfor<1, 8>([&](auto k2_) {
+    constexpr int k2 = decltype(k2_)::value;
+    x[k2] = w.template mul<k2>(x[k2]);
+  });
+}
+
+// T2: lane (k1, r) writes its x[q] to row 8 k1 + q, column r; lane 8 k1 + q
+// reads the row (forward: r = n3, q = k2; inverse: r = k2, q = c)
+RF_HD void t2_write(int k1, int r, const double2 (&x)[8], double2 *L) {
+  sfor<0, 8>([&](auto q_) {
+    constexpr int q = decltype(q_)::value;
+    L[(8 * k1 + q) * S2 + r] = x[q];
+  });
+}
+RF_HD void t2_read(int row, const double2 *L, double2 (&y)[8]) {
+  sfor<0, 8>([&](auto r_) {
+    constexpr int r = decltype(r_)::value;
+    y[r] = L[row * S2 + r];
+  });
+}
+
+// K values lane (k1, k2) multiplies by: kv[k3] = K[p], p = k1 + P k2 + 8 P k3
+// (K[M - p] for p > Nu); kh(p) returns K[p], p <= Nu
+template <int P, class Kh>
+RF_HD void khat_vals(int k1, int k2, Kh kh, double (&kv)[8]) {
+  constexpr int M = 64 * P, Nu = 32 * P;
+  sfor<0, 8>([&](auto k3_) {
+    constexpr int k3 = decltype(k3_)::value;
+    const int p = k1 + P * k2 + 8 * P * k3;
+    kv[k3] = kh(p <= Nu ? p : M - p);
+  });
+}
+
+// pass C, lane (k1, k2): forward DFT over n3, times K, inverse DFT over k3,
+// times W_64^(-c k2)
+template <int P>
+RF_HD void passC(int k2, const double (&kv)[8], const double2 *tw, double2 (&y)[8]) {
+  const conv::TwRow<8, +1> w(tw, P * k2);
+  dft<8, -1>(y);
+  sfor<0, 8>([&](auto k3_) {
+    constexpr int k3 = decltype(k3_)::value;
+    y[k3].x *= kv[k3];
+    y[k3].y *= kv[k3];
+  });
+  dft<8, +1>(y);
+  sfor<1, 8>([&](auto c_) {
+    constexpr int c = decltype(c_)::value;
+    y[c] = w.template mul<c>(y[c]);
+  });
+}
+
+// pass B', lane (k1, c): inverse DFT over k2, times W_M^(-(8 b + c) k1)
+// (table values, (8 b + c) k1 < M)
+RF_HD void passB_inv(int k1, int c, const double2 *tw, double2 (&z)[8]) {
+  double2 w[8];
+  sfor<0, 8>([&](auto b_) {
+    constexpr int b = decltype(b_)::value;
+    w[b] = tw[(8 * b + c) * k1];
+    w[b].y = -w[b].y;
+  });
+  dft<8, +1>(z);
+  sfor<0, 8>([&](auto b_) {
+    constexpr int b = decltype(b_)::value;
+    z[b] = cmul(z[b], w[b]);
+  });
+}
+
+// T1 backwards: lane (k1, c) writes E[b] to [k1][8 b + c]; lane m reads E[.][m]
+template <int P>
+RF_HD void t4_write(int k1, int c, const double2 (&z)[8], double2 *L) {
+  sfor<0, 8>([&](auto b_) {
+    constexpr int b = decltype(b_)::value;
+    L[k1 * S1 + 8 * b + c] = z[b];
+  });
+}
+template <int P>
+RF_HD void t4_read(int m, const double2 *L, double2 (&w)[P]) {
+  sfor<0, P>([&](auto k1_) {
+    constexpr int k1 = decltype(k1_)::value;
+    w[k1] = L[k1 * S1 + m];
+  });
+}
+
+// pass A', lane m: st(row, value) receives rows m + 64 a, a < P/2 (all < Nu)
+template <int P, class St>
+RF_HD void passA_inv(int m, double2 (&w)[P], St st) {
+  dft_half_out<P, +1>(w);
+  sfor<0, P / 2>([&](auto a_) {
+    constexpr int a = decltype(a_)::value;
+    st(m + 64 * a, w[a]);
+  });
+}
+
+}  // namespace conv3
+
 // Row transforms of length n = N1 N2 by the same two passes, N1 <= N2: pass 1
 // on lanes t < N2 (inputs x[N2 n1 + t]), the transpose of conv::fwd_write /
 // fwd_read_c with H = 1, pass 2 on lanes k1 < N1, which then hold bins

@@ -308,9 +308,10 @@ int build_layout(dfk_handle *h) {
       const int prb = h->rows_ip == 2 ? dfk_rows_reg_pairs(D.Nphi) : D.pr;
       for (int g = 0; g < ceil_div(D.Nu, 2 * prb); ++g) mr.push_back(make_int2(d, g));
       lr = std::max(lr, h->rows_ip == 2 ? dfk_rows_reg_lds(D.Nphi) : lds_rows(D.pr));
-      if (D.conv_wave == 3) {   // columns per workgroup fixed by the transform length
-        for (int g = 0; g < ceil_div(D.Nc, dfk_conv_reg_cols(D.M)); ++g) mreg.push_back(make_int2(d, g));
-        lcr = std::max(lcr, dfk_conv_reg_lds(D.M));
+      if (D.conv_wave >= 3) {   // columns per workgroup fixed by the transform length (one in mode 4)
+        const int cpw = D.conv_wave == 4 ? 1 : dfk_conv_reg_cols(D.M);
+        for (int g = 0; g < ceil_div(D.Nc, cpw); ++g) mreg.push_back(make_int2(d, g));
+        lcr = std::max(lcr, D.conv_wave == 4 ? dfk_conv_reg3_lds() : dfk_conv_reg_lds(D.M));
       } else {
         std::vector<int2> &cm = D.conv_wave == 2 ? mcg : (D.conv_wave == 1 ? mcw : mc);
         for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) cm.push_back(make_int2(d, g));
@@ -1508,6 +1509,7 @@ int dfk_set_fft_options(dfk_handle *h, int d, int rows_tw_lds, int cols_tw_lds, 
   const bool fast = D.M <= 1024 && inplace_radices(D.rad_col, D.nst_col, 2);
   int w = (fast && D.selfgrav && (cols_wave == 1 || cols_wave == 2)) ? cols_wave : 0;
   if (cols_wave == 3 && D.selfgrav && dfk_conv_reg_cols(D.M) > 0) w = 3;
+  if (cols_wave == 4 && D.selfgrav && dfk_conv_reg3_ok(D.M)) w = 4;
   if (v != D.tw_lds || w != D.conv_wave) {
     D.tw_lds = v;
     D.conv_wave = w;

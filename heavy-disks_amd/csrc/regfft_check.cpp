@@ -8,6 +8,7 @@
 //                         stdout: rows a, b (N reals each, unnormalised inverse)
 //   regfft_check conv M   stdin: column x[r], r < M/2 ("re im"), then K[p], p <= M/2
 //                         stdout: rows r < M/2 of inverse(K x forward(x zero-padded to M))
+//   regfft_check conv3 M  the same by the three-pass column transform (M = 256, 384, 512)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -130,6 +131,56 @@ void conv() {
   for (const double2 &q : out) std::printf("%.17g %.17g\n", q.x, q.y);
 }
 
+// The three-pass column convolution, lane by lane over the 64 lanes of one
+// wavefront (lanes above 8 P in passes B .. B' repeat lane 8 (P-1) + l mod 8).
+template <int P>
+void conv3() {
+  constexpr int M = 64 * P, Nu = M / 2;
+  std::vector<double2> col(Nu);
+  for (auto &z : col) { z.x = rd(); z.y = rd(); }
+  std::vector<double> K(Nu + 1);
+  for (auto &k : K) k = rd();
+  const std::vector<double2> tw = table(M);
+  std::vector<double2> L(rfft::conv3::lds_elems(), make_double2(NAN, NAN)), out(Nu, make_double2(NAN, NAN));
+  double2 v[64][P], x[64][8];
+  double kv[64][8];
+  for (int l = 0; l < 64; ++l)
+    rfft::conv3::khat_vals<P>(rfft::conv3::lane_k1<P>(l), l & 7, [&](int p) { return K.at(p); }, kv[l]);
+  for (int m = 0; m < 64; ++m) {
+    rfft::conv3::passA<P>(m, [&](int r) { return col.at(r); }, tw.data(), v[m]);
+    rfft::conv3::t1_write<P>(m, v[m], L.data());
+  }
+  for (int l = 0; l < 64; ++l) rfft::conv3::t1_read(rfft::conv3::lane_k1<P>(l), l & 7, L.data(), x[l]);
+  for (int l = 0; l < 64; ++l) {
+    rfft::conv3::passB<P>(l & 7, tw.data(), x[l]);
+    rfft::conv3::t2_write(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
+  }
+  for (int l = 0; l < 64; ++l) rfft::conv3::t2_read(8 * rfft::conv3::lane_k1<P>(l) + (l & 7), L.data(), x[l]);
+  for (int l = 0; l < 64; ++l) {
+    rfft::conv3::passC<P>(l & 7, kv[l], tw.data(), x[l]);
+    rfft::conv3::t2_write(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
+  }
+  for (int l = 0; l < 64; ++l) rfft::conv3::t2_read(8 * rfft::conv3::lane_k1<P>(l) + (l & 7), L.data(), x[l]);
+  for (int l = 0; l < 64; ++l) {
+    rfft::conv3::passB_inv(rfft::conv3::lane_k1<P>(l), l & 7, tw.data(), x[l]);
+    rfft::conv3::t4_write<P>(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
+  }
+  for (int m = 0; m < 64; ++m) {
+    rfft::conv3::t4_read<P>(m, L.data(), v[m]);
+    rfft::conv3::passA_inv<P>(m, v[m], [&](int r, double2 q) { out.at(r) = q; });
+  }
+  for (const double2 &q : out) std::printf("%.17g %.17g\n", q.x, q.y);
+}
+
+bool conv3_dispatch(int M) {
+  switch (M) {
+    case 256: conv3<4>(); return true;
+    case 384: conv3<6>(); return true;
+    case 512: conv3<8>(); return true;
+    default: return false;
+  }
+}
+
 // the (N1, N2) splits of the kernels (transposes in H parts for the columns)
 template <template <int, int> class F>
 bool rows_dispatch(int n) {
@@ -164,7 +215,7 @@ bool conv_dispatch(int M) {
 
 int main(int argc, char **argv) {
   if (argc != 3) {
-    std::fprintf(stderr, "usage: regfft_check r2c|c2r|conv N\n");
+    std::fprintf(stderr, "usage: regfft_check r2c|c2r|conv|conv3 N\n");
     return 2;
   }
   const int n = std::atoi(argv[2]);
@@ -172,6 +223,7 @@ int main(int argc, char **argv) {
   if (!std::strcmp(argv[1], "r2c")) ok = rows_dispatch<R2C>(n);
   else if (!std::strcmp(argv[1], "c2r")) ok = rows_dispatch<C2R>(n);
   else if (!std::strcmp(argv[1], "conv")) ok = conv_dispatch(n);
+  else if (!std::strcmp(argv[1], "conv3")) ok = conv3_dispatch(n);
   if (!ok) {
     std::fprintf(stderr, "regfft_check: no %s transform of length %d\n", argv[1], n);
     return 2;

@@ -184,7 +184,11 @@ int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
  *       register transform per direction (2Nu = N1 N2, in-register N1- and
  *       N2-point DFTs, one LDS transpose between them, 64 / N2 columns per
  *       wavefront), for 2Nu in {128, 192, 256, 384, 512, 768, 1024} only
- *       (otherwise read back as 0); it ignores cb and cols_tw_lds.
+ *       (otherwise read back as 0); it ignores cb and cols_tw_lds; 4 = three
+ *       passes of at most 8-point in-register DFTs per direction, one column
+ *       per wavefront (shorter per-lane chains for small, latency-bound
+ *       grids), for 2Nu in {256, 384, 512} only (otherwise read back as 0),
+ *       ignoring cb and cols_tw_lds.
  * The twiddle placement does not change a bit of the results; the in-place
  * and register kernels agree with the cooperative one to round-off. */
 int dfk_set_fft_options(dfk_handle *h, int disk, int rows_tw_lds, int cols_tw_lds, int cols_wave);

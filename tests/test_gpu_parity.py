@@ -156,6 +156,28 @@ def test_potential_register_columns(orc, dfk, Nu, Nphi, ok):
     assert relerr(p_reg, k.potential(Sig)) < 1e-13
 
 
+@pytest.mark.parametrize("Nu,Nphi,ok", [(128, 128, 1), (192, 192, 1), (256, 256, 1), (256, 30, 1),
+                                        (128, 1024, 1), (64, 64, 0), (512, 512, 0), (35, 40, 0)])
+def test_potential_three_pass_columns(orc, dfk, Nu, Nphi, ok):
+    """Column mode 4, the three-pass register convolution (one column per
+    wavefront, 2Nu = 256, 384, 512): round-off agreement with the
+    cooperative transform and the oracle; other lengths keep the cooperative
+    one (read back as 0), the same bits."""
+    print(f"case potential {Nu}x{Nphi} three-pass columns", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    rng = np.random.default_rng(Nu + 5 * Nphi)
+    Sig = rng.random((Nu, Nphi))
+    p_ref = e.potential(Sig)
+    e.set_fft_options(1, 0, 4)
+    assert e.get_fft_options(0)[2] == 4 * ok
+    p_3 = e.potential(Sig)
+    if ok:
+        assert relerr(p_3, p_ref) < 1e-14
+    else:
+        assert np.array_equal(p_3, p_ref)
+    assert relerr(p_3, k.potential(Sig)) < 1e-13
+
+
 @pytest.mark.parametrize("Nu,Nphi,mode", [(64, 128, 2), (37, 192, 2), (256, 256, 2), (33, 384, 2),
                                           (512, 512, 2), (96, 768, 2), (512, 1024, 2), (3, 128, 2),
                                           (5, 256, 2), (40, 96, 1), (35, 40, 0)])
@@ -423,6 +445,31 @@ def test_powerlaw_ten_step_loop_parity(orc, dfk, Nu, Nphi, kind):
     assert sg.shape == so.shape and abs(sg[0, 2] - so[0, 2]) < 1e-12 * so[0, 1]
 
 
+@pytest.mark.parametrize("Nu,Nphi", [(128, 128), (192, 192), (256, 256), (256, 96)])
+def test_three_pass_columns_loop_parity(orc, dfk, Nu, Nphi):
+    """The three-pass register column convolution (column mode 4) in the time
+    loop of the power-law disk: 10 steps within 1e-10 of the oracle, the
+    state within 1e-12 of the same run with the cooperative columns."""
+    print(f"case power-law 10-step loop {Nu}x{Nphi} three-pass columns", flush=True)
+    p, ic, mD, h, k = _powerlaw_ic(orc, dfk, Nu, Nphi)
+    out = []
+    for mode in (0, 4):
+        e = dfk.Ensemble(p)
+        e.set_fft_options(1, 0, mode)
+        assert e.get_fft_options(0)[2] == mode
+        e.set_state(*ic)
+        e.run_init(1e30, dtout=0.25, cfl=0.4, mD=mD, max_steps=10)
+        e.advance(12)
+        assert e.status()["steps"] == 10
+        out.append(e.get_state())
+        e.close()
+    orc.run(h, k, 1, 1e30, dtout=0.25, cfl=0.4, max_steps=10, mD=mD)
+    errs = [relerr(a, b) for a, b in zip(out[1], h.state())]
+    print("  rel errors Sigma, vR, vphi:", errs, flush=True)
+    assert max(errs) <= LOOP_TOL
+    assert max(relerr(a, b) for a, b in zip(out[1], out[0])) < 1e-12
+
+
 def test_loop_samples_and_clip(orc, dfk):
     """Output schedule and last-step clip of diskmain.cpp:92-110 over ~300 steps."""
     Nu, Nphi, eps, tlim = 64, 64, 0.02, 0.6
@@ -641,16 +688,17 @@ def test_register_rows_ensemble(orc, dfk, big):
 @pytest.mark.parametrize("rows_ip", [0, 1])
 def test_mixed_transform_modes_ensemble(orc, dfk, rows_ip):
     """One handle whose disks use different column-convolution modes
-    (cooperative, wave-local, workgroup in place batched and single) and
-    batch sizes gives each disk the same bits as running it alone with the
-    same options."""
+    (cooperative, wave-local, workgroup in place batched and single, the
+    two- and three-pass register forms, which share a launch, at one
+    transform length too) and batch sizes gives each disk the same bits as
+    running it alone with the same options."""
     print(f"case mixed transform modes, rows in place {rows_ip}", flush=True)
     # every Nphi factors into 2/3/4/8 (in-place rows are a handle-wide choice);
     # 2 x 40 = 80 has a factor 5, so that disk keeps the cooperative columns
     shapes = [(32, 48, 0.02), (64, 64, 0.01), (48, 24, 0.04), (40, 128, 0.01), (24, 24, 0.03),
-              (96, 48, 0.02), (64, 32, 0.01)]
+              (96, 48, 0.02), (64, 32, 0.01), (128, 64, 0.01), (192, 48, 0.02), (128, 32, 0.02)]
     opts = [(1, 1, 0, 2), (1, 0, 1, 3), (0, 0, 2, 2), (1, 0, 2, 1), (1, 1, 2, 4),
-            (1, 0, 3, 1), (0, 0, 3, 2)]   # tw rows, tw cols, mode, cb
+            (1, 0, 3, 1), (0, 0, 3, 2), (1, 0, 4, 1), (0, 0, 4, 1), (1, 0, 3, 1)]   # tw rows, tw cols, mode, cb
     params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
     ics, mDs = [], []
     for a, b, c in shapes:

@@ -4,7 +4,8 @@ every length the kernels are built for, against numpy's FFT: the row r2c
 (two real rows packed as one complex row, split into their half spectra), the
 row c2r (FFTW's convention: the imaginary parts of the DC and Nyquist bins
 ignored) and the column convolution (zero-padded forward transform, times a
-real even kernel, inverse, first half kept)."""
+real even kernel, inverse, first half kept), the latter also by the
+three-pass form (one column per wavefront)."""
 import os
 import subprocess
 
@@ -66,6 +67,44 @@ def test_column_convolution(M):
     assert np.max(np.abs(z - ref)) / np.max(np.abs(ref)) < TOL
 
 
+@pytest.mark.parametrize("M", [256, 384, 512])
+def test_column_convolution_three_pass(M):
+    """The three-pass column convolution (one column per wavefront, P-, 8-
+    and 8-point DFTs, lanes above 8P repeating lane 8(P-1)'s work for P < 8)
+    against numpy, and against the two-pass form of the same length."""
+    rng = np.random.default_rng(M + 5)
+    Nu = M // 2
+    x = rng.standard_normal(Nu) + 1j * rng.standard_normal(Nu)
+    K = rng.standard_normal(Nu + 1)
+    Kfull = np.r_[K, K[1:Nu][::-1]]
+    ref = np.fft.ifft(Kfull * np.fft.fft(np.r_[x, np.zeros(Nu)]))[:Nu] * M
+    w = run("conv3", M, np.r_[cplx(x), K]).reshape(-1, 2)
+    z = w[:, 0] + 1j * w[:, 1]
+    assert np.all(np.isfinite(z))
+    assert np.max(np.abs(z - ref)) / np.max(np.abs(ref)) < TOL
+    w2 = run("conv", M, np.r_[cplx(x), K]).reshape(-1, 2)
+    assert np.max(np.abs(z - (w2[:, 0] + 1j * w2[:, 1]))) / np.max(np.abs(ref)) < TOL
+
+
+def test_column_convolution_three_pass_impulses():
+    """Impulses at the first and last rows and a K that is 1 at one bin:
+    the result is that bin's plane wave restricted to the rows (a transposed
+    index, a dropped twiddle or a wrong K bin moves or scales it)."""
+    M, Nu = 512, 256
+    for r0, p0 in ((0, 0), (Nu - 1, 3), (77, Nu), (130, 101)):
+        x = np.zeros(Nu, complex)
+        x[r0] = 1.0
+        K = np.zeros(Nu + 1)
+        K[p0] = 1.0
+        Kfull = np.r_[K, K[1:Nu][::-1]]
+        ref = np.fft.ifft(Kfull * np.fft.fft(np.r_[x, np.zeros(Nu)]))[:Nu] * M
+        w = run("conv3", M, np.r_[cplx(x), K]).reshape(-1, 2)
+        z = w[:, 0] + 1j * w[:, 1]
+        assert np.max(np.abs(z - ref)) < 1e-14, (r0, p0)
+
+
 def test_unsupported_length_refused():
     r = subprocess.run([EXE, "r2c", "96"], input="", capture_output=True, text=True)
     assert r.returncode == 2 and "no r2c transform of length 96" in r.stderr
+    r = subprocess.run([EXE, "conv3", "1024"], input="", capture_output=True, text=True)
+    assert r.returncode == 2 and "no conv3 transform of length 1024" in r.stderr
