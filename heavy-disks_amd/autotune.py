@@ -143,8 +143,9 @@ def tune(dfk, Nu, Nphi, regime="throughput", nrep=30, verbose=True):
     # of its disks needs, so the knob only pays where the big transforms use
     # it; for the rows it lost in mixed ensembles and is not tuned.
     # ... and the row transforms cooperative (0), in place (1) or by the
-    # two-pass register kernels (2, whose row pairs per workgroup are fixed)
-    for rip in (0, 1, 2):
+    # two- or three-pass register kernels (2, 3: their row pairs per
+    # workgroup are fixed)
+    for rip in (0, 1, 2, 3):
         e.set_row_transform(rip)
         if e.get_row_transform() != rip:
             continue                   # this Nphi has no such row transform

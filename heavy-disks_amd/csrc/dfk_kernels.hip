@@ -577,18 +577,60 @@ __device__ __forceinline__ double2 *mid_stages(double2 *x, double2 *y, int n, in
 }
 
 // ------------------------------------------------------------------ reductions
+// v of the lane that the DPP control CTRL names within the lane's row of 16
+// (quad permutations, row mirrors), each 32-bit half moved on its own.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  return __hiloint2double(__builtin_amdgcn_update_dpp(0, hi, CTRL, 0xf, 0xf, false),
+                          __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// Wavefront reduction of (max, 4 sums) in a fixed order: within each row of
+// 16 lanes, neighbours (quad_perm 1,0,3,2), pairs of them (quad_perm
+// 2,3,0,1), quads (half-row mirror), halves (row mirror) -- every lane of a
+// row then holds the row's value, the two operands of each step being the
+// same whichever lane forms it -- then rows 0 + 1 + 2 + 3.  Cross-lane moves
+// by DPP and readlane, not through LDS.
+__device__ __forceinline__ void wave_reduce5(double &mx, double &s0, double &s1, double &s2, double &s3) {
+  auto step = [&](auto ctrl) {
+    constexpr int C = decltype(ctrl)::value;
+    const double o = dpp_row<C>(mx);
+    mx = (o > mx) ? o : mx;
+    s0 += dpp_row<C>(s0);
+    s1 += dpp_row<C>(s1);
+    s2 += dpp_row<C>(s2);
+    s3 += dpp_row<C>(s3);
+  };
+  step(std::integral_constant<int, 0xb1>{});    // quad_perm [1, 0, 3, 2]
+  step(std::integral_constant<int, 0x4e>{});    // quad_perm [2, 3, 0, 1]
+  step(std::integral_constant<int, 0x141>{});   // row_half_mirror
+  step(std::integral_constant<int, 0x140>{});   // row_mirror
+  auto rows = [&](double &v, bool is_max) {
+    const double r0 = lane_value(v, 0), r1 = lane_value(v, 16), r2 = lane_value(v, 32),
+                 r3 = lane_value(v, 48);
+    if (is_max) {
+      const double a = (r1 > r0) ? r1 : r0, b = (r3 > r2) ? r3 : r2;
+      v = (b > a) ? b : a;
+    } else {
+      v = (r0 + r1) + (r2 + r3);
+    }
+  };
+  rows(mx, true);
+  rows(s0, false);
+  rows(s1, false);
+  rows(s2, false);
+  rows(s3, false);
+}
+
 // Workgroup reduction of (max, 4 sums) in a fixed order (deterministic).
+// All threads of the workgroup must call it (full wavefronts).
 __device__ void wg_reduce5(double &mx, double &s0, double &s1, double &s2, double &s3,
                            double *red) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_xor(mx, off);
-    mx = (o > mx) ? o : mx;
-    s0 += __shfl_xor(s0, off);
-    s1 += __shfl_xor(s1, off);
-    s2 += __shfl_xor(s2, off);
-    s3 += __shfl_xor(s3, off);
-  }
+  wave_reduce5(mx, s0, s1, s2, s3);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
     red[w * 5 + 0] = mx; red[w * 5 + 1] = s0; red[w * 5 + 2] = s1;
@@ -1462,6 +1504,114 @@ __global__ void __launch_bounds__(64) k_rows_c2r_reg(C2R_PARAMS) {
   }
 }
 
+// Row transforms by the three-pass register form (rfft::rows3: n = Nphi =
+// 64 P, P = 2, 3, 4), one row pair per 64-thread workgroup: four times the
+// workgroups of the two-pass kernels and at most 8-point DFTs per lane (the
+// latency regime).  Row pairs as in the two-pass kernels.
+template <int P>
+__device__ __forceinline__ void r2c3_body(const DiskDev &D, int pair, double2 *L,
+                                          const double *__restrict__ S, const double *__restrict__ R32,
+                                          const double2 *__restrict__ tw, double2 *__restrict__ out) {
+  constexpr int n = 64 * P;
+  const int Nu = D.Nu, l = threadIdx.x, r = l & 7, k1 = rfft::conv3::lane_k1<P>(l);
+  const RegPair q = reg_pair<64>(pair, Nu);
+  const double pa = R32[q.ra], pb = R32[q.rb];
+  const double *Sa = S + (i64)q.ra * n, *Sb = S + (i64)q.rb * n;
+  double2 v[P], x[8];
+  rfft::rows3::passA<P, -1>(l, [&](int j) {
+    const double b = pb * Sb[j];
+    return make_double2(pa * Sa[j], q.hasb ? b : 0.0);
+  }, tw, v);
+  rfft::conv3::t1_write<P>(l, v, L);
+  wave_lds_sync();
+  rfft::conv3::t1_read(k1, r, L, x);
+  wave_lds_sync();
+  rfft::rows3::passB<P, -1>(r, tw, x);
+  rfft::conv3::t2_write(k1, r, x, L);
+  wave_lds_sync();
+  rfft::conv3::t2_read(8 * k1 + r, L, x);
+  wave_lds_sync();
+  rfft::rows3::passC_spill<P, -1>(k1, r, x, L);
+  wave_lds_sync();
+  if (q.live)
+    rfft::rows3::split<P>(l, L, [&](int k, double2 xa, double2 xb) {
+      out[(i64)k * Nu + q.ra] = xa;
+      if (q.hasb) out[(i64)k * Nu + q.ra + 1] = xb;
+    });
+}
+
+template <int P>
+__device__ __forceinline__ void c2r3_body(const DiskDev &D, int pair, double2 *L,
+                                          const double2 *__restrict__ in, const double *__restrict__ sc,
+                                          const double2 *__restrict__ tw, double *__restrict__ Pout) {
+  constexpr int n = 64 * P;
+  const int Nu = D.Nu, l = threadIdx.x, r = l & 7, k1 = rfft::conv3::lane_k1<P>(l);
+  const RegPair q = reg_pair<64>(pair, Nu);
+  double2 v[P], x[8];
+  rfft::rows3::passA<P, +1>(l, [&](int m) {
+    return rfft::rows::c2r_in<P, 64>(m, [&](int k, double2 &xa, double2 &xb) {
+      xa = in[(i64)k * Nu + q.ra];
+      const double2 b = in[(i64)k * Nu + q.rb];
+      xb = q.hasb ? b : make_double2(0.0, 0.0);
+    });
+  }, tw, v);
+  rfft::conv3::t1_write<P>(l, v, L);
+  wave_lds_sync();
+  rfft::conv3::t1_read(k1, r, L, x);
+  wave_lds_sync();
+  rfft::rows3::passB<P, +1>(r, tw, x);
+  rfft::conv3::t2_write(k1, r, x, L);
+  wave_lds_sync();
+  rfft::conv3::t2_read(8 * k1 + r, L, x);
+  wave_lds_sync();
+  rfft::rows3::passC_spill<P, +1>(k1, r, x, L);
+  wave_lds_sync();
+  if (q.live) {
+    const double sa = sc[q.ra], sb = sc[q.rb];
+    double *Pa = Pout + (i64)q.ra * n, *Pb = Pout + (i64)q.rb * n;
+    rfft::rows3::unspill<P>(l, L, [&](int j, double2 z) {
+      Pa[j] = z.x * sa;
+      if (q.hasb) Pb[j] = z.y * sb;
+    });
+  }
+}
+
+__global__ void __launch_bounds__(64) k_rows_r2c_reg3(R2C_PARAMS) {
+  extern __shared__ double2 lds2[];
+  if ((int)blockIdx.x >= nrow_blk) {
+    trailing_control(disks, ctl, ctl0 + blockIdx.x - nrow_blk, pcfl, pcm, samples, cap, par, mirror);
+    return;
+  }
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const double *S = stp(st, ntot, par, 0, D), *R32 = gtab(geom, D, GT_R32);
+  const double2 *TW = tw + D.two_row;
+  double2 *out = spec + D.spo;
+  switch (D.Nphi) {
+    case 128: r2c3_body<2>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 192: r2c3_body<3>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 256: r2c3_body<4>(D, wm.y, lds2, S, R32, TW, out); break;
+    default: break;   // the runtime sets row mode 3 only when every Nphi has a case
+  }
+}
+
+__global__ void __launch_bounds__(64) k_rows_c2r_reg3(C2R_PARAMS) {
+  extern __shared__ double2 lds2[];
+  const int2 wm = map[map_base + blockIdx.x];
+  const DiskDev &D = disks[wm.x];
+  if (ctl[wm.x].done) return;
+  const double *sc = gtab(geom, D, GT_PHISC);
+  const double2 *TW = tw + D.two_row, *in = spec + D.spo;
+  double *P = phi + D.so;
+  switch (D.Nphi) {
+    case 128: c2r3_body<2>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 192: c2r3_body<3>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 256: c2r3_body<4>(D, wm.y, lds2, in, sc, TW, P); break;
+    default: break;
+  }
+}
+
 // ------------------------------------------------------------------ K-hat setup
 // Row r2c of the M x Nphi kernel table (kalnajs.hpp:105-114), all M rows, into
 // tmp[Nc][M]; then column transforms keep the real part of rows 0..Nu.
@@ -1696,7 +1846,8 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
     const i64 c = (i64)i * Nphi + j;
     So[c] = so; Uo[c] = uo; Wo[c] = wo;
     sink(lr, j, so);
-    cfl_max<G2>(D, mx, so, uo, wo, tInvdR[i], tInvdl[i], tAface[i], tDl[i]);
+    const double rate = cfl_rate<G2>(D, so, uo, wo, tInvdR[i], tInvdl[i], tAface[i], tDl[i]);
+    mx = (rate > mx) ? rate : mx;
     a0 += so * dA[i];
     a1 += so * c2[j] * dA[i];
     a2 -= so * s2[j] * dA[i];
@@ -2391,6 +2542,11 @@ hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           int nctl, int ctl0, const double *pcfl, const double *pcm,
                           DfkSampleDev *samples, int cap, int inplace, DiskCtl *mirror) {
   if (nblk + nctl <= 0) return hipSuccess;
+  if (inplace == 5) {
+    hipLaunchKernelGGL(k_rows_r2c_reg3, dim3(nblk + nctl), dim3(64), lds, s, disks, map, base, ctl, st,
+                       ntot, geom, tw, spec, par, nblk, ctl0, pcfl, pcm, samples, cap, mirror);
+    return hipGetLastError();
+  }
   if (inplace >= 3) {
     auto k = inplace == 3 ? k_rows_r2c_reg<512> : k_rows_r2c_reg<1024>;
     hipLaunchKernelGGL(k, dim3(nblk + nctl), dim3(64), lds, s, disks, map, base, ctl, st, ntot, geom,
@@ -2464,6 +2620,8 @@ int dfk_conv_reg_cols(int M) {
 
 int dfk_conv_reg3_ok(int M) { return M == 256 || M == 384 || M == 512; }
 
+int dfk_rows_reg3_ok(int n) { return n == 128 || n == 192 || n == 256; }
+
 size_t dfk_conv_reg3_lds() { return (size_t)rfft::conv3::lds_elems() * sizeof(double2); }
 
 size_t dfk_conv_reg_lds(int M) {
@@ -2525,6 +2683,11 @@ hipError_t dfk_launch_c2r(hipStream_t s, int nblk, int nthr, size_t lds, const D
                           const int2 *map, int base, const DiskCtl *ctl, const double2 *spec,
                           const double *geom, const double2 *tw, double *phi, int inplace) {
   if (nblk <= 0) return hipSuccess;
+  if (inplace == 5) {
+    hipLaunchKernelGGL(k_rows_c2r_reg3, dim3(nblk), dim3(64), lds, s, disks, map, base, ctl, spec, geom,
+                       tw, phi);
+    return hipGetLastError();
+  }
   if (inplace >= 3) {
     auto k = inplace == 3 ? k_rows_c2r_reg<512> : k_rows_c2r_reg<1024>;
     hipLaunchKernelGGL(k, dim3(nblk), dim3(64), lds, s, disks, map, base, ctl, spec, geom, tw, phi);

@@ -20,7 +20,8 @@ hipError_t dfk_launch_diag(hipStream_t s, int nblk, const DiskDev *disks, const 
 // inplace: 0 the cooperative row transforms, 1 the in-place ones (k_rows_*_group) with
 // one complex row per workgroup (pr = 1 for every disk), 2 in place with batches,
 // 3 the register two-pass kernels (k_rows_*_reg; every Nphi <= 512, 64 threads,
-// dfk_rows_reg_pairs row pairs per workgroup), 4 the same with Nphi up to 1024.
+// dfk_rows_reg_pairs row pairs per workgroup), 4 the same with Nphi up to 1024,
+// 5 the register three-pass kernels (k_rows_*_reg3, one row pair per workgroup).
 hipError_t dfk_launch_r2c(hipStream_t s, int nblk, int nthr, size_t lds, const DiskDev *disks,
                           const int2 *map, int base, DiskCtl *ctl, const double *st,
                           long long ntot, const double *geom, const double2 *tw, double2 *spec,
@@ -56,6 +57,10 @@ size_t dfk_conv_reg_lds(int M);
 // it, and its dynamic LDS bytes.
 int dfk_conv_reg3_ok(int M);
 size_t dfk_conv_reg3_lds();
+// Three-pass register row transforms (row launch code 5: one row pair per
+// 64-thread workgroup, dfk_conv_reg3_lds bytes of LDS): whether a row of n
+// points (128, 192, 256) has them.
+int dfk_rows_reg3_ok(int n);
 // Register row transforms: row pairs per 64-thread workgroup for a row of n
 // points (0: n unsupported) and the dynamic LDS bytes of such a disk.
 int dfk_rows_reg_pairs(int n);

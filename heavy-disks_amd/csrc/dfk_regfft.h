@@ -517,6 +517,81 @@ RF_HD void passA_inv(int m, double2 (&w)[P], St st) {
 
 }  // namespace conv3
 
+// ------------------------------------------------------------------ row transforms, three passes
+// Row transforms of length n = 64 P (P = 2, 3, 4), one packed row pair per
+// wavefront, by the index splits and transposes of conv3 (all P inputs of
+// pass A nonzero): pass C leaves Z[k1 + P k2 + 8 P k3] on lane (k1, k2),
+// which spills the transform to LDS in natural order.  r2c: rows a, b as
+// real and imaginary parts, forward transform, split into the two half
+// spectra (rows::split's formulas); c2r: the packed spectrum of rows::c2r_in,
+// inverse transform, rows a, b as its real and imaginary parts.
+namespace rows3 {
+
+// pass A, lane m: ld(j) returns x[j], j = 64 n1 + m
+template <int P, int SIGN, class Ld>
+RF_HD void passA(int m, Ld ld, const double2 *tw, double2 (&v)[P]) {
+  sfor<0, P>([&](auto n1_) {
+    constexpr int n1 = decltype(n1_)::value;
+    v[n1] = ld(64 * n1 + m);
+  });
+  const conv::TwRow<P, SIGN> w(tw, m);
+  dft<P, SIGN>(v);
+  sfor<1, P>([&](auto k1_) {
+    constexpr int k1 = decltype(k1_)::value;
+    v[k1] = w.template mul<k1>(v[k1]);
+  });
+}
+
+// pass B, lane (k1, n3): DFT over n2, times W_64^(SIGN n3 k2)
+template <int P, int SIGN>
+RF_HD void passB(int n3, const double2 *tw, double2 (&x)[8]) {
+  const conv::TwRow<8, SIGN> w(tw, P * n3);
+  dft<8, SIGN>(x);
+  sfor<1, 8>([&](auto k2_) {
+    constexpr int k2 = decltype(k2_)::value;
+    x[k2] = w.template mul<k2>(x[k2]);
+  });
+}
+
+// pass C, lane (k1, k2): DFT over n3, the values into L[0, n) in natural order
+template <int P, int SIGN>
+RF_HD void passC_spill(int k1, int k2, double2 (&y)[8], double2 *L) {
+  dft<8, SIGN>(y);
+  sfor<0, 8>([&](auto k3_) {
+    constexpr int k3 = decltype(k3_)::value;
+    L[k1 + P * k2 + 8 * P * k3] = y[k3];
+  });
+}
+
+// r2c split, lane l: bins k = l + 64 j <= n/2 of the two rows from the
+// packed transform Z in L (rows::split's formulas), to st(k, Xa, Xb)
+template <int P, class St>
+RF_HD void split(int l, const double2 *L, St st) {
+  constexpr int n = 64 * P, h = n / 2;
+  sfor<0, (h + 64) / 64>([&](auto j_) {
+    constexpr int j = decltype(j_)::value;
+    const int k = l + 64 * j;
+    if (k <= h) {
+      const double2 z = L[k];
+      double2 y = L[k == 0 ? 0 : n - k];
+      y.y = -y.y;
+      st(k, make_double2(0.5 * (z.x + y.x), 0.5 * (z.y + y.y)),
+         make_double2(0.5 * (z.y - y.y), -0.5 * (z.x - y.x)));
+    }
+  });
+}
+
+// c2r output, lane m: points j = m + 64 a of the packed inverse in L, to st(j, z)
+template <int P, class St>
+RF_HD void unspill(int m, const double2 *L, St st) {
+  sfor<0, P>([&](auto a_) {
+    constexpr int a = decltype(a_)::value;
+    st(m + 64 * a, L[m + 64 * a]);
+  });
+}
+
+}  // namespace rows3
+
 // Row transforms of length n = N1 N2 by the same two passes, N1 <= N2: pass 1
 // on lanes t < N2 (inputs x[N2 n1 + t]), the transpose of conv::fwd_write /
 // fwd_read_c with H = 1, pass 2 on lanes k1 < N1, which then hold bins

@@ -234,6 +234,8 @@ int build_layout(dfk_handle *h) {
   h->rows_ip = h->row_inplace;
   h->rows_big = 0;
   for (const DiskDev &D : h->hD)
+    if (D.selfgrav && h->rows_ip == 3 && !dfk_rows_reg3_ok(D.Nphi)) h->rows_ip = 2;
+  for (const DiskDev &D : h->hD)
     if (D.selfgrav) {
       if (h->rows_ip == 2 && dfk_rows_reg_pairs(D.Nphi) == 0) h->rows_ip = 1;
       h->rows_big |= D.Nphi > 512;
@@ -297,7 +299,7 @@ int build_layout(dfk_handle *h) {
       auto lds_cols = [&](int cb) {
         return (size_t)((D.conv_wave ? 1 : 2) * cb + ((D.tw_lds >> 1) & 1)) * D.M * sizeof(double2);
       };
-      while (h->rows_ip != 2 && D.pr > 1 &&
+      while (h->rows_ip < 2 && D.pr > 1 &&
              (lds_rows(D.pr) > kMaxLds || (h->rows_ip == 1 && D.pr * D.Nphi > ip_nmax(h->thr_rows))))
         --D.pr;
       while (D.cb > 1 && (lds_cols(D.cb) > kMaxLds ||
@@ -305,9 +307,10 @@ int build_layout(dfk_handle *h) {
         --D.cb;
       // register rows: row pairs per workgroup fixed by Nphi, twiddles read
       // through the caches (the disk's own pr is kept for the other modes)
-      const int prb = h->rows_ip == 2 ? dfk_rows_reg_pairs(D.Nphi) : D.pr;
+      const int prb = h->rows_ip == 3 ? 1 : (h->rows_ip == 2 ? dfk_rows_reg_pairs(D.Nphi) : D.pr);
       for (int g = 0; g < ceil_div(D.Nu, 2 * prb); ++g) mr.push_back(make_int2(d, g));
-      lr = std::max(lr, h->rows_ip == 2 ? dfk_rows_reg_lds(D.Nphi) : lds_rows(D.pr));
+      lr = std::max(lr, h->rows_ip == 3 ? dfk_conv_reg3_lds()
+                                        : (h->rows_ip == 2 ? dfk_rows_reg_lds(D.Nphi) : lds_rows(D.pr)));
       if (D.conv_wave >= 3) {   // columns per workgroup fixed by the transform length (one in mode 4)
         const int cpw = D.conv_wave == 4 ? 1 : dfk_conv_reg_cols(D.M);
         for (int g = 0; g < ceil_div(D.Nc, cpw); ++g) mreg.push_back(make_int2(d, g));
@@ -428,6 +431,7 @@ double *field(dfk_handle *h, int buf, int f, int d) {
 
 // Launch variant of the row kernels (dfk_launch_r2c / dfk_launch_c2r).
 int row_kernel(const dfk_handle *h) {
+  if (h->rows_ip == 3) return 5;
   if (h->rows_ip == 2) return h->rows_big ? 4 : 3;
   return h->rows_ip ? 1 + h->rows_batched : 0;
 }
@@ -1557,8 +1561,9 @@ int dfk_set_threads(dfk_handle *h, int rows, int cols, int update) {
 }
 
 int dfk_set_row_transform(dfk_handle *h, int inplace) {
-  if (!h || inplace < 0 || inplace > 2) {
-    set_err("row transform must be 0 (cooperative), 1 (in place) or 2 (register)");
+  if (!h || inplace < 0 || inplace > 3) {
+    set_err("row transform must be 0 (cooperative), 1 (in place), 2 (register) or 3 (register, "
+            "three passes)");
     return DFK_EINVAL;
   }
   if (inplace != h->row_inplace) {

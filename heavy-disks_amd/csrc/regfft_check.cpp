@@ -9,6 +9,7 @@
 //   regfft_check conv M   stdin: column x[r], r < M/2 ("re im"), then K[p], p <= M/2
 //                         stdout: rows r < M/2 of inverse(K x forward(x zero-padded to M))
 //   regfft_check conv3 M  the same by the three-pass column transform (M = 256, 384, 512)
+//   regfft_check r2c3 N / c2r3 N   the row transforms by the three-pass form (N = 128, 192, 256)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -172,6 +173,76 @@ void conv3() {
   for (const double2 &q : out) std::printf("%.17g %.17g\n", q.x, q.y);
 }
 
+// The three-pass row transforms, lane by lane over one wavefront.
+template <int P, int SIGN, class Ld>
+void rows3_fft(Ld ld, const std::vector<double2> &tw, std::vector<double2> &L) {
+  double2 v[64][P], x[64][8];
+  for (int m = 0; m < 64; ++m) {
+    rfft::rows3::passA<P, SIGN>(m, ld, tw.data(), v[m]);
+    rfft::conv3::t1_write<P>(m, v[m], L.data());
+  }
+  for (int l = 0; l < 64; ++l) rfft::conv3::t1_read(rfft::conv3::lane_k1<P>(l), l & 7, L.data(), x[l]);
+  for (int l = 0; l < 64; ++l) {
+    rfft::rows3::passB<P, SIGN>(l & 7, tw.data(), x[l]);
+    rfft::conv3::t2_write(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
+  }
+  for (int l = 0; l < 64; ++l) rfft::conv3::t2_read(8 * rfft::conv3::lane_k1<P>(l) + (l & 7), L.data(), x[l]);
+  for (int l = 0; l < 64; ++l) rfft::rows3::passC_spill<P, SIGN>(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
+}
+
+template <int P>
+void r2c3() {
+  constexpr int n = 64 * P;
+  std::vector<double> a(n), b(n);
+  for (auto &v : a) v = rd();
+  for (auto &v : b) v = rd();
+  const std::vector<double2> tw = table(n);
+  std::vector<double2> L(rfft::conv3::lds_elems(), make_double2(NAN, NAN));
+  rows3_fft<P, -1>([&](int j) { return make_double2(a.at(j), b.at(j)); }, tw, L);
+  std::vector<double2> Xa(n / 2 + 1), Xb(n / 2 + 1);
+  std::vector<int> seen(n / 2 + 1, 0);
+  for (int l = 0; l < 64; ++l)
+    rfft::rows3::split<P>(l, L.data(), [&](int k, double2 p, double2 q) {
+      Xa.at(k) = p;
+      Xb.at(k) = q;
+      seen.at(k) += 1;
+    });
+  for (int k = 0; k <= n / 2; ++k)
+    if (seen[k] != 1) {
+      std::fprintf(stderr, "regfft_check: bin %d written %d times\n", k, seen[k]);
+      std::exit(3);
+    }
+  for (const double2 &z : Xa) std::printf("%.17g %.17g\n", z.x, z.y);
+  for (const double2 &z : Xb) std::printf("%.17g %.17g\n", z.x, z.y);
+}
+
+template <int P>
+void c2r3() {
+  constexpr int n = 64 * P;
+  std::vector<double2> Xa(n / 2 + 1), Xb(n / 2 + 1);
+  for (auto &z : Xa) { z.x = rd(); z.y = rd(); }
+  for (auto &z : Xb) { z.x = rd(); z.y = rd(); }
+  const std::vector<double2> tw = table(n);
+  std::vector<double2> L(rfft::conv3::lds_elems(), make_double2(NAN, NAN));
+  rows3_fft<P, +1>([&](int m) {
+    return rfft::rows::c2r_in<P, 64>(m, [&](int k, double2 &p, double2 &q) { p = Xa.at(k); q = Xb.at(k); });
+  }, tw, L);
+  std::vector<double> a(n, NAN), b(n, NAN);
+  for (int m = 0; m < 64; ++m)
+    rfft::rows3::unspill<P>(m, L.data(), [&](int j, double2 z) { a.at(j) = z.x; b.at(j) = z.y; });
+  for (double z : a) std::printf("%.17g\n", z);
+  for (double z : b) std::printf("%.17g\n", z);
+}
+
+bool rows3_dispatch(bool r2c, int n) {
+  switch (n) {
+    case 128: r2c ? r2c3<2>() : c2r3<2>(); return true;
+    case 192: r2c ? r2c3<3>() : c2r3<3>(); return true;
+    case 256: r2c ? r2c3<4>() : c2r3<4>(); return true;
+    default: return false;
+  }
+}
+
 bool conv3_dispatch(int M) {
   switch (M) {
     case 256: conv3<4>(); return true;
@@ -215,7 +286,7 @@ bool conv_dispatch(int M) {
 
 int main(int argc, char **argv) {
   if (argc != 3) {
-    std::fprintf(stderr, "usage: regfft_check r2c|c2r|conv|conv3 N\n");
+    std::fprintf(stderr, "usage: regfft_check r2c|c2r|conv|conv3|r2c3|c2r3 N\n");
     return 2;
   }
   const int n = std::atoi(argv[2]);
@@ -224,6 +295,8 @@ int main(int argc, char **argv) {
   else if (!std::strcmp(argv[1], "c2r")) ok = rows_dispatch<C2R>(n);
   else if (!std::strcmp(argv[1], "conv")) ok = conv_dispatch(n);
   else if (!std::strcmp(argv[1], "conv3")) ok = conv3_dispatch(n);
+  else if (!std::strcmp(argv[1], "r2c3")) ok = rows3_dispatch(true, n);
+  else if (!std::strcmp(argv[1], "c2r3")) ok = rows3_dispatch(false, n);
   if (!ok) {
     std::fprintf(stderr, "regfft_check: no %s transform of length %d\n", argv[1], n);
     return 2;

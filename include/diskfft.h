@@ -206,12 +206,15 @@ int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update);
  * complex rows with two LDS buffers (default); 1 = in place in one LDS buffer
  * (half the LDS per row; pr x Nphi is then kept <= max(8 x threads, 1024));
  * 2 = two-pass register transforms (one 64-thread wavefront per 4 row pairs
- * for Nphi <= 256, per 2 above; pr and the row threads do not apply).  In
- * place needs every self-gravitating disk's Nphi to factor into 2, 3, 4 and
- * 8, register every such Nphi to be 128, 192, 256, 384, 512, 768 or 1024;
- * otherwise the handle takes the next of register -> in place ->
- * cooperative that all its disks allow, and dfk_get_row_transform reports
- * the variant in use.  All agree to round-off. */
+ * for Nphi <= 256, per 2 above; pr and the row threads do not apply);
+ * 3 = three-pass register transforms (one wavefront per row pair, at most
+ * 8-point DFTs per lane: small, latency-bound grids).  In place needs every
+ * self-gravitating disk's Nphi to factor into 2, 3, 4 and 8, register every
+ * such Nphi to be 128, 192, 256, 384, 512, 768 or 1024, three-pass register
+ * 128, 192 or 256; otherwise the handle takes the next of three-pass ->
+ * register -> in place -> cooperative that all its disks allow, and
+ * dfk_get_row_transform reports the variant in use.  All agree to
+ * round-off. */
 int dfk_set_row_transform(dfk_handle *h, int inplace);
 int dfk_get_row_transform(dfk_handle *h, int *inplace);
 

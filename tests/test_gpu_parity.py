@@ -208,6 +208,32 @@ def test_potential_register_rows(orc, dfk, Nu, Nphi, mode):
     assert relerr(p_both, k.potential(Sig)) < 1e-13
 
 
+@pytest.mark.parametrize("Nu,Nphi,mode", [(64, 128, 3), (37, 192, 3), (256, 256, 3), (3, 128, 3),
+                                          (5, 256, 3), (128, 384, 2), (40, 96, 1)])
+def test_potential_three_pass_rows(orc, dfk, Nu, Nphi, mode):
+    """The three-pass register row transforms (dfk_set_row_transform 3, one
+    row pair per wavefront) at Nphi = 128, 192, 256, with odd Nu and disks of
+    one or two pairs: round-off agreement with the cooperative transforms and
+    the oracle, alone and with the three-pass columns; other Nphi take the
+    next variant their rows allow (same bits as asking for it)."""
+    print(f"case potential {Nu}x{Nphi} three-pass rows", flush=True)
+    e, h, k = _pair(orc, dfk, Nu, Nphi, eps=0.01, Rin=0.1)
+    rng = np.random.default_rng(Nu + 7 * Nphi)
+    Sig = rng.random((Nu, Nphi))
+    p_ref = e.potential(Sig)
+    e.set_row_transform(mode)
+    p_fb = e.potential(Sig)
+    e.set_row_transform(3)
+    assert e.get_row_transform() == mode
+    p_3 = e.potential(Sig)
+    assert np.array_equal(p_3, p_fb)
+    if mode == 3:
+        assert relerr(p_3, p_ref) < 1e-14
+    assert relerr(p_3, k.potential(Sig)) < 1e-13
+    e.set_fft_options(1, 0, 4)
+    assert relerr(e.potential(Sig), k.potential(Sig)) < 1e-13
+
+
 @pytest.mark.parametrize("Nu,Nphi", [(16, 34), (20, 38), (17, 46), (12, 1031), (1024, 1024),
                                      (64, 4096), (2048, 64), (2048, 4096)])
 def test_potential_any_size(orc, dfk, Nu, Nphi):
@@ -447,9 +473,11 @@ def test_powerlaw_ten_step_loop_parity(orc, dfk, Nu, Nphi, kind):
 
 @pytest.mark.parametrize("Nu,Nphi", [(128, 128), (192, 192), (256, 256), (256, 96)])
 def test_three_pass_columns_loop_parity(orc, dfk, Nu, Nphi):
-    """The three-pass register column convolution (column mode 4) in the time
-    loop of the power-law disk: 10 steps within 1e-10 of the oracle, the
-    state within 1e-12 of the same run with the cooperative columns."""
+    """The three-pass register column convolution (column mode 4) and row
+    transforms (row mode 3, or the in-place rows for Nphi = 96, which has no
+    register form) in the time loop of the power-law disk: 10 steps within
+    1e-10 of the oracle, the state within 1e-12 of the same run with the
+    cooperative transforms."""
     print(f"case power-law 10-step loop {Nu}x{Nphi} three-pass columns", flush=True)
     p, ic, mD, h, k = _powerlaw_ic(orc, dfk, Nu, Nphi)
     out = []
@@ -457,6 +485,9 @@ def test_three_pass_columns_loop_parity(orc, dfk, Nu, Nphi):
         e = dfk.Ensemble(p)
         e.set_fft_options(1, 0, mode)
         assert e.get_fft_options(0)[2] == mode
+        if mode == 4:   # with the three-pass rows where Nphi has them (96: in place)
+            e.set_row_transform(3)
+            assert e.get_row_transform() == (3 if Nphi in (128, 192, 256) else 1)
         e.set_state(*ic)
         e.run_init(1e30, dtout=0.25, cfl=0.4, mD=mD, max_steps=10)
         e.advance(12)
@@ -631,6 +662,47 @@ def test_loop_10_steps_inplace_transforms(orc, dfk, Nu, Nphi, eps, rows, cols):
     errs = [relerr(a, b) for a, b in zip(e.get_state(), h.state())]
     print("  rel errors Sigma, vR, vphi:", errs, flush=True)
     assert e.status()["steps"] == 10 and max(errs) <= LOOP_TOL
+
+
+def test_three_pass_rows_ensemble(orc, dfk):
+    """A handle of disks of every three-pass row length (odd and even Nu;
+    cooperative, two- and three-pass register columns), rows in mode 3: each
+    disk the same bits as alone with the same options, and within the loop
+    gate of the oracle."""
+    print("case three-pass rows ensemble", flush=True)
+    shapes = [(64, 128, 0.02), (37, 192, 0.01), (48, 256, 0.02), (128, 128, 0.01), (33, 256, 0.02)]
+    cols = [3, 0, 4, 4, 3]
+    params = [dfk.disk_params(a, b, eps=c) for a, b, c in shapes]
+    ics, mDs, hs = [], [], []
+    for a, b, c in shapes:
+        h, k = _lka98(orc, a, b, c)
+        ics.append(h.state())
+        mDs.append(h.disk_mass())
+        hs.append((h, k))
+
+    def setup(e, ds):
+        e.set_row_transform(3)
+        for q, d in enumerate(ds):
+            e.set_fft_options(1, 0, cols[d], d=q)
+            e.set_state(*ics[d], d=q)
+
+    ens = dfk.Ensemble(params)
+    setup(ens, range(len(shapes)))
+    assert ens.get_row_transform() == 3
+    ens.run_init(1e9, cfl=0.3, mD=mDs, max_steps=10)
+    ens.advance(11)
+    for d, p in enumerate(params):
+        one = dfk.Ensemble(p)
+        setup(one, [d])
+        one.run_init(1e9, cfl=0.3, mD=mDs[d], max_steps=10)
+        one.advance(11)
+        for a, b in zip(ens.get_state(d), one.get_state()):
+            assert np.array_equal(a, b)
+        one.close()
+        h, k = hs[d]
+        orc.run(h, k, 1, 1e9, cfl=0.3, max_steps=10, mD=mDs[d])
+        assert max(state_errs(ens.get_state(d), h.state())) <= LOOP_TOL
+    ens.close()
 
 
 @pytest.mark.parametrize("big", [False, True])

@@ -67,6 +67,30 @@ def test_column_convolution(M):
     assert np.max(np.abs(z - ref)) / np.max(np.abs(ref)) < TOL
 
 
+@pytest.mark.parametrize("n", [128, 192, 256])
+def test_rows_three_pass(n):
+    """The three-pass row transforms (one row pair per wavefront, P-, 8- and
+    8-point DFTs) against numpy: r2c of two real rows, and c2r with the DC
+    and Nyquist imaginary parts dropped; impulses at the rows' ends too."""
+    rng = np.random.default_rng(n + 9)
+    for a, b in ((rng.standard_normal(n), rng.standard_normal(n)),
+                 (np.eye(n)[0], np.eye(n)[n - 1])):
+        v = run("r2c3", n, np.r_[a, b]).reshape(-1, 2)
+        X = v[:, 0] + 1j * v[:, 1]
+        scale = max(np.max(np.abs(np.fft.rfft(a))), np.max(np.abs(np.fft.rfft(b))))
+        assert np.max(np.abs(X[:n // 2 + 1] - np.fft.rfft(a))) / scale < TOL
+        assert np.max(np.abs(X[n // 2 + 1:] - np.fft.rfft(b))) / scale < TOL
+    A, B = np.fft.rfft(rng.standard_normal(n)), np.fft.rfft(rng.standard_normal(n))
+    ra, rb = np.fft.irfft(A, n) * n, np.fft.irfft(B, n) * n
+    A[0] += 0.3j
+    B[-1] -= 0.2j
+    y = run("c2r3", n, np.r_[cplx(A), cplx(B)])
+    assert np.all(np.isfinite(y))
+    scale = np.max(np.abs(ra))
+    assert np.max(np.abs(y[:n] - ra)) / scale < TOL
+    assert np.max(np.abs(y[n:] - rb)) / scale < TOL
+
+
 @pytest.mark.parametrize("M", [256, 384, 512])
 def test_column_convolution_three_pass(M):
     """The three-pass column convolution (one column per wavefront, P-, 8-
