@@ -37,7 +37,7 @@ def candidates(Nu, Nphi):
     M = 2 * Nu
     tiles = [(ti, tj) for ti in (2, 4, 8, 16, 32) for tj in (16, 32, 64, 128, 256)
              if tj <= Nphi and ti <= Nu
-             and 8 * (3 * (ti + 4) * (tj + 4) + 3 * (ti + 1) * (tj + 4)) <= LDS_MAX]
+             and 8 * (3 * (ti + 4) * (tj + 4) + 3 * (ti + 1) * (tj + 4) + 16 * (ti + 4) + 2 * tj) <= LDS_MAX]
     prs = [p for p in (1, 2, 4, 8, 16) if (2 * p + 1) * Nphi * 16 <= LDS_MAX and 2 * p <= Nu]
     cbs = [c for c in (1, 2, 4, 8, 16) if (2 * c + 1) * M * 16 <= LDS_MAX and c <= Nphi // 2 + 1]
     return tiles, prs, cbs

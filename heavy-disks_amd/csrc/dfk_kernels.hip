@@ -926,7 +926,7 @@ __device__ __forceinline__ void c2r_load(const double2 *__restrict__ in, double2
 // TI x n tile), rounded up to whole double2; and of its staged state rows
 // (Sigma on rows i0-3 .. i0+TI+2, v_R and v_phi on i0-2 .. i0+TI+1).
 __host__ __device__ __forceinline__ int fused_tile_doubles(int TI, int n) {
-  return (3 * (TI + 4) * (n + 4) + 3 * (TI + 1) * (n + 4) + 1) & ~1;
+  return (3 * (TI + 4) * (n + 4) + 3 * (TI + 1) * (n + 4) + (TI + 4) * RR_STRIDE + 2 * n + 1) & ~1;
 }
 __host__ __device__ __forceinline__ int fused_stage_doubles(int TI, int n) { return (3 * TI + 14) * n; }
 
@@ -1698,6 +1698,25 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
   const int SW = TJ + 4, SH = TI + 4;
   double *sS = lds, *sU = sS + SH * SW, *sW = sU + SH * SW;
   double *fM = sW + SH * SW, *fU = fM + (TI + 1) * SW, *fW = fU + (TI + 1) * SW;
+  // the row records of rows i0-2 .. i0+TI+1 (rt, RR_STRIDE doubles each;
+  // record lr is row i0-2+lr, clamped into the records 0 .. Nu) and cos 2phi,
+  // sin 2phi of the tile's columns (trg), staged with phase 1's loads so that
+  // the later phases read their row tables from LDS, not from the caches
+  double *rt = fW + (TI + 1) * SW, *trg = rt + SH * RR_STRIDE;
+  {
+    const double *rec0 = gtab(geom, D, GT_COUNT);
+    for (int e = threadIdx.x; e < SH * RR_STRIDE + 2 * TJ; e += blockDim.x) {
+      if (e < SH * RR_STRIDE) {
+        const int i = min(max(i0 - 2 + e / RR_STRIDE, 0), Nu);
+        rt[e] = rec0[i * RR_STRIDE + (e % RR_STRIDE)];
+      } else {
+        const int q = e - SH * RR_STRIDE, sn = q >= TJ ? 1 : 0;
+        const int j = min(j0 + q - sn * TJ, Nphi - 1);
+        trg[q] = trig[D.cos_off + sn * Nphi + j];
+      }
+    }
+  }
+  auto rrow = [&](int lr, int fld) { return rt[lr * RR_STRIDE + fld]; };   // row i0-2+lr
   const double *tInvdRc = gtab(geom, D, GT_INVDRC), *tInvR = gtab(geom, D, GT_INVR),
                *tGstar = gtab(geom, D, GT_GSTAR), *tGrc = gtab(geom, D, GT_GRC),
                *tGpc = gtab(geom, D, GT_GPC);
@@ -1737,9 +1756,6 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
   __syncthreads();
 
   // ---- phase 2: radial faces f = i0..i0+TI (between cells f-1 and f)
-  const double *tRf = gtab(geom, D, GT_RF), *tInvdR = gtab(geom, D, GT_INVDR),
-               *tAf = gtab(geom, D, GT_AF), *tInvVol = gtab(geom, D, GT_INVVOL);
-  (void)tRf;
   for (int e = threadIdx.x; e < (TI + 1) * SW; e += blockDim.x) {
     const int lf = qdiv(e, rcpf((float)SW)), lc = e - lf * SW;
     const int f = i0 + lf;
@@ -1752,10 +1768,10 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
       double fac;
       if (vf > 0) {
         lcell = la; cell = f - 1;
-        fac = 0.5 * (1.0 - vf * dt * tInvdR[f - 1]);
+        fac = 0.5 * (1.0 - vf * dt * rrow(lf + 1, RR_INVDR));
       } else {
         lcell = lb; cell = f;
-        fac = -0.5 * (1.0 + vf * dt * tInvdR[f]);
+        fac = -0.5 * (1.0 + vf * dt * rrow(lf + 2, RR_INVDR));
       }
       double ss = sS[lcell], su = sU[lcell], sw = sW[lcell];
       if (cell > 0 && cell < Nu - 1) {
@@ -1763,7 +1779,7 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
         su += vanleer(sU[lcell + SW] - sU[lcell], sU[lcell] - sU[lcell - SW]) * fac;
         sw += vanleer(sW[lcell + SW] - sW[lcell], sW[lcell] - sW[lcell - SW]) * fac;
       }
-      fm = vf * ss * tAf[f];
+      fm = vf * ss * rrow(lf + 2, RR_AFZ);   // the face area (f interior here)
       fu = fm * su;
       fw = fm * sw;
     }
@@ -1779,7 +1795,7 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
     const int l = (lr + 2) * SW + lc;
     const double s = sS[l];
     const double mR = s * sU[l], mP = s * sW[l];
-    const double iv = dt * tInvVol[i];
+    const double iv = dt * rrow(lr + 2, RR_INVVOL);
     const double ns = s + (fM[e] - fM[e + SW]) * iv;
     const double nmR = mR + (fU[e] - fU[e + SW]) * iv;
     const double nmP = mP + (fW[e] - fW[e + SW]) * iv;
@@ -1789,7 +1805,6 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
   __syncthreads();
 
   // ---- phase 4: azimuthal faces j = j0..j0+TJ (between j-1 and j), rows i0..i0+TI-1
-  const double *tInvdl = gtab(geom, D, GT_INVDL), *tAface = gtab(geom, D, GT_AFACE);
   const int FW = TJ + 1;
   for (int e = threadIdx.x; e < TI * FW; e += blockDim.x) {
     const int lr = qdiv(e, rcpf((float)FW)), lf = e - lr * FW;
@@ -1801,17 +1816,18 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
       const double vf = 0.5 * (sW[la] + sW[lb]);
       int lcell;
       double fac;
+      const double idl = rrow(lr + 2, RR_INVDL);
       if (vf > 0) {
         lcell = la;
-        fac = 0.5 * (1.0 - vf * dt * tInvdl[i]);
+        fac = 0.5 * (1.0 - vf * dt * idl);
       } else {
         lcell = lb;
-        fac = -0.5 * (1.0 + vf * dt * tInvdl[i]);
+        fac = -0.5 * (1.0 + vf * dt * idl);
       }
       const double ss = sS[lcell] + vanleer(sS[lcell + 1] - sS[lcell], sS[lcell] - sS[lcell - 1]) * fac;
       const double su = sU[lcell] + vanleer(sU[lcell + 1] - sU[lcell], sU[lcell] - sU[lcell - 1]) * fac;
       const double sw = sW[lcell] + vanleer(sW[lcell + 1] - sW[lcell], sW[lcell] - sW[lcell - 1]) * fac;
-      fm = vf * ss * tAface[i];
+      fm = vf * ss * rrow(lr + 2, RR_AFACE);
       fu = fm * su;
       fw = fm * sw;
     }
@@ -1822,8 +1838,6 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
   // ---- phase 5: azimuthal gather + floor, write, next-step CFL and C_m partials
   double *So = stpw(st, ntot, 1 - par, 0, D), *Uo = stpw(st, ntot, 1 - par, 1, D),
          *Wo = stpw(st, ntot, 1 - par, 2, D);
-  const double *dA = gtab(geom, D, GT_DA), *tDl = gtab(geom, D, GT_DL);
-  const double *c2 = trig + D.cos_off, *s2 = trig + D.cos_off + Nphi;
   double mx = 1e-30, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (int e = threadIdx.x; e < TI * TJ; e += blockDim.x) {
     const int lr = qdiv(e, rcpf((float)TJ)), lc = e - lr * TJ;
@@ -1836,7 +1850,7 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
     const int f = lr * FW + lc;
     const double s = sS[l];
     const double mR = s * sU[l], mP = s * sW[l];
-    const double iv = dt * tInvVol[i];
+    const double iv = dt * rrow(lr + 2, RR_INVVOL);
     const double ns = s + (fM[f] - fM[f + 1]) * iv;
     const double nmR = mR + (fU[f] - fU[f + 1]) * iv;
     const double nmP = mP + (fW[f] - fW[f + 1]) * iv;
@@ -1846,11 +1860,13 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
     const i64 c = (i64)i * Nphi + j;
     So[c] = so; Uo[c] = uo; Wo[c] = wo;
     sink(lr, j, so);
-    const double rate = cfl_rate<G2>(D, so, uo, wo, tInvdR[i], tInvdl[i], tAface[i], tDl[i]);
+    const double rate = cfl_rate<G2>(D, so, uo, wo, rrow(lr + 2, RR_INVDR), rrow(lr + 2, RR_INVDL),
+                                     rrow(lr + 2, RR_AFACE), rrow(lr + 2, RR_DL));
     mx = (rate > mx) ? rate : mx;
-    a0 += so * dA[i];
-    a1 += so * c2[j] * dA[i];
-    a2 -= so * s2[j] * dA[i];
+    const double dA = rrow(lr + 2, RR_DA);
+    a0 += so * dA;
+    a1 += so * trg[lc] * dA;
+    a2 -= so * trg[TJ + lc] * dA;
     a3 += nonfinite(so, uo, wo);
   }
   wg_reduce5(mx, a0, a1, a2, a3, red);

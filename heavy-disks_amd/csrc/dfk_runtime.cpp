@@ -278,7 +278,9 @@ int build_layout(dfk_handle *h) {
     } else {
       for (int t = 0; t < nt; ++t) mu.push_back(make_int2(d, t));
     }
-    size_t l = (size_t)(3 * (D.ti + 4) * (D.tj + 4) + 3 * (D.ti + 1) * (D.tj + 4)) * sizeof(double);
+    // the tile, its face fluxes, the staged row records and cos/sin of its columns
+    size_t l = (size_t)(3 * (D.ti + 4) * (D.tj + 4) + 3 * (D.ti + 1) * (D.tj + 4) +
+                        (D.ti + 4) * RR_STRIDE + 2 * D.tj) * sizeof(double);
     if (h->upd_kind == 0) lu = std::max(lu, l);
     D.pr = std::max(1, std::min(D.pr, ceil_div(D.Nu, 2)));
     D.cb = std::max(1, std::min(D.cb, D.Nc));
