@@ -34,6 +34,24 @@ static int die(const char *what, int rc) {
   return 1;
 }
 
+// The transform variants the autotuner picks for one disk on MI355X (the
+// driver reads no tuned-shape cache): register row transforms, three-pass
+// where Nphi has them (128, 192, 256), else two-pass, else what the library
+// falls back to; the column convolution likewise (three-pass for 2Nu = 256,
+// 384, 512, else two-pass, else the workgroup in-place kernel, else the
+// cooperative one).  All agree with the defaults to round-off.
+static int fast_transforms(dfk_handle *h) {
+  int rc = dfk_set_row_transform(h, 3);
+  if (rc) return rc;
+  for (int mode : {4, 3, 2}) {
+    int r = 0, c = 0, w = 0;
+    if ((rc = dfk_set_fft_options(h, 0, 1, 1, mode))) return rc;
+    if ((rc = dfk_get_fft_options(h, 0, &r, &c, &w))) return rc;
+    if (w == mode) return 0;
+  }
+  return dfk_set_fft_options(h, 0, 1, 1, 0);
+}
+
 int main(int argc, char **argv) {
   const char *pos[7] = {0};
   int npos = 0;
@@ -85,6 +103,7 @@ int main(int argc, char **argv) {
   dfk_handle *h = nullptr;
   int rc = dfk_create(&p, 1, device, &h);
   if (rc) return die("dfk_create", rc);
+  if ((rc = fast_transforms(h))) return die("transform options", rc);
 
   const size_t N = (size_t)Nu * Nphi;
   std::vector<double> sig0(Nu), gRdisk(Nu, 0.0), Sig(N), vR(N), vp(N);
@@ -111,6 +130,7 @@ int main(int argc, char **argv) {
     dfk_destroy(h);
     p.K = m.K;
     if ((rc = dfk_create(&p, 1, device, &h))) return die("dfk_create", rc);
+    if ((rc = fast_transforms(h))) return die("transform options", rc);
   }
   if ((rc = dfk_set_state(h, 0, Sig.data(), vR.data(), vp.data()))) return die("dfk_set_state", rc);
   const double mD = dfk_model_mass(&m, Nu, Nphi, Sig.data());   // diskmain.cpp:65-70

@@ -67,11 +67,12 @@ class DevBuf:
         self.hip.hipFree(self.p)
 
 
-@pytest.mark.parametrize("modes", [None, (2, 3)])
+@pytest.mark.parametrize("modes", [None, (2, 3), (3, 4)])
 def test_potential_noalloc_matches_potential(dfk, hip, orc, modes):
-    """modes: the handle's default transforms, or the register row (2) and
-    column (3) kernels."""
-    shapes = [(256, 256, 0.01), (64, 96 if modes is None else 128, 0.02)]   # then a smaller disk
+    """modes: the handle's default transforms, the two-pass register row (2)
+    and column (3) kernels, or the three-pass ones (3, 4)."""
+    second = {None: (64, 96), (2, 3): (64, 128), (3, 4): (128, 128)}[modes]
+    shapes = [(256, 256, 0.01), second + (0.02,)]   # then a smaller disk
     print("case potential_noalloc", shapes, "modes", modes, flush=True)
     e = dfk.Ensemble([dfk.disk_params(a, b, eps=c, Rin=0.1) for a, b, c in shapes])
     if modes is not None:
