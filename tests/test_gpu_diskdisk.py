@@ -24,6 +24,8 @@ def _run(exe, args, cwd):
                                   ["48", "36", "0.01", "0.6", "0", "2e-3"],
                                   ["96", "80", "0.005", "0.8"],
                                   ["64", "34", "0.02", "0.3"],          # 34 = 2 x 17
+                                  ["128", "128", "0.02", "0.5"],        # three-pass rows and columns
+                                  ["192", "256", "0.01", "0.2"],        # three-pass, 2Nu = 384
                                   ["1024", "1024", "0.005", "0.0004"]])
 def test_diskdisk_matches_reference_driver(tmp_path, args):
     print("case diskdisk", " ".join(args), flush=True)
