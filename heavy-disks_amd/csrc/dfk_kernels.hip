@@ -1216,41 +1216,57 @@ __device__ __forceinline__ void conv_reg_body(const DiskDev &D, int grp, double2
   });
 }
 
-// The three-pass form (rfft::conv3: M = 64 P, P = 4, 6, 8), one column per
-// 64-thread workgroup: the latency regime's register convolution.  Every
+// The three-pass form (rfft::conv3: M = 64 P, P = 4, 6, 8, 12, 16), one
+// column per 64-thread workgroup (two k1 groups per lane for P > 8).  Every
 // table and K-hat read is issued before the first transpose.
 template <int P>
 __device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L,
                                            const double *__restrict__ khat,
                                            const double2 *__restrict__ tw,
                                            double2 *__restrict__ spec) {
-  constexpr int Nu = 32 * P;
-  const int l = threadIdx.x, r = l & 7, k1 = rfft::conv3::lane_k1<P>(l);
+  constexpr int Nu = 32 * P, G = rfft::conv3::groups<P>();
+  const int l = threadIdx.x, r = l & 7;
+  int k1[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) k1[g] = rfft::conv3::lane_k1<P>(l, g);
   const bool live = col < D.Nc;
   const int cc = live ? col : D.Nc - 1;   // a column past the spectrum reads the last one, stores nothing
   const double2 *TW = tw + D.two_col;
   double2 *io = spec + D.spo + (i64)cc * Nu;
   const double *kh = khat + D.kho + (i64)cc * (Nu + 1);
-  double kv[8];
-  rfft::conv3::khat_vals<P>(k1, r, [&](int p) { return kh[p]; }, kv);
-  double2 v[P], x[8];
+  double kv[G][8];
+#pragma unroll
+  for (int g = 0; g < G; ++g) rfft::conv3::khat_vals<P>(k1[g], r, [&](int p) { return kh[p]; }, kv[g]);
+  double2 v[P], x[G][8];
   rfft::conv3::passA<P>(l, [&](int row) { return io[row]; }, TW, v);
   rfft::conv3::t1_write<P>(l, v, L);
   wave_lds_sync();
-  rfft::conv3::t1_read(k1, r, L, x);
+#pragma unroll
+  for (int g = 0; g < G; ++g) rfft::conv3::t1_read(k1[g], r, L, x[g]);
   wave_lds_sync();
-  rfft::conv3::passB<P>(r, TW, x);
-  rfft::conv3::t2_write(k1, r, x, L);
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    rfft::conv3::passB<P>(r, TW, x[g]);
+    rfft::conv3::t2_write(k1[g], r, x[g], L);
+  }
   wave_lds_sync();
-  rfft::conv3::t2_read(8 * k1 + r, L, x);
+#pragma unroll
+  for (int g = 0; g < G; ++g) rfft::conv3::t2_read(8 * k1[g] + r, L, x[g]);
   wave_lds_sync();
-  rfft::conv3::passC<P>(r, kv, TW, x);
-  rfft::conv3::t2_write(k1, r, x, L);
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    rfft::conv3::passC<P>(r, kv[g], TW, x[g]);
+    rfft::conv3::t2_write(k1[g], r, x[g], L);
+  }
   wave_lds_sync();
-  rfft::conv3::t2_read(8 * k1 + r, L, x);
+#pragma unroll
+  for (int g = 0; g < G; ++g) rfft::conv3::t2_read(8 * k1[g] + r, L, x[g]);
   wave_lds_sync();
-  rfft::conv3::passB_inv(k1, r, TW, x);
-  rfft::conv3::t4_write<P>(k1, r, x, L);
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    rfft::conv3::passB_inv(k1[g], r, TW, x[g]);
+    rfft::conv3::t4_write<P>(k1[g], r, x[g], L);
+  }
   wave_lds_sync();
   rfft::conv3::t4_read<P>(l, L, v);
   rfft::conv3::passA_inv<P>(l, v, [&](int row, double2 q) {
@@ -1276,6 +1292,8 @@ k_cols_conv_reg(const DiskDev *__restrict__ disks, const int2 *__restrict__ map,
       case 256: conv3_body<4>(D, wm.y, lds2, khat, tw, spec); break;
       case 384: conv3_body<6>(D, wm.y, lds2, khat, tw, spec); break;
       case 512: conv3_body<8>(D, wm.y, lds2, khat, tw, spec); break;
+      case 768: conv3_body<12>(D, wm.y, lds2, khat, tw, spec); break;
+      case 1024: conv3_body<16>(D, wm.y, lds2, khat, tw, spec); break;
       default: break;   // the runtime marks no other length mode 4
     }
     return;
@@ -1505,9 +1523,9 @@ __global__ void __launch_bounds__(64) k_rows_c2r_reg(C2R_PARAMS) {
 }
 
 // Row transforms by the three-pass register form (rfft::rows3: n = Nphi =
-// 64 P, P = 2, 3, 4), one row pair per 64-thread workgroup: four times the
-// workgroups of the two-pass kernels and at most 8-point DFTs per lane (the
-// latency regime).  Row pairs as in the two-pass kernels.
+// 64 P, P = 2, 3, 4, 6, 8), one row pair per 64-thread workgroup: two to
+// four times the workgroups of the two-pass kernels and at most 8-point DFTs
+// per lane (the latency regime).  Row pairs as in the two-pass kernels.
 template <int P>
 __device__ __forceinline__ void r2c3_body(const DiskDev &D, int pair, double2 *L,
                                           const double *__restrict__ S, const double *__restrict__ R32,
@@ -1592,6 +1610,8 @@ __global__ void __launch_bounds__(64) k_rows_r2c_reg3(R2C_PARAMS) {
     case 128: r2c3_body<2>(D, wm.y, lds2, S, R32, TW, out); break;
     case 192: r2c3_body<3>(D, wm.y, lds2, S, R32, TW, out); break;
     case 256: r2c3_body<4>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 384: r2c3_body<6>(D, wm.y, lds2, S, R32, TW, out); break;
+    case 512: r2c3_body<8>(D, wm.y, lds2, S, R32, TW, out); break;
     default: break;   // the runtime sets row mode 3 only when every Nphi has a case
   }
 }
@@ -1608,6 +1628,8 @@ __global__ void __launch_bounds__(64) k_rows_c2r_reg3(C2R_PARAMS) {
     case 128: c2r3_body<2>(D, wm.y, lds2, in, sc, TW, P); break;
     case 192: c2r3_body<3>(D, wm.y, lds2, in, sc, TW, P); break;
     case 256: c2r3_body<4>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 384: c2r3_body<6>(D, wm.y, lds2, in, sc, TW, P); break;
+    case 512: c2r3_body<8>(D, wm.y, lds2, in, sc, TW, P); break;
     default: break;
   }
 }
@@ -2634,11 +2656,18 @@ int dfk_conv_reg_cols(int M) {
   return n2 ? 64 / n2 : 0;
 }
 
-int dfk_conv_reg3_ok(int M) { return M == 256 || M == 384 || M == 512; }
+int dfk_conv_reg3_ok(int M) { return M == 256 || M == 384 || M == 512 || M == 768 || M == 1024; }
 
-int dfk_rows_reg3_ok(int n) { return n == 128 || n == 192 || n == 256; }
+int dfk_rows_reg3_ok(int n) { return n == 128 || n == 192 || n == 256 || n == 384 || n == 512; }
 
-size_t dfk_conv_reg3_lds() { return (size_t)rfft::conv3::lds_elems() * sizeof(double2); }
+size_t dfk_conv_reg3_lds(int M) {
+  const int e = M == 1024 ? rfft::conv3::lds_elems<16>()
+              : M == 768  ? rfft::conv3::lds_elems<12>()
+                          : rfft::conv3::lds_elems<8>();
+  return (size_t)e * sizeof(double2);
+}
+
+size_t dfk_rows_reg3_lds() { return (size_t)rfft::conv3::lds_elems<8>() * sizeof(double2); }
 
 size_t dfk_conv_reg_lds(int M) {
   int n1, n2, h;

@@ -53,14 +53,15 @@ hipError_t dfk_launch_conv_group(hipStream_t s, int nblk, int nthr, size_t lds, 
 int dfk_conv_reg_cols(int M);
 size_t dfk_conv_reg_lds(int M);
 // Three-pass register column convolution (column mode 4, one column per
-// 64-thread workgroup in the same launch): whether M (= 256, 384, 512) has
-// it, and its dynamic LDS bytes.
+// 64-thread workgroup in the same launch): whether M (= 256, 384, 512, 768,
+// 1024) has it, and the dynamic LDS bytes a disk of that M needs.
 int dfk_conv_reg3_ok(int M);
-size_t dfk_conv_reg3_lds();
+size_t dfk_conv_reg3_lds(int M);
 // Three-pass register row transforms (row launch code 5: one row pair per
-// 64-thread workgroup, dfk_conv_reg3_lds bytes of LDS): whether a row of n
-// points (128, 192, 256) has them.
+// 64-thread workgroup): whether a row of n points (128, 192, 256, 384, 512)
+// has them, and their dynamic LDS bytes.
 int dfk_rows_reg3_ok(int n);
+size_t dfk_rows_reg3_lds();
 // Register row transforms: row pairs per 64-thread workgroup for a row of n
 // points (0: n unsupported) and the dynamic LDS bytes of such a disk.
 int dfk_rows_reg_pairs(int n);

@@ -361,7 +361,7 @@ RF_HD void pass3(int b, double2 (&z)[N1], St st) {
 }  // namespace conv
 
 // ------------------------------------------------------------------ column convolution, three passes
-// The same zero-padded convolution for M = 64 P (P = 4, 6, 8: Nu = 32 P) by
+// The same zero-padded convolution for M = 64 P (P = 4, 6, 8, 12, 16: Nu = 32 P) by
 // one column per wavefront and three passes of in-register DFTs per
 // direction (P, 8 and 8 points), two LDS transposes per direction, so that
 // no lane holds more than one 8-point DFT's values (latency: the two-pass
@@ -375,17 +375,24 @@ RF_HD void pass3(int b, double2 (&z)[N1], St st) {
 //     pass B' (lane 8 k1 + c):      E[b] = W_M^(-(8 b + c) k1) IDFT_8(C[.])[b]        -> T1 [k1][8 b + c]
 //     pass A' (lane m = 8 b + c):   y[m + 64 a] = IDFT_P(E[.][m])[a], a < P/2
 // (the exponents of W_M that the index splits leave are multiples of M).
-// Passes B to B' use lanes 8 k1 + r, k1 < P; for P < 8 the lanes above 8 P
-// take k1 = P - 1 and repeat that lane's work (same reads, same values
-// written), so every access stays inside the tables and the LDS area.
+// Passes B to B' run on lanes 8 k1 + r for k1 < 8 and, for P > 8, on the
+// same lanes again for k1 + 8 (two groups of eight values per lane); a
+// (lane, group) whose k1 reaches P takes k1 = P - 1 and repeats that lane's
+// work (same reads, same values written), so every access stays inside the
+// tables and the LDS area.
 namespace conv3 {
 
-constexpr int S1 = 65, S2 = 9;   // row strides of T1 ([P][64]) and T2 ([64][8]) in double2
-constexpr int lds_elems() { return 64 * S2; }   // >= 8 * S1
-
-// lane l's k1 in passes B .. B'
+constexpr int S1 = 65, S2 = 9;   // row strides of T1 ([P][64]) and T2 ([8 P][8]) in double2
 template <int P>
-RF_HD int lane_k1(int l) { return (l >> 3) < P ? (l >> 3) : P - 1; }
+constexpr int lds_elems() {      // T1 and T2 (at least 64 rows) share the area
+  return P * S1 > (8 * P > 64 ? 8 * P : 64) * S2 ? P * S1 : (8 * P > 64 ? 8 * P : 64) * S2;
+}
+
+// k1 groups per lane in passes B .. B', and lane l's k1 in group g
+template <int P>
+constexpr int groups() { return (P + 7) / 8; }
+template <int P>
+RF_HD int lane_k1(int l, int g = 0) { return (l >> 3) + 8 * g < P ? (l >> 3) + 8 * g : P - 1; }
 
 // pass A, lane m: ld(row) returns row `row` (< Nu) of the column
 template <int P, class Ld>
@@ -518,7 +525,7 @@ RF_HD void passA_inv(int m, double2 (&w)[P], St st) {
 }  // namespace conv3
 
 // ------------------------------------------------------------------ row transforms, three passes
-// Row transforms of length n = 64 P (P = 2, 3, 4), one packed row pair per
+// Row transforms of length n = 64 P (P = 2, 3, 4, 6, 8), one packed row pair per
 // wavefront, by the index splits and transposes of conv3 (all P inputs of
 // pass A nonzero): pass C leaves Z[k1 + P k2 + 8 P k3] on lane (k1, k2),
 // which spills the transform to LDS in natural order.  r2c: rows a, b as

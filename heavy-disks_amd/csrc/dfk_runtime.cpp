@@ -311,12 +311,12 @@ int build_layout(dfk_handle *h) {
       // through the caches (the disk's own pr is kept for the other modes)
       const int prb = h->rows_ip == 3 ? 1 : (h->rows_ip == 2 ? dfk_rows_reg_pairs(D.Nphi) : D.pr);
       for (int g = 0; g < ceil_div(D.Nu, 2 * prb); ++g) mr.push_back(make_int2(d, g));
-      lr = std::max(lr, h->rows_ip == 3 ? dfk_conv_reg3_lds()
+      lr = std::max(lr, h->rows_ip == 3 ? dfk_rows_reg3_lds()
                                         : (h->rows_ip == 2 ? dfk_rows_reg_lds(D.Nphi) : lds_rows(D.pr)));
       if (D.conv_wave >= 3) {   // columns per workgroup fixed by the transform length (one in mode 4)
         const int cpw = D.conv_wave == 4 ? 1 : dfk_conv_reg_cols(D.M);
         for (int g = 0; g < ceil_div(D.Nc, cpw); ++g) mreg.push_back(make_int2(d, g));
-        lcr = std::max(lcr, D.conv_wave == 4 ? dfk_conv_reg3_lds() : dfk_conv_reg_lds(D.M));
+        lcr = std::max(lcr, D.conv_wave == 4 ? dfk_conv_reg3_lds(D.M) : dfk_conv_reg_lds(D.M));
       } else {
         std::vector<int2> &cm = D.conv_wave == 2 ? mcg : (D.conv_wave == 1 ? mcw : mc);
         for (int g = 0; g < ceil_div(D.Nc, D.cb); ++g) cm.push_back(make_int2(d, g));

@@ -36,7 +36,7 @@ static int die(const char *what, int rc) {
 
 // The transform variants the autotuner picks for one disk on MI355X (the
 // driver reads no tuned-shape cache): register row transforms, three-pass
-// where Nphi has them (128, 192, 256), else two-pass, else what the library
+// where Nphi has them (128 to 512), else two-pass, else what the library
 // falls back to; the column convolution likewise (three-pass for 2Nu = 256,
 // 384, 512, else two-pass, else the workgroup in-place kernel, else the
 // cooperative one).  All agree with the defaults to round-off.

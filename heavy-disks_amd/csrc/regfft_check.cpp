@@ -8,8 +8,9 @@
 //                         stdout: rows a, b (N reals each, unnormalised inverse)
 //   regfft_check conv M   stdin: column x[r], r < M/2 ("re im"), then K[p], p <= M/2
 //                         stdout: rows r < M/2 of inverse(K x forward(x zero-padded to M))
-//   regfft_check conv3 M  the same by the three-pass column transform (M = 256, 384, 512)
-//   regfft_check r2c3 N / c2r3 N   the row transforms by the three-pass form (N = 128, 192, 256)
+//   regfft_check conv3 M  the same by the three-pass column transform (M = 256 .. 1024)
+//   regfft_check r2c3 N / c2r3 N   the row transforms by the three-pass form (N = 128 .. 512)
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -142,33 +143,45 @@ void conv3() {
   std::vector<double> K(Nu + 1);
   for (auto &k : K) k = rd();
   const std::vector<double2> tw = table(M);
-  std::vector<double2> L(rfft::conv3::lds_elems(), make_double2(NAN, NAN)), out(Nu, make_double2(NAN, NAN));
-  double2 v[64][P], x[64][8];
-  double kv[64][8];
+  constexpr int G = rfft::conv3::groups<P>();
+  std::vector<double2> L(rfft::conv3::lds_elems<P>(), make_double2(NAN, NAN)), out(Nu, make_double2(NAN, NAN));
+  std::vector<std::array<double2, P>> v(64);
+  double2 x[64][G][8];
+  double kv[64][G][8];
+  auto k1 = [](int l, int g) { return rfft::conv3::lane_k1<P>(l, g); };
   for (int l = 0; l < 64; ++l)
-    rfft::conv3::khat_vals<P>(rfft::conv3::lane_k1<P>(l), l & 7, [&](int p) { return K.at(p); }, kv[l]);
+    for (int g = 0; g < G; ++g)
+      rfft::conv3::khat_vals<P>(k1(l, g), l & 7, [&](int p) { return K.at(p); }, kv[l][g]);
   for (int m = 0; m < 64; ++m) {
-    rfft::conv3::passA<P>(m, [&](int r) { return col.at(r); }, tw.data(), v[m]);
-    rfft::conv3::t1_write<P>(m, v[m], L.data());
+    double2(&vm)[P] = *reinterpret_cast<double2(*)[P]>(v[m].data());
+    rfft::conv3::passA<P>(m, [&](int r) { return col.at(r); }, tw.data(), vm);
+    rfft::conv3::t1_write<P>(m, vm, L.data());
   }
-  for (int l = 0; l < 64; ++l) rfft::conv3::t1_read(rfft::conv3::lane_k1<P>(l), l & 7, L.data(), x[l]);
-  for (int l = 0; l < 64; ++l) {
-    rfft::conv3::passB<P>(l & 7, tw.data(), x[l]);
-    rfft::conv3::t2_write(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
-  }
-  for (int l = 0; l < 64; ++l) rfft::conv3::t2_read(8 * rfft::conv3::lane_k1<P>(l) + (l & 7), L.data(), x[l]);
-  for (int l = 0; l < 64; ++l) {
-    rfft::conv3::passC<P>(l & 7, kv[l], tw.data(), x[l]);
-    rfft::conv3::t2_write(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
-  }
-  for (int l = 0; l < 64; ++l) rfft::conv3::t2_read(8 * rfft::conv3::lane_k1<P>(l) + (l & 7), L.data(), x[l]);
-  for (int l = 0; l < 64; ++l) {
-    rfft::conv3::passB_inv(rfft::conv3::lane_k1<P>(l), l & 7, tw.data(), x[l]);
-    rfft::conv3::t4_write<P>(rfft::conv3::lane_k1<P>(l), l & 7, x[l], L.data());
-  }
+  for (int l = 0; l < 64; ++l)
+    for (int g = 0; g < G; ++g) rfft::conv3::t1_read(k1(l, g), l & 7, L.data(), x[l][g]);
+  for (int l = 0; l < 64; ++l)
+    for (int g = 0; g < G; ++g) {
+      rfft::conv3::passB<P>(l & 7, tw.data(), x[l][g]);
+      rfft::conv3::t2_write(k1(l, g), l & 7, x[l][g], L.data());
+    }
+  for (int l = 0; l < 64; ++l)
+    for (int g = 0; g < G; ++g) rfft::conv3::t2_read(8 * k1(l, g) + (l & 7), L.data(), x[l][g]);
+  for (int l = 0; l < 64; ++l)
+    for (int g = 0; g < G; ++g) {
+      rfft::conv3::passC<P>(l & 7, kv[l][g], tw.data(), x[l][g]);
+      rfft::conv3::t2_write(k1(l, g), l & 7, x[l][g], L.data());
+    }
+  for (int l = 0; l < 64; ++l)
+    for (int g = 0; g < G; ++g) rfft::conv3::t2_read(8 * k1(l, g) + (l & 7), L.data(), x[l][g]);
+  for (int l = 0; l < 64; ++l)
+    for (int g = 0; g < G; ++g) {
+      rfft::conv3::passB_inv(k1(l, g), l & 7, tw.data(), x[l][g]);
+      rfft::conv3::t4_write<P>(k1(l, g), l & 7, x[l][g], L.data());
+    }
   for (int m = 0; m < 64; ++m) {
-    rfft::conv3::t4_read<P>(m, L.data(), v[m]);
-    rfft::conv3::passA_inv<P>(m, v[m], [&](int r, double2 q) { out.at(r) = q; });
+    double2(&vm)[P] = *reinterpret_cast<double2(*)[P]>(v[m].data());
+    rfft::conv3::t4_read<P>(m, L.data(), vm);
+    rfft::conv3::passA_inv<P>(m, vm, [&](int r, double2 q) { out.at(r) = q; });
   }
   for (const double2 &q : out) std::printf("%.17g %.17g\n", q.x, q.y);
 }
@@ -197,7 +210,7 @@ void r2c3() {
   for (auto &v : a) v = rd();
   for (auto &v : b) v = rd();
   const std::vector<double2> tw = table(n);
-  std::vector<double2> L(rfft::conv3::lds_elems(), make_double2(NAN, NAN));
+  std::vector<double2> L(rfft::conv3::lds_elems<8>(), make_double2(NAN, NAN));
   rows3_fft<P, -1>([&](int j) { return make_double2(a.at(j), b.at(j)); }, tw, L);
   std::vector<double2> Xa(n / 2 + 1), Xb(n / 2 + 1);
   std::vector<int> seen(n / 2 + 1, 0);
@@ -223,7 +236,7 @@ void c2r3() {
   for (auto &z : Xa) { z.x = rd(); z.y = rd(); }
   for (auto &z : Xb) { z.x = rd(); z.y = rd(); }
   const std::vector<double2> tw = table(n);
-  std::vector<double2> L(rfft::conv3::lds_elems(), make_double2(NAN, NAN));
+  std::vector<double2> L(rfft::conv3::lds_elems<8>(), make_double2(NAN, NAN));
   rows3_fft<P, +1>([&](int m) {
     return rfft::rows::c2r_in<P, 64>(m, [&](int k, double2 &p, double2 &q) { p = Xa.at(k); q = Xb.at(k); });
   }, tw, L);
@@ -239,6 +252,8 @@ bool rows3_dispatch(bool r2c, int n) {
     case 128: r2c ? r2c3<2>() : c2r3<2>(); return true;
     case 192: r2c ? r2c3<3>() : c2r3<3>(); return true;
     case 256: r2c ? r2c3<4>() : c2r3<4>(); return true;
+    case 384: r2c ? r2c3<6>() : c2r3<6>(); return true;
+    case 512: r2c ? r2c3<8>() : c2r3<8>(); return true;
     default: return false;
   }
 }
@@ -248,6 +263,8 @@ bool conv3_dispatch(int M) {
     case 256: conv3<4>(); return true;
     case 384: conv3<6>(); return true;
     case 512: conv3<8>(); return true;
+    case 768: conv3<12>(); return true;
+    case 1024: conv3<16>(); return true;
     default: return false;
   }
 }

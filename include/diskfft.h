@@ -187,8 +187,8 @@ int dfk_get_shapes(dfk_handle *h, int disk, int *ti, int *tj, int *pr, int *cb);
  *       (otherwise read back as 0); it ignores cb and cols_tw_lds; 4 = three
  *       passes of at most 8-point in-register DFTs per direction, one column
  *       per wavefront (shorter per-lane chains for small, latency-bound
- *       grids), for 2Nu in {256, 384, 512} only (otherwise read back as 0),
- *       ignoring cb and cols_tw_lds.
+ *       grids), for 2Nu in {256, 384, 512, 768, 1024} only (otherwise read
+ *       back as 0), ignoring cb and cols_tw_lds.
  * The twiddle placement does not change a bit of the results; the in-place
  * and register kernels agree with the cooperative one to round-off. */
 int dfk_set_fft_options(dfk_handle *h, int disk, int rows_tw_lds, int cols_tw_lds, int cols_wave);
@@ -211,7 +211,7 @@ int dfk_get_threads(dfk_handle *h, int *rows, int *cols, int *update);
  * 8-point DFTs per lane: small, latency-bound grids).  In place needs every
  * self-gravitating disk's Nphi to factor into 2, 3, 4 and 8, register every
  * such Nphi to be 128, 192, 256, 384, 512, 768 or 1024, three-pass register
- * 128, 192 or 256; otherwise the handle takes the next of three-pass ->
+ * 128, 192, 256, 384 or 512; otherwise the handle takes the next of three-pass ->
  * register -> in place -> cooperative that all its disks allow, and
  * dfk_get_row_transform reports the variant in use.  All agree to
  * round-off. */

@@ -157,10 +157,12 @@ def test_potential_register_columns(orc, dfk, Nu, Nphi, ok):
 
 
 @pytest.mark.parametrize("Nu,Nphi,ok", [(128, 128, 1), (192, 192, 1), (256, 256, 1), (256, 30, 1),
-                                        (128, 1024, 1), (64, 64, 0), (512, 512, 0), (35, 40, 0)])
+                                        (128, 1024, 1), (384, 512, 1), (512, 512, 1), (512, 34, 1),
+                                        (64, 64, 0), (1024, 64, 0), (35, 40, 0)])
 def test_potential_three_pass_columns(orc, dfk, Nu, Nphi, ok):
     """Column mode 4, the three-pass register convolution (one column per
-    wavefront, 2Nu = 256, 384, 512): round-off agreement with the
+    wavefront, 2Nu = 256, 384, 512, 768, 1024; two k1 groups per lane from
+    768): round-off agreement with the
     cooperative transform and the oracle; other lengths keep the cooperative
     one (read back as 0), the same bits."""
     print(f"case potential {Nu}x{Nphi} three-pass columns", flush=True)
@@ -209,11 +211,12 @@ def test_potential_register_rows(orc, dfk, Nu, Nphi, mode):
 
 
 @pytest.mark.parametrize("Nu,Nphi,mode", [(64, 128, 3), (37, 192, 3), (256, 256, 3), (3, 128, 3),
-                                          (5, 256, 3), (128, 384, 2), (40, 96, 1)])
+                                          (5, 256, 3), (128, 384, 3), (33, 512, 3), (64, 1024, 2),
+                                          (40, 96, 1)])
 def test_potential_three_pass_rows(orc, dfk, Nu, Nphi, mode):
     """The three-pass register row transforms (dfk_set_row_transform 3, one
-    row pair per wavefront) at Nphi = 128, 192, 256, with odd Nu and disks of
-    one or two pairs: round-off agreement with the cooperative transforms and
+    row pair per wavefront) at Nphi = 128, 192, 256, 384, 512, with odd Nu and
+    disks of one or two pairs: round-off agreement with the cooperative transforms and
     the oracle, alone and with the three-pass columns; other Nphi take the
     next variant their rows allow (same bits as asking for it)."""
     print(f"case potential {Nu}x{Nphi} three-pass rows", flush=True)
