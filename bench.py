@@ -216,7 +216,7 @@ def main():
                     help="column-convolution mode of every ensemble disk (dfk_set_fft_options)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+        print("bench.py: measures one GPU; multi-GPU runs are not part of it "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
     import diskfft_amd as dfk

@@ -48,7 +48,7 @@ def _key(Nu, Nphi, regime, device_name):
 
 
 def device_name():
-    """This build targets gfx950 only; the cache key keeps the field so tuned
+    """The library targets gfx950 only; the cache key keeps the field so tuned
     shapes from another architecture are never applied by mistake."""
     return "gfx950"
 

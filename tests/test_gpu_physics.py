@@ -6,7 +6,7 @@ The bar: gamma and Omega_p within 1% of the FP64 CPU path from the same
 state (achieved: 1e-12 relative).  The reference's own 128 x 128 setup
 (diskfft/README.md:27-30: LKA98 ring, tlim 9, amp 1e-3) runs in a couple of
 seconds on both sides; its published numbers (diskfft/README.md:27,37;
-README.md:42) are checked at the agreement this build reaches with them
+README.md:42) are checked at the agreement the library reaches with them
 (their fit window is not published, so a few percent).
 """
 import os

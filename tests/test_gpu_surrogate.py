@@ -19,7 +19,7 @@ Statistic per output (relative error |p - h| / |h| over the held-out runs
 whose |h| exceeds 5% of the training targets' 5-95% range, as
 surrogate_fit.errors reports it):
   - with the hydro fallback where the members' spread exceeds 5% (those
-    points take the hydro value, the north star's system): mean <= 3% for
+    points take the hydro value, as the surrogate-with-fallback scan does): mean <= 3% for
     each output (recorded 0.24%, 0.06%, 2.1%);
   - surrogate alone: median <= 3% for each output (recorded 0.30%, 0.09%,
     2.7%);
