@@ -8,7 +8,7 @@ the training set's median |y|, then standardised with the mean and deviation
 of the compressed training values.
 Where the members disagree by more than `spread_tol` (relative deviation of
 any output across members) the caller should fall back to the hydro, which
-stays the judge of the surrogate's error.
+remains the measure of the surrogate's error.
 """
 import ctypes
 import os
