@@ -1217,8 +1217,9 @@ __device__ __forceinline__ void conv_reg_body(const DiskDev &D, int grp, double2
 }
 
 // The three-pass form (rfft::conv3: M = 64 P, P = 4, 6, 8, 12, 16), one
-// column per 64-thread workgroup (two k1 groups per lane for P > 8).  Every
-// table and K-hat read is issued before the first transpose.
+// column per 64-thread workgroup (two k1 groups per lane for P > 8).  The
+// K-hat values are loaded with the column, before pass A; each pass loads
+// its own twiddles.
 template <int P>
 __device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L,
                                            const double *__restrict__ khat,
