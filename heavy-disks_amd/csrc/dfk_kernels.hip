@@ -1218,8 +1218,10 @@ __device__ __forceinline__ void conv_reg_body(const DiskDev &D, int grp, double2
 
 // The three-pass form (rfft::conv3: M = 64 P, P = 4, 6, 8, 12, 16), one
 // column per 64-thread workgroup (two k1 groups per lane for P > 8).  The
-// K-hat values are loaded with the column, before pass A; each pass loads
-// its own twiddles.
+// K-hat values are loaded with the column, before pass A, and so are the
+// later passes' twiddles for P <= 8, so that no pass waits on a load after a
+// transpose (256^2 disk: 3.86 -> 3.53 us); with two k1 groups the passes
+// load their own (preloading measured 7.2 -> 8.0 us at 2Nu = 1024).
 template <int P>
 __device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L,
                                            const double *__restrict__ khat,
@@ -1238,6 +1240,12 @@ __device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L
   double kv[G][8];
 #pragma unroll
   for (int g = 0; g < G; ++g) rfft::conv3::khat_vals<P>(k1[g], r, [&](int p) { return kh[p]; }, kv[g]);
+  constexpr bool PRE = G == 1;
+  // passes B and C: W_64^(-+ r k); pass B' (preloaded for P <= 8)
+  const rfft::conv::TwRow<8, -1> wB = PRE ? rfft::conv::TwRow<8, -1>(TW, P * r) : rfft::conv::TwRow<8, -1>();
+  const rfft::conv::TwRow<8, +1> wC = PRE ? rfft::conv::TwRow<8, +1>(TW, P * r) : rfft::conv::TwRow<8, +1>();
+  double2 wI[G][8];
+  if constexpr (PRE) rfft::conv3::binv_tw(k1[0], r, TW, wI[0]);
   double2 v[P], x[G][8];
   rfft::conv3::passA<P>(l, [&](int row) { return io[row]; }, TW, v);
   rfft::conv3::t1_write<P>(l, v, L);
@@ -1247,7 +1255,8 @@ __device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L
   wave_lds_sync();
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    rfft::conv3::passB<P>(r, TW, x[g]);
+    if constexpr (PRE) rfft::conv3::passB<P>(wB, x[g]);
+    else rfft::conv3::passB<P>(r, TW, x[g]);
     rfft::conv3::t2_write(k1[g], r, x[g], L);
   }
   wave_lds_sync();
@@ -1256,7 +1265,8 @@ __device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L
   wave_lds_sync();
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    rfft::conv3::passC<P>(r, kv[g], TW, x[g]);
+    if constexpr (PRE) rfft::conv3::passC<P>(kv[g], wC, x[g]);
+    else rfft::conv3::passC<P>(r, kv[g], TW, x[g]);
     rfft::conv3::t2_write(k1[g], r, x[g], L);
   }
   wave_lds_sync();
@@ -1265,7 +1275,8 @@ __device__ __forceinline__ void conv3_body(const DiskDev &D, int col, double2 *L
   wave_lds_sync();
 #pragma unroll
   for (int g = 0; g < G; ++g) {
-    rfft::conv3::passB_inv(k1[g], r, TW, x[g]);
+    if constexpr (PRE) rfft::conv3::passB_inv(wI[g], x[g]);
+    else rfft::conv3::passB_inv(k1[g], r, TW, x[g]);
     rfft::conv3::t4_write<P>(k1[g], r, x[g], L);
   }
   wave_lds_sync();
@@ -1536,6 +1547,7 @@ __device__ __forceinline__ void r2c3_body(const DiskDev &D, int pair, double2 *L
   const RegPair q = reg_pair<64>(pair, Nu);
   const double pa = R32[q.ra], pb = R32[q.rb];
   const double *Sa = S + (i64)q.ra * n, *Sb = S + (i64)q.rb * n;
+  const rfft::conv::TwRow<8, -1> wB(tw, P * r);   // pass B's twiddles, loaded with the rows
   double2 v[P], x[8];
   rfft::rows3::passA<P, -1>(l, [&](int j) {
     const double b = pb * Sb[j];
@@ -1545,7 +1557,7 @@ __device__ __forceinline__ void r2c3_body(const DiskDev &D, int pair, double2 *L
   wave_lds_sync();
   rfft::conv3::t1_read(k1, r, L, x);
   wave_lds_sync();
-  rfft::rows3::passB<P, -1>(r, tw, x);
+  rfft::rows3::passB<P, -1>(wB, x);
   rfft::conv3::t2_write(k1, r, x, L);
   wave_lds_sync();
   rfft::conv3::t2_read(8 * k1 + r, L, x);
@@ -1566,6 +1578,7 @@ __device__ __forceinline__ void c2r3_body(const DiskDev &D, int pair, double2 *L
   constexpr int n = 64 * P;
   const int Nu = D.Nu, l = threadIdx.x, r = l & 7, k1 = rfft::conv3::lane_k1<P>(l);
   const RegPair q = reg_pair<64>(pair, Nu);
+  const rfft::conv::TwRow<8, +1> wB(tw, P * r);   // pass B's twiddles, loaded with the bins
   double2 v[P], x[8];
   rfft::rows3::passA<P, +1>(l, [&](int m) {
     return rfft::rows::c2r_in<P, 64>(m, [&](int k, double2 &xa, double2 &xb) {
@@ -1578,7 +1591,7 @@ __device__ __forceinline__ void c2r3_body(const DiskDev &D, int pair, double2 *L
   wave_lds_sync();
   rfft::conv3::t1_read(k1, r, L, x);
   wave_lds_sync();
-  rfft::rows3::passB<P, +1>(r, tw, x);
+  rfft::rows3::passB<P, +1>(wB, x);
   rfft::conv3::t2_write(k1, r, x, L);
   wave_lds_sync();
   rfft::conv3::t2_read(8 * k1 + r, L, x);

@@ -235,6 +235,7 @@ template <int K, int SIGN>
 struct TwRow {
   static constexpr int NL = K < 8 ? K : 8, NH = (K + 7) / 8;
   double2 lo[NL], hi[NH];
+  RF_HD TwRow() {}   // unset (a row the caller does not use)
   RF_HD TwRow(const double2 *tw, int m) {
     sfor<0, NL>([&](auto j_) {
       constexpr int j = decltype(j_)::value;
@@ -424,15 +425,19 @@ RF_HD void t1_read(int k1, int n3, const double2 *L, double2 (&x)[8]) {
   });
 }
 
-// pass B, lane (k1, n3)
+// pass B, lane (k1, n3), with the twiddle row W_M^(-P n3 k2) = W_64^(-n3 k2)
+// already loaded (conv::TwRow<8, -1>(tw, P n3)), or loading it
 template <int P>
-RF_HD void passB(int n3, const double2 *tw, double2 (&x)[8]) {
-  const conv::TwRow<8, -1> w(tw, P * n3);
+RF_HD void passB(const conv::TwRow<8, -1> &w, double2 (&x)[8]) {
   dft<8, -1>(x);
   sfor<1, 8>([&](auto k2_) {
     constexpr int k2 = decltype(k2_)::value;
     x[k2] = w.template mul<k2>(x[k2]);
   });
+}
+template <int P>
+RF_HD void passB(int n3, const double2 *tw, double2 (&x)[8]) {
+  passB<P>(conv::TwRow<8, -1>(tw, P * n3), x);
 }
 
 // T2: lane (k1, r) writes its x[q] to row 8 k1 + q, column r; lane 8 k1 + q
@@ -465,8 +470,7 @@ RF_HD void khat_vals(int k1, int k2, Kh kh, double (&kv)[8]) {
 // pass C, lane (k1, k2): forward DFT over n3, times K, inverse DFT over k3,
 // times W_64^(-c k2)
 template <int P>
-RF_HD void passC(int k2, const double (&kv)[8], const double2 *tw, double2 (&y)[8]) {
-  const conv::TwRow<8, +1> w(tw, P * k2);
+RF_HD void passC(const double (&kv)[8], const conv::TwRow<8, +1> &w, double2 (&y)[8]) {
   dft<8, -1>(y);
   sfor<0, 8>([&](auto k3_) {
     constexpr int k3 = decltype(k3_)::value;
@@ -479,16 +483,30 @@ RF_HD void passC(int k2, const double (&kv)[8], const double2 *tw, double2 (&y)[
     y[c] = w.template mul<c>(y[c]);
   });
 }
+template <int P>
+RF_HD void passC(int k2, const double (&kv)[8], const double2 *tw, double2 (&y)[8]) {
+  passC<P>(kv, conv::TwRow<8, +1>(tw, P * k2), y);
+}
 
 // pass B', lane (k1, c): inverse DFT over k2, times W_M^(-(8 b + c) k1)
-// (table values, (8 b + c) k1 < M)
-RF_HD void passB_inv(int k1, int c, const double2 *tw, double2 (&z)[8]) {
-  double2 w[8];
+// (table values, (8 b + c) k1 < M), loaded by binv_tw or by the pass
+RF_HD void binv_tw(int k1, int c, const double2 *tw, double2 (&w)[8]) {
   sfor<0, 8>([&](auto b_) {
     constexpr int b = decltype(b_)::value;
     w[b] = tw[(8 * b + c) * k1];
     w[b].y = -w[b].y;
   });
+}
+RF_HD void passB_inv(const double2 (&w)[8], double2 (&z)[8]) {
+  dft<8, +1>(z);
+  sfor<0, 8>([&](auto b_) {
+    constexpr int b = decltype(b_)::value;
+    z[b] = cmul(z[b], w[b]);
+  });
+}
+RF_HD void passB_inv(int k1, int c, const double2 *tw, double2 (&z)[8]) {
+  double2 w[8];
+  binv_tw(k1, c, tw, w);
   dft<8, +1>(z);
   sfor<0, 8>([&](auto b_) {
     constexpr int b = decltype(b_)::value;
@@ -549,7 +567,16 @@ RF_HD void passA(int m, Ld ld, const double2 *tw, double2 (&v)[P]) {
   });
 }
 
-// pass B, lane (k1, n3): DFT over n2, times W_64^(SIGN n3 k2)
+// pass B, lane (k1, n3): DFT over n2, times W_64^(SIGN n3 k2) (the twiddle
+// row conv::TwRow<8, SIGN>(tw, P n3), loaded by the caller or by the pass)
+template <int P, int SIGN>
+RF_HD void passB(const conv::TwRow<8, SIGN> &w, double2 (&x)[8]) {
+  dft<8, SIGN>(x);
+  sfor<1, 8>([&](auto k2_) {
+    constexpr int k2 = decltype(k2_)::value;
+    x[k2] = w.template mul<k2>(x[k2]);
+  });
+}
 template <int P, int SIGN>
 RF_HD void passB(int n3, const double2 *tw, double2 (&x)[8]) {
   const conv::TwRow<8, SIGN> w(tw, P * n3);
