@@ -51,6 +51,17 @@ __device__ __forceinline__ double frcp(double x) {
 __device__ __forceinline__ double vanleer(double a, double b) {
   return (a * b > 0.0) ? (2.0 * a * b) * frcp(a + b) : 0.0;
 }
+// mid + vanleer(hi - mid, mid - lo) * fac where use is set, else mid, with
+// the quotient formed on every lane and then selected (no branch around the
+// reciprocal; where a*b <= 0 or use is clear the quotient, perhaps inf or
+// NaN, is dropped): the same operations as the branch of hydro.hpp:16-18.
+__device__ __forceinline__ double slope_add(double mid, double lo, double hi, bool use,
+                                            double fac) {
+  const double a = hi - mid, b = mid - lo;
+  const double q = (2.0 * a * b) * frcp(a + b);
+  const double vl = (use && a * b > 0.0) ? q : 0.0;
+  return mid + vl * fac;
+}
 // Half the van Leer slope of a cell, vanleer(hi - mid, mid - lo) / 2, with
 // the quotient formed on every lane and then selected, so that no branch
 // surrounds the reciprocal (where a*b <= 0 the quotient may be inf or NaN;
@@ -711,10 +722,12 @@ __device__ __forceinline__ double *stpw(double *st, i64 ntot, int buf, int f, co
 // previous update (or the initial diag) wrote for the state in buffer `par`;
 // stop at t >= tlim or at the step cap; record an output sample once t has
 // reached tnext; dt = cfl / max, clipped to land on tlim; t += dt.  It
-// touches only the control block and the sample ring.
-__device__ void control_disk(const DiskDev &D, DiskCtl &C, int d, const double *__restrict__ pcfl,
-                             const double *__restrict__ pcm, DfkSampleDev *__restrict__ samples,
-                             int cap, int par) {
+// touches only the control block and the sample ring.  c: the block as read
+// at the start (its loads in flight with the partials'); C: where thread 0
+// writes the changes.  A disk already done reduces and changes nothing.
+__device__ void control_disk(const DiskDev &D, const DiskCtl &c, DiskCtl &C, int d,
+                             const double *__restrict__ pcfl, const double *__restrict__ pcm,
+                             DfkSampleDev *__restrict__ samples, int cap, int par) {
   __shared__ double red[5 * 16];
   const int nt = D.ntile_i * D.ntile_j, B = blockDim.x;
   double mx = 1e-30, s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -725,29 +738,29 @@ __device__ void control_disk(const DiskDev &D, DiskCtl &C, int d, const double *
   constexpr int KF = 8;
   for (int t0 = threadIdx.x; t0 < nt; t0 += KF * B) {
     double v[KF];
-    double4 c[KF];
+    double4 c4[KF];
 #pragma unroll
     for (int q = 0; q < KF; ++q) {
       const i64 t = D.part + min(t0 + q * B, nt - 1);
       v[q] = pcfl[t];
-      c[q] = *reinterpret_cast<const double4 *>(pcm + 4 * t);
+      c4[q] = *reinterpret_cast<const double4 *>(pcm + 4 * t);
     }
 #pragma unroll
     for (int q = 0; q < KF; ++q) {
       if (t0 + q * B < nt) {
         mx = (v[q] > mx) ? v[q] : mx;
-        s0 += c[q].x;
-        s1 += c[q].y;
-        s2 += c[q].z;
-        s3 += c[q].w;
+        s0 += c4[q].x;
+        s1 += c4[q].y;
+        s2 += c4[q].z;
+        s3 += c4[q].w;
       }
     }
   }
   wg_reduce5(mx, s0, s1, s2, s3, red);
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || c.done) return;
   if (s3 > 0.0) {
-    C.flags |= DFK_FLAG_NONFINITE;
-    if (C.stop_on_nonfinite) {
+    C.flags = c.flags | DFK_FLAG_NONFINITE;
+    if (c.stop_on_nonfinite) {
       C.done = 1;
       C.cur = par;
       C.fin_re2 = s1;
@@ -755,26 +768,26 @@ __device__ void control_disk(const DiskDev &D, DiskCtl &C, int d, const double *
       return;
     }
   }
-  const double t = C.t;
-  if (!(t < C.tlim) || (C.max_steps > 0 && C.step >= C.max_steps)) {
+  const double t = c.t;
+  if (!(t < c.tlim) || (c.max_steps > 0 && c.step >= c.max_steps)) {
     C.done = 1;
     C.cur = par;
     C.fin_re2 = s1;
     C.fin_im2 = s2;
     return;
   }
-  if (t >= C.tnext - 1e-12) {
+  if (t >= c.tnext - 1e-12) {
     DfkSampleDev smp;
     smp.t = t; smp.re0 = s0; smp.re2 = s1; smp.im2 = s2;
-    samples[(i64)d * cap + (C.nsamples % cap)] = smp;
-    C.nsamples += 1;
-    C.tnext += C.dtout;
+    samples[(i64)d * cap + (c.nsamples % cap)] = smp;
+    C.nsamples = c.nsamples + 1;
+    C.tnext = c.tnext + c.dtout;
   }
-  double dt = C.cfl / mx;
-  if (t + dt > C.tlim) dt = C.tlim - t;
+  double dt = c.cfl / mx;
+  if (t + dt > c.tlim) dt = c.tlim - t;
   C.dt = dt;
   C.t = t + dt;
-  C.step += 1;
+  C.step = c.step + 1;
   C.cur = 1 - par;   // where this iteration's update writes
 }
 
@@ -792,7 +805,8 @@ __device__ __forceinline__ void trailing_control(const DiskDev *__restrict__ dis
                                                  const double *__restrict__ pcm,
                                                  DfkSampleDev *__restrict__ samples, int cap, int par,
                                                  DiskCtl *__restrict__ mirror) {
-  if (!ctl[d].done) control_disk(disks[d], ctl[d], d, pcfl, pcm, samples, cap, par);
+  const DiskCtl c = ctl[d];
+  control_disk(disks[d], c, ctl[d], d, pcfl, pcm, samples, cap, par);
   // thread 0 made every change to the block (the others left control_disk
   // after the reduction)
   if (mirror && threadIdx.x == 0) {
@@ -1738,20 +1752,22 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
   // record lr is row i0-2+lr, clamped into the records 0 .. Nu) and cos 2phi,
   // sin 2phi of the tile's columns (trg), staged with phase 1's loads so that
   // the later phases read their row tables from LDS, not from the caches
+  // The first staged value of each thread is loaded here and stored after
+  // phase 1's loads, so that the two memory latencies overlap; any further
+  // ones (tiles with more table entries than threads) after phase 1.
   double *rt = fW + (TI + 1) * SW, *trg = rt + SH * RR_STRIDE;
-  {
-    const double *rec0 = gtab(geom, D, GT_COUNT);
-    for (int e = threadIdx.x; e < SH * RR_STRIDE + 2 * TJ; e += blockDim.x) {
-      if (e < SH * RR_STRIDE) {
-        const int i = min(max(i0 - 2 + e / RR_STRIDE, 0), Nu);
-        rt[e] = rec0[i * RR_STRIDE + (e % RR_STRIDE)];
-      } else {
-        const int q = e - SH * RR_STRIDE, sn = q >= TJ ? 1 : 0;
-        const int j = min(j0 + q - sn * TJ, Nphi - 1);
-        trg[q] = trig[D.cos_off + sn * Nphi + j];
-      }
+  const int NST = SH * RR_STRIDE + 2 * TJ;
+  const double *rec0 = gtab(geom, D, GT_COUNT);
+  auto stage_ld = [&](int e) {   // entry e of rt/trg (trg = rt + SH * RR_STRIDE)
+    if (e < SH * RR_STRIDE) {
+      const int i = min(max(i0 - 2 + e / RR_STRIDE, 0), Nu);
+      return rec0[i * RR_STRIDE + (e % RR_STRIDE)];
     }
-  }
+    const int q = e - SH * RR_STRIDE, sn = q >= TJ ? 1 : 0;
+    const int j = min(j0 + q - sn * TJ, Nphi - 1);
+    return trig[D.cos_off + sn * Nphi + j];
+  };
+  const double st0 = stage_ld(min((int)threadIdx.x, NST - 1));
   auto rrow = [&](int lr, int fld) { return rt[lr * RR_STRIDE + fld]; };   // row i0-2+lr
   const double *tInvdRc = gtab(geom, D, GT_INVDRC), *tInvR = gtab(geom, D, GT_INVR),
                *tGstar = gtab(geom, D, GT_GSTAR), *tGrc = gtab(geom, D, GT_GRC),
@@ -1770,6 +1786,7 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
     const double s = ld(0, i, j), u = ld(1, i, j), w = ld(2, i, j);
     const double dPdR = (pres<G2>(D, ld(0, ip, j)) - pres<G2>(D, ld(0, im, j))) * tInvdRc[i];
     const double dPdphi = (pres<G2>(D, ld(0, i, jp)) - pres<G2>(D, ld(0, i, jm))) * D.inv2dphi;
+    const double invR = tInvR[i];   // loaded ahead of the gravity branch, whose end waits
     double gR = tGstar[i];
     double gp = 0.0;
     if (GARR) {
@@ -1782,44 +1799,40 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
       gp += -(phi(i, jp) - phi(i, jm)) * tGpc[i];
     }
     const double rs = frcp(s);
-    const double invR = tInvR[i];
     const double aR = -dPdR * rs + gR + w * w * invR;
     const double ap = -(rs * invR) * dPdphi + gp - u * w * invR;
     sS[e] = s;
     sU[e] = u + dt * aR;
     sW[e] = w + dt * ap;
   }
+  if ((int)threadIdx.x < NST) rt[threadIdx.x] = st0;
+  for (int e = threadIdx.x + blockDim.x; e < NST; e += blockDim.x) rt[e] = stage_ld(e);
   __syncthreads();
 
-  // ---- phase 2: radial faces f = i0..i0+TI (between cells f-1 and f)
+  // ---- phase 2: radial faces f = i0..i0+TI (between cells f-1 and f).
+  // Branch-free: both upwind choices are formed and one selected, and the
+  // three slopes are formed on every lane and dropped where the reference
+  // skips them (a face outside 1..Nu-1, a cell at the radial edge, a*b <= 0),
+  // so the LDS reads of all three are issued together.  Every address read
+  // lies inside the staged rows; values read for dropped terms are unused.
   for (int e = threadIdx.x; e < (TI + 1) * SW; e += blockDim.x) {
     const int lf = qdiv(e, rcpf((float)SW)), lc = e - lf * SW;
     const int f = i0 + lf;
-    double fm = 0.0, fu = 0.0, fw = 0.0;
-    if (f >= 1 && f <= Nu - 1) {
-      const int la = (lf + 1) * SW + lc;   // cell f-1
-      const int lb = la + SW;              // cell f
-      const double vf = 0.5 * (sU[la] + sU[lb]);
-      int lcell, cell;
-      double fac;
-      if (vf > 0) {
-        lcell = la; cell = f - 1;
-        fac = 0.5 * (1.0 - vf * dt * rrow(lf + 1, RR_INVDR));
-      } else {
-        lcell = lb; cell = f;
-        fac = -0.5 * (1.0 + vf * dt * rrow(lf + 2, RR_INVDR));
-      }
-      double ss = sS[lcell], su = sU[lcell], sw = sW[lcell];
-      if (cell > 0 && cell < Nu - 1) {
-        ss += vanleer(sS[lcell + SW] - sS[lcell], sS[lcell] - sS[lcell - SW]) * fac;
-        su += vanleer(sU[lcell + SW] - sU[lcell], sU[lcell] - sU[lcell - SW]) * fac;
-        sw += vanleer(sW[lcell + SW] - sW[lcell], sW[lcell] - sW[lcell - SW]) * fac;
-      }
-      fm = vf * ss * rrow(lf + 2, RR_AFZ);   // the face area (f interior here)
-      fu = fm * su;
-      fw = fm * sw;
-    }
-    fM[e] = fm; fU[e] = fu; fW[e] = fw;
+    const int la = (lf + 1) * SW + lc;   // cell f-1
+    const int lb = la + SW;              // cell f
+    const double vf = 0.5 * (sU[la] + sU[lb]);
+    const bool up = vf > 0;
+    const double fa = 0.5 * (1.0 - vf * dt * rrow(lf + 1, RR_INVDR));
+    const double fb = -0.5 * (1.0 + vf * dt * rrow(lf + 2, RR_INVDR));
+    const int lcell = up ? la : lb, cell = up ? f - 1 : f;
+    const double fac = up ? fa : fb;
+    const bool lim = cell > 0 && cell < Nu - 1;
+    const double ss = slope_add(sS[lcell], sS[lcell - SW], sS[lcell + SW], lim, fac);
+    const double su = slope_add(sU[lcell], sU[lcell - SW], sU[lcell + SW], lim, fac);
+    const double sw = slope_add(sW[lcell], sW[lcell - SW], sW[lcell + SW], lim, fac);
+    const bool face = f >= 1 && f <= Nu - 1;
+    const double fm = face ? vf * ss * rrow(lf + 2, RR_AFZ) : 0.0;   // the face area (f interior)
+    fM[e] = fm; fU[e] = face ? fm * su : 0.0; fW[e] = face ? fm * sw : 0.0;
   }
   __syncthreads();
 
@@ -1840,7 +1853,8 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
   }
   __syncthreads();
 
-  // ---- phase 4: azimuthal faces j = j0..j0+TJ (between j-1 and j), rows i0..i0+TI-1
+  // ---- phase 4: azimuthal faces j = j0..j0+TJ (between j-1 and j), rows
+  // i0..i0+TI-1, branch-free as phase 2
   const int FW = TJ + 1;
   for (int e = threadIdx.x; e < TI * FW; e += blockDim.x) {
     const int lr = qdiv(e, rcpf((float)FW)), lf = e - lr * FW;
@@ -1850,19 +1864,15 @@ __device__ __forceinline__ void update_tile_body(const DiskDev &D, double dt, in
       const int la = (lr + 2) * SW + lf + 1;   // cell j-1
       const int lb = la + 1;                   // cell j
       const double vf = 0.5 * (sW[la] + sW[lb]);
-      int lcell;
-      double fac;
+      const bool up = vf > 0;
       const double idl = rrow(lr + 2, RR_INVDL);
-      if (vf > 0) {
-        lcell = la;
-        fac = 0.5 * (1.0 - vf * dt * idl);
-      } else {
-        lcell = lb;
-        fac = -0.5 * (1.0 + vf * dt * idl);
-      }
-      const double ss = sS[lcell] + vanleer(sS[lcell + 1] - sS[lcell], sS[lcell] - sS[lcell - 1]) * fac;
-      const double su = sU[lcell] + vanleer(sU[lcell + 1] - sU[lcell], sU[lcell] - sU[lcell - 1]) * fac;
-      const double sw = sW[lcell] + vanleer(sW[lcell + 1] - sW[lcell], sW[lcell] - sW[lcell - 1]) * fac;
+      const double fa = 0.5 * (1.0 - vf * dt * idl);
+      const double fb = -0.5 * (1.0 + vf * dt * idl);
+      const int lcell = up ? la : lb;
+      const double fac = up ? fa : fb;
+      const double ss = slope_add(sS[lcell], sS[lcell - 1], sS[lcell + 1], true, fac);
+      const double su = slope_add(sU[lcell], sU[lcell - 1], sU[lcell + 1], true, fac);
+      const double sw = slope_add(sW[lcell], sW[lcell - 1], sW[lcell + 1], true, fac);
       fm = vf * ss * rrow(lr + 2, RR_AFACE);
       fu = fm * su;
       fw = fm * sw;

@@ -398,11 +398,11 @@ RF_HD int lane_k1(int l, int g = 0) { return (l >> 3) + 8 * g < P ? (l >> 3) + 8
 // pass A, lane m: ld(row) returns row `row` (< Nu) of the column
 template <int P, class Ld>
 RF_HD void passA(int m, Ld ld, const double2 *tw, double2 (&v)[P]) {
+  const conv::TwRow<P, -1> w(tw, m);   // issued before the data loads
   sfor<0, P / 2>([&](auto n1_) {
     constexpr int n1 = decltype(n1_)::value;
     v[n1] = ld(64 * n1 + m);
   });
-  const conv::TwRow<P, -1> w(tw, m);
   dft_half_in<P, -1>(v);
   sfor<1, P>([&](auto k1_) {
     constexpr int k1 = decltype(k1_)::value;
@@ -555,11 +555,11 @@ namespace rows3 {
 // pass A, lane m: ld(j) returns x[j], j = 64 n1 + m
 template <int P, int SIGN, class Ld>
 RF_HD void passA(int m, Ld ld, const double2 *tw, double2 (&v)[P]) {
+  const conv::TwRow<P, SIGN> w(tw, m);   // issued before the data loads
   sfor<0, P>([&](auto n1_) {
     constexpr int n1 = decltype(n1_)::value;
     v[n1] = ld(64 * n1 + m);
   });
-  const conv::TwRow<P, SIGN> w(tw, m);
   dft<P, SIGN>(v);
   sfor<1, P>([&](auto k1_) {
     constexpr int k1 = decltype(k1_)::value;
@@ -691,7 +691,11 @@ RF_HD double2 c2r_in(int m, Bin bin) {
   const int k = lo ? m : n - m;
   double2 xa, xb;
   bin(k, xa, xb);
-  if (k == 0 || 2 * k == n) { xa.y = 0.0; xb.y = 0.0; }
+  // a select, not a branch: a branch writing the registers of loads still in
+  // flight makes the wavefront wait for them before its later loads issue
+  const bool real = k == 0 || 2 * k == n;
+  xa.y = real ? 0.0 : xa.y;
+  xb.y = real ? 0.0 : xb.y;
   return lo ? make_double2(xa.x - xb.y, xa.y + xb.x) : make_double2(xa.x + xb.y, -xa.y + xb.x);
 }
 
